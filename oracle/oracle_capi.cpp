@@ -1,0 +1,149 @@
+// CPU ORACLE C API -- test infrastructure only.  Loaded by tests/ (ctypes),
+// __graft_entry__.smoke() and bench.py's CPU-baseline leg.
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "oracle.hpp"
+#include "../veritas_amd/csrc/vx_math.h"
+
+namespace {
+struct Run {
+    vx::Scenario scn;
+    std::unique_ptr<orc::Solver> s;
+};
+
+orc::Patch &base(Run *r) { return *r->s->meshes[0]->levels.back()->rectangles[0]; }
+
+int copy_out(const std::vector<double> &v, double *out, int n) {
+    int m = (int)v.size();
+    if (out && n >= m) std::memcpy(out, v.data(), sizeof(double) * m);
+    return m;
+}
+
+// interior of a patch plane, row-major (i over n_x, j over n_p)
+int copy_plane(const orc::Patch &p, const std::vector<double> &v, int ghosts, double *out, int n) {
+    int g = ghosts;
+    int nx = p.n_x + 2 * g, np = p.n_p + 2 * g;
+    if (out && n >= nx * np)
+        for (int i = 0; i < nx; i++)
+            for (int j = 0; j < np; j++) out[i * np + j] = v[p.NS(i - g, j - g)];
+    return nx * np;
+}
+}  // namespace
+
+extern "C" {
+
+void *vxo_create(const vx_scenario_params *prm) {
+    Run *r = new Run;
+    r->scn = vx::Scenario::from_params(*prm);
+    r->s.reset(new orc::Solver(r->scn));
+    return r;
+}
+
+void vxo_destroy(void *h) { delete (Run *)h; }
+
+double vxo_calc_dt(void *h, double cfl) { return ((Run *)h)->s->CalculateDt(cfl); }
+
+void vxo_advance(void *h, double dt, int nsteps) {
+    for (int i = 0; i < nsteps; i++) ((Run *)h)->s->Advance(dt);
+}
+
+void vxo_advance_fields(void *h, double dt, int nsteps) {
+    for (int i = 0; i < nsteps; i++) ((Run *)h)->s->AdvanceFields(dt);
+}
+
+// One RK stage of SolverManager::Advance (SolverManager.cpp:29-38), and the
+// pieces of it, for stage-by-stage parity checks.
+void vxo_stage_moments(void *h) { ((Run *)h)->s->em->AssembleRhoAndJ(); }
+void vxo_stage_poisson(void *h) { ((Run *)h)->s->em->UpdatePotential(); }
+void vxo_stage_vlasov(void *h, double dt, int step) {
+    for (auto &m : ((Run *)h)->s->meshes) m->Advance(dt, step);
+}
+void vxo_stage_vlasov_sub(void *h, double dt, int step, int sub) {
+    Run *r = (Run *)h;
+    for (auto &m : r->s->meshes) {
+        for (int i = (int)m->levels.size() - 1; i > -1; i--) m->levels[i]->FCTTimeStep(dt, step, sub);
+        if (sub == 0) m->PushData(2);
+        if (sub == 1) m->PushBoundaryC();
+        if (sub == 2) m->PushData(1);
+    }
+}
+void vxo_stage_fields(void *h, double dt, int step) {
+    Run *r = (Run *)h;
+    r->s->settings.UpdateTime(step, dt);
+    r->s->em->RGKStep(step, dt);
+}
+
+// Named arrays.  Patch planes (base patch of species 0) are returned with
+// `ghosts` ghost layers (0..2), row-major over (x, p).
+int vxo_get(void *h, const char *what, int ghosts, double *out, int n) {
+    Run *r = (Run *)h;
+    orc::Field &em = *r->s->em;
+    orc::Patch &p = base(r);
+    std::string w(what);
+    if (w == "f0") return copy_plane(p, p.f[0], ghosts, out, n);
+    if (w == "f1") return copy_plane(p, p.f[1], ghosts, out, n);
+    if (w == "f2") return copy_plane(p, p.f[2], ghosts, out, n);
+    if (w == "FxDS") return copy_plane(p, p.FxDS, ghosts, out, n);
+    if (w == "FpDS") return copy_plane(p, p.FpDS, ghosts, out, n);
+    if (w == "Rp") return copy_plane(p, p.Rp, ghosts, out, n);
+    if (w == "Rm") return copy_plane(p, p.Rm, ghosts, out, n);
+    if (w == "Cx") return copy_plane(p, p.Cx, ghosts, out, n);
+    if (w == "Cp") return copy_plane(p, p.Cp, ghosts, out, n);
+    for (int k = 0; k < 6; k++) {
+        if (w == "FxH" + std::to_string(k)) return copy_plane(p, p.FxH[k], ghosts, out, n);
+        if (w == "FpH" + std::to_string(k)) return copy_plane(p, p.FpH[k], ghosts, out, n);
+        if (w == "FxL" + std::to_string(k)) return copy_plane(p, p.FxL[k], ghosts, out, n);
+        if (w == "FpL" + std::to_string(k)) return copy_plane(p, p.FpL[k], ghosts, out, n);
+    }
+    if (w == "phi") return copy_out(em.PHI, out, n);
+    if (w == "charge") return copy_out(em.charge, out, n);
+    if (w == "J") return copy_out(em.J, out, n);
+    if (w == "neutral") return copy_out(em.neutralizationCharge, out, n);
+    if (w == "a2") return copy_out(em.a_squared, out, n);
+    if (w == "By") return copy_out(em.By, out, n);
+    if (w == "Bz") return copy_out(em.Bz, out, n);
+    if (w == "Ey") return copy_out(em.Ey, out, n);
+    if (w == "Ez") return copy_out(em.Ez, out, n);
+    if (w == "Ay") return copy_out(em.Ay, out, n);
+    if (w == "Az") return copy_out(em.Az, out, n);
+    if (w == "E") {
+        std::vector<double> e(em.x_size);
+        for (unsigned i = 0; i < em.x_size; i++) e[i] = em.GetEfield(i);
+        return copy_out(e, out, n);
+    }
+    if (w == "dNdP") {
+        em.AssembleEnergy();
+        return copy_out(em.energies[0], out, n);
+    }
+    if (w == "scalars") {  // Ex0, time, dx, dp, pmin, mass, charge, n_prepad
+        std::vector<double> s = {em.Ex0, r->s->settings.time, p.dx, p.dp, r->s->settings.pmin[0],
+                                 r->s->settings.m[0], r->s->settings.q[0], (double)em.n_prepad};
+        return copy_out(s, out, n);
+    }
+    return -1;
+}
+
+// Standalone pieces for unit tests.
+double vxo_weno_nomax(double f1, double f2, double f3, double f4, int right) {
+    return orc::weno_nomax(f1, f2, f3, f4, right != 0);
+}
+double vxo_face_weno(double f1, double f2, double f3, double f4) {
+    return orc::face_weno(f1, f2, f3, f4);
+}
+void vxo_poisson(void *h, const double *b, double *x) {
+    Run *r = (Run *)h;
+    std::vector<double> bv(b, b + r->s->em->x_size), xv;
+    r->s->em->PoissonSolve(bv, xv);
+    std::memcpy(x, xv.data(), sizeof(double) * xv.size());
+}
+void vxo_poisson_approx(void *h, const double *b, double *x) {
+    Run *r = (Run *)h;
+    std::vector<double> bv(b, b + r->s->em->x_size), xv;
+    r->s->em->PoissonApprox(bv, xv);
+    std::memcpy(x, xv.data(), sizeof(double) * xv.size());
+}
+double vxo_log(double x) { return vx_log(x); }
+double vxo_k1e(double z) { return vx::Scenario::k1e(z); }
+}

@@ -1,0 +1,81 @@
+"""Run descriptions shared by the Python bindings, the tests and bench.py.
+
+`ScenarioParams` mirrors the C struct `vx_scenario_params`
+(veritas_amd/csrc/vx_params.h): the reference's Input/Particles settings
+(Settings.hpp:4-31) plus the scenario values that the reference leaves to user
+hooks (veritas.cpp:7-115).  Values are normalised (c/omega_p, m_e c, m_e c^2).
+"""
+import ctypes
+import math
+
+
+class ScenarioParams(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("nx", ctypes.c_int32),
+        ("np", ctypes.c_int32),
+        ("levels", ctypes.c_int32),
+        ("r", ctypes.c_int32),
+        ("quadrature_depth", ctypes.c_int32),
+        ("modes", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("seed", ctypes.c_uint64),
+        ("cfl", ctypes.c_double),
+        ("min_efficiency", ctypes.c_double),
+        ("refinement_criteria", ctypes.c_double),
+        ("size_weight", ctypes.c_double),
+        ("pre_length", ctypes.c_double),
+        ("post_length", ctypes.c_double),
+        ("n_ref", ctypes.c_double),
+        ("length", ctypes.c_double),
+        ("pmin", ctypes.c_double),
+        ("pmax", ctypes.c_double),
+        ("theta", ctypes.c_double),
+        ("amplitude", ctypes.c_double),
+        ("k", ctypes.c_double),
+        ("drift", ctypes.c_double),
+        ("a0", ctypes.c_double),
+        ("n_over_nc", ctypes.c_double),
+        ("ramp", ctypes.c_double),
+        ("fwhm_cycles", ctypes.c_double),
+        ("lambda0", ctypes.c_double),
+    ]
+
+    def replace(self, **kw):
+        out = ScenarioParams()
+        ctypes.memmove(ctypes.byref(out), ctypes.byref(self), ctypes.sizeof(self))
+        for k, v in kw.items():
+            setattr(out, k, v)
+        return out
+
+
+LANDAU, TWO_STREAM, LASER = 0, 1, 2
+
+
+def _base(**kw):
+    p = ScenarioParams(kind=LANDAU, nx=64, np=256, levels=1, r=2, quadrature_depth=1,
+                       modes=8, seed=42, cfl=0.4, min_efficiency=0.6,
+                       refinement_criteria=1e-8, size_weight=0.0, pre_length=0.0,
+                       post_length=0.0, n_ref=1.0e24, length=math.pi / 2, pmin=-1.0,
+                       pmax=1.0, theta=0.01, amplitude=0.01, k=4.0, drift=0.2, a0=0.5,
+                       n_over_nc=0.01, ramp=50.0, fwhm_cycles=10.0, lambda0=1.0e-6)
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def landau(**kw):
+    """Relativistic 1D1P Landau damping: x in [0, pi/2) with Nx=64, p in [-1, 1]
+    with Np=256, f = (1 + 0.01 cos 4x) MJ(theta=0.01), one electron species
+    neutralised by immobile ions (SolverManager.cpp:25)."""
+    return _base(**kw)
+
+
+def two_stream(**kw):
+    """Relativistic two-stream: two MJ beams (theta=1e-3) drifting at p=+-0.2,
+    density 0.5 each, perturbed by 1e-3 * sum_{m=1..8} cos(m x/16 + phi_m) with
+    seeded phases; x length 32 pi with Nx=4096, p in [-0.6, 0.6] with Np=4096."""
+    d = dict(kind=TWO_STREAM, nx=4096, np=4096, length=32 * math.pi, pmin=-0.6, pmax=0.6,
+             theta=1e-3, amplitude=1e-3, drift=0.2, modes=8)
+    d.update(kw)
+    return _base(**d)
