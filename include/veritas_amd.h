@@ -1,0 +1,170 @@
+/* veritas_amd: MI355X (gfx950) kernels for the Veritas Vlasov-Maxwell time
+ * advance, behind a plain C ABI.
+ *
+ * The reference (Libbum/Veritas) has no FFI: its boundary is the C++ class API
+ * SolverManager / Mesh / Level / Rectangle / EMFieldSolver (SolverManager.hpp,
+ * Mesh.hpp, Level.hpp, Rectangle.hpp, EMSolver.hpp).  The C++ host library in
+ * veritas_amd/csrc/host keeps those classes and calls the entry points below,
+ * each of which replaces one reference routine (cited per function).
+ *
+ * Conventions: every call is stream-ordered on the context's HIP stream and
+ * returns 0 on success or a negative VX_E* code; vx_last_error() gives the
+ * message.  Pointers inside the descriptors are device pointers owned by the
+ * caller (allocate them with vx_malloc).  Planes are FP64, laid out row-major
+ * over x with p contiguous: element (i, j), i in [-2, n_x + 2), j in
+ * [-2, n_p + 2), lives at (i + 2) * pitch + (j + 2); pitch is a multiple of 8.
+ */
+#ifndef VERITAS_AMD_H
+#define VERITAS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "../veritas_amd/csrc/vx_params.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VX_OK 0
+#define VX_EINVAL -1   /* bad argument or shape */
+#define VX_EHIP -2     /* HIP runtime error */
+#define VX_ENOMEM -3   /* device allocation failed */
+#define VX_ENOTSUP -4  /* configuration outside what the kernels handle */
+
+typedef struct vx_ctx vx_ctx;
+
+/* ---- context (no reference counterpart: the reference runs in-process) ---- */
+int vx_ctx_create(int device, vx_ctx **out);
+void vx_ctx_destroy(vx_ctx *ctx);
+const char *vx_last_error(vx_ctx *ctx);
+int vx_sync(vx_ctx *ctx);
+void *vx_stream(vx_ctx *ctx); /* the hipStream_t all calls are ordered on */
+int vx_malloc(vx_ctx *ctx, size_t bytes, void **dptr); /* zero-filled */
+int vx_free(vx_ctx *ctx, void *dptr);
+int vx_memcpy_h2d(vx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int vx_memcpy_d2h(vx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int vx_memcpy_d2d(vx_ctx *ctx, void *dst, const void *src, size_t bytes);
+int vx_fill(vx_ctx *ctx, double *dst, double value, size_t count);
+
+/* ---- one AMR patch (Rectangle, Rectangle.hpp:6-80) on the device ---- */
+typedef struct vx_patch {
+    int32_t n_x, n_p, x_pos, p_pos; /* Rectangle::n_x, n_p, x_pos, p_pos */
+    int32_t rtb;                    /* Rectangle::relativeToBottom */
+    int32_t pitch;                  /* doubles per x-row of every plane */
+    int32_t left, right, down, up;  /* patch touches the domain edge (Rectangle.cpp:1257-1260) */
+    double dx, dp, pmin, q, m;      /* cell sizes, Settings::pmin, charge, mass (SI) */
+    /* derived exactly as the reference evaluates them (Rectangle.cpp:1271-1275,
+     * Rectangle.hpp:181-183): 1/dx, 1/dp, cs*cs*m, (m_inv*c_inv)^2 */
+    double dx_inv, dp_inv, cc, mc2inv;
+    double *f[3];           /* Rectangle::f slots: 0 = f^n, 1 = current stage, 2 = LO predictor */
+    double *FxH[6], *FpH[6]; /* high-order fluxes of each RK stage (Rectangle::FxH/FpH) */
+    double *FxL0, *FpL0;    /* low-order fluxes of stage 0 (the only ones used, F8) */
+    double *FxDS, *FpDS;    /* antidiffusive fluxes of the current stage */
+    double *Rp, *Rm, *Cx, *Cp; /* Zalesak limiter state */
+    uint8_t *nested;        /* per cell (plane layout, 1 byte): covered by a finer patch */
+    double *chargeR, *currentR; /* n_x * rtb each */
+} vx_patch;
+
+/* 1-D finest-grid field data the patches read during a stage. */
+typedef struct vx_fields {
+    int32_t n_f;       /* Settings::x_size_finest */
+    int32_t n_prepad;  /* EMFieldSolver::n_prepad */
+    int32_t n_e;       /* n_f + n_prepad + n_postpad */
+    int32_t pad;
+    const double *E;   /* EMFieldSolver::GetEfield(i), i in [0, n_f) (periodic) */
+    const double *a2;  /* EMFieldSolver::a_squared, n_f + 1 faces */
+    const double *Ay1, *Az1; /* transverse potentials, RK slot 1, n_e cells */
+} vx_fields;
+
+/* Plane geometry helper: pitch (doubles) for a patch with n_p cells. */
+int32_t vx_plane_pitch(int32_t n_p);
+size_t vx_plane_elems(int32_t n_x, int32_t n_p);
+
+/* FCT sub-step 0 (Rectangle::FCTTimeStep subStep 0, Rectangle.cpp:1269-1535):
+ * WENO face values, high- and low-order fluxes of RK stage `step`, the RK flux
+ * combination (FxDS/FpDS) and the low-order predictor f[2]. */
+int vx_fct_sub0(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl);
+/* FCT sub-step 1: Zalesak R+-, then C on interior faces (Rectangle.cpp:1536-1594). */
+int vx_fct_sub1(vx_ctx *ctx, const vx_patch *p);
+/* FCT sub-step 2: f[1] = f[2] + limited antidiffusive fluxes (Rectangle.cpp:1595-1613). */
+int vx_fct_sub2(vx_ctx *ctx, const vx_patch *p);
+/* FCT sub-step 3: f[0] <- f[1] including ghosts (Rectangle.cpp:1614-1622). */
+int vx_fct_sub3(vx_ctx *ctx, const vx_patch *p);
+
+/* Whole RK stage of one patch whose four edges are domain walls (a single
+ * unrefined level): sub-steps 0-2 with the zero ghost fills in between, fused
+ * (Mesh::Advance, Mesh.cpp:64-89, for one patch).  Equivalent to
+ * sub0 + ghost(2) + sub1 + sub2 + ghost(1) [+ sub3 at step 5]. */
+int vx_fct_stage_walls(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl);
+
+/* Ghost fill of slot `slot` of a patch whose neighbours are all the domain-edge
+ * sentinel (BoundaryCondition::GetValueFromSameLevel returns 0,
+ * BoundaryCondition.cpp:6-8): Rectangle::UpdateSameLevelBoundaries +
+ * UpdateCornerPoints (Rectangle.cpp:562-614, 1130-1214) for that case. */
+int vx_ghost_zero(vx_ctx *ctx, const vx_patch *p, int slot);
+
+/* p-moments of one patch (Rectangle::CalculateRhoAndJ, Rectangle.cpp:157-282):
+ * chargeR and currentR, summed over p in the reference's order. */
+int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl);
+/* Level/Mesh/EMFieldSolver assembly of per-patch moments onto the finest grid
+ * (Level.cpp:42-62, Mesh.cpp:52-56, EMSolver.cpp:104-122), patches given in the
+ * reference's order (species, then levels finest->coarsest, then rectangles).
+ * level_start[s * n_levels + l] indexes the first patch of level l of species s
+ * in `patches`.  A plan holds the device copy of that table; build it after
+ * every change of the hierarchy. */
+typedef struct vx_moment_layout {
+    int32_t n_species;
+    int32_t n_levels;            /* per species */
+    const int32_t *level_start;  /* host array, n_species * n_levels + 1 entries */
+} vx_moment_layout;
+typedef struct vx_moment_plan vx_moment_plan;
+int vx_moment_plan_create(vx_ctx *ctx, const vx_patch *patches, const vx_moment_layout *lay,
+                          int32_t n_f, vx_moment_plan **out);
+void vx_moment_plan_destroy(vx_moment_plan *plan);
+/* charge, J: n_f each; charges_species (optional): n_species * n_f */
+int vx_moments_assemble(vx_ctx *ctx, const vx_moment_plan *plan, double *charge, double *J,
+                        double *charges_species);
+
+/* ---- 1-D field solver (EMFieldSolver, EMSolver.hpp:9-49) ---- */
+typedef struct vx_field_state {
+    int32_t n_f, n_prepad, n_postpad, n_e;
+    double dx_f;      /* finest dx */
+    double dx2;       /* pow(dx_f, 2.0) as the host evaluates it (EMSolver.cpp:163) */
+    double field_coef; /* 1 / (12 dx_f) (EMSolver.cpp:88) */
+    double *charge, *J, *neutral; /* n_f each */
+    double *phi;      /* n_f */
+    double *E;        /* n_f: GetEfield(i) after the last Poisson solve */
+    double *Ex0;      /* 1 scalar on the device */
+    double *By, *Bz, *Ey, *Ez, *Ay, *Az; /* 8 RK slots * n_e each, slot-major */
+    double *a2;       /* n_f + 1 */
+    double *work;     /* scratch, vx_field_work_elems(n_f) doubles */
+} vx_field_state;
+size_t vx_field_work_elems(int32_t n_f);
+
+/* EMFieldSolver::UpdatePotential (EMSolver.cpp:156-192): right-hand side,
+ * Poisson solve of the reference's matrix (EMSolver.cpp:28-88) to near correct
+ * rounding, E on the finest grid and the Ex0 update. */
+int vx_poisson(vx_ctx *ctx, const vx_field_state *fs);
+/* EMFieldSolver::RGKStep (EMSolver.cpp:194-202): transverse-field RHS, RK
+ * update and WENO A^2 at faces.  bz0/by0 are the GetBZ/GetBY hook values at
+ * x = 0 for this stage's time (veritas.cpp:80-104). */
+int vx_maxwell_stage(vx_ctx *ctx, const vx_field_state *fs, int step, double dt, double by0,
+                     double bz0);
+/* EMFieldSolver::EnforceChargeNeutralization tail (EMSolver.cpp:624-628):
+ * J = 0, neutral = -charge. */
+int vx_neutralize(vx_ctx *ctx, const vx_field_state *fs);
+/* EMFieldSolver::EstimateCFLBound (EMSolver.cpp:631-664) for n_species species
+ * (host arrays q, m, dp_finest); result written to *out on the host. */
+int vx_cfl_bound(vx_ctx *ctx, const vx_field_state *fs, int n_species, const double *q,
+                 const double *m, const double *dp_finest, double *out);
+
+/* ---- timing helpers (HIP events on the context stream) ---- */
+int vx_event_record(vx_ctx *ctx, int slot);
+int vx_event_elapsed_ms(vx_ctx *ctx, int slot_a, int slot_b, float *ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -331,15 +331,18 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
         double dx_inv = 1 / dx;
         double dp_inv = 1 / dp;
         double cc = vx_cs * vx_cs * settings->m.at(sp);
+#pragma omp parallel for schedule(static)
         for (int i = -1; i < nx2; i++) {  // 1278-1286
             double as = q * q * em->GetASquared(GetFinestIndex(i));
             for (int j = -1; j < np1; j++)
                 ex[NS(i, j)] = dp_inv * cc * (Gamma(Momentum(j + 1), as) - Gamma(Momentum(j), as));
         }
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < nx1; i++)  // 1287-1293
             for (int j = -1; j < np1; j++)
                 fx[NS(i, j)] = WenoNoMax(f1[NS(i - 2, j)], f1[NS(i - 1, j)], f1[NS(i, j)],
                                          f1[NS(i + 1, j)], ex[NS(i, j)] > 0.0);
+#pragma omp parallel for schedule(static)
         for (int i = -1; i < nx1; i++) {  // 1294-1305
             double as_1 = q * q * em->GetASquared(GetFinestIndex(i));
             double as_2 = q * q * em->GetASquared(GetFinestIndex(i + 1));
@@ -349,11 +352,13 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
                 ep[NS(i, j)] = Em - cc * dx_inv * (Gamma(momentum, as_2) - Gamma(momentum, as_1));
             }
         }
+#pragma omp parallel for schedule(static)
         for (int i = -1; i < nx1; i++)  // 1306-1312
             for (int j = 0; j < np1; j++)
                 fp[NS(i, j)] = WenoNoMax(f1[NS(i, j - 2)], f1[NS(i, j - 1)], f1[NS(i, j)],
                                          f1[NS(i, j + 1)], ep[NS(i, j)] > 0.0);
         // 1313-1394; coarse-fine faces (finer_level_x/p) arrive with AMR
+#pragma omp parallel for schedule(static)
         for (int i = 0; i < nx1; i++)
             for (int j = -1; j < np1; j++) {
                 double am = ex[NS(i, j)], ap1 = ex[NS(i, j + 1)], am1 = ex[NS(i, j - 1)];
@@ -361,6 +366,7 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
                 FxH[step][NS(i, j)] = dx_inv * (fm * am + w3 * (fp1 - fm1) * (ap1 - am1));
                 FxL[step][NS(i, j)] = dx_inv * ((am > 0.0 ? f1[NS(i - 1, j)] : f1[NS(i, j)]) * am);
             }
+#pragma omp parallel for schedule(static)
         for (int i = -1; i < nx1; i++)
             for (int j = 0; j < np1; j++) {
                 double am = ep[NS(i, j)], ap1 = ep[NS(i + 1, j)], am1 = ep[NS(i - 1, j)];
@@ -374,6 +380,7 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
         for (int k = 0; k < ns; k++) a[k] = vx::rk_a[step][k] * timestep;
         double aSum = a[0];
         for (int k = 1; k < ns; k++) aSum = aSum + a[k];
+#pragma omp parallel for schedule(static)
         for (int i = -2; i < nx2; i++)
             for (int j = -2; j < np2; j++) {
                 int idx = NS(i, j);
@@ -397,8 +404,9 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
                 f2[NS(i, j - 1)] -= FpLS[NS(i, j)];
             }
     } else if (subStep == 1) {  // 1536-1594
-        std::vector<double> Ppv(np2), Pmv(np2), Qmv(np2), Qpv(np2);
+#pragma omp parallel for schedule(static)
         for (int i = -1; i < nx1; i++) {
+            std::vector<double> Ppv(np2), Pmv(np2), Qmv(np2), Qpv(np2);
             for (int j = -1; j < np1; j++) {
                 int idx = NS(i, j);
                 int i2 = NS(i, j);
@@ -429,6 +437,7 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
         }
         std::fill(Cp.begin(), Cp.end(), 1.0);
         std::fill(Cx.begin(), Cx.end(), 1.0);
+#pragma omp parallel for schedule(static)
         for (int i = 1; i < nx; i++)
             for (int j = 0; j < np; j++) {
                 int idx = NS(i, j), idxi = NS(i - 1, j), idxj = NS(i, j - 1);

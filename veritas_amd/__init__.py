@@ -1,0 +1,148 @@
+"""veritas_amd: MI355X-native Veritas Vlasov-Maxwell time advance.
+
+Python view of the C++ host library (veritas_amd/lib/libveritas.so), which
+keeps the reference's SolverManager/Mesh/Level/Rectangle/EMFieldSolver
+interface and runs every numerical step in HIP kernels for gfx950
+(veritas_amd/lib/libvx_kernels.so, C ABI in include/veritas_amd.h).
+
+There is no CPU fallback: constructing a `Run` without a usable GPU or without
+the built libraries raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .params import ScenarioParams, landau, two_stream  # noqa: F401
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(HERE, "lib")
+KERNEL_LIB = os.path.join(LIB_DIR, "libvx_kernels.so")
+HOST_LIB = os.path.join(LIB_DIR, "libveritas.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "veritas_amd.h")
+
+PLANES = ("f0", "f1", "f2", "FxDS", "FpDS", "Rp", "Rm", "Cx", "Cp", "FxL0", "FpL0")
+
+_host = None
+
+
+class VeritasError(RuntimeError):
+    pass
+
+
+def build():
+    from .build import build as _b
+    return _b()
+
+
+def host_lib():
+    """Load libveritas.so (and through it libvx_kernels.so); raise if absent."""
+    global _host
+    if _host is None:
+        if not os.path.exists(HOST_LIB) or not os.path.exists(KERNEL_LIB):
+            raise VeritasError("veritas_amd libraries are not built; run veritas_amd/build.py")
+        ctypes.CDLL(KERNEL_LIB, mode=ctypes.RTLD_GLOBAL)
+        L = ctypes.CDLL(HOST_LIB)
+        vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
+        L.vxh_create.restype = vp
+        L.vxh_create.argtypes = [vp, i]
+        L.vxh_create_error.restype = ctypes.c_char_p
+        L.vxh_last_error.restype = ctypes.c_char_p
+        L.vxh_last_error.argtypes = [vp]
+        L.vxh_destroy.argtypes = [vp]
+        L.vxh_calc_dt.argtypes = [vp, d, ctypes.POINTER(d)]
+        L.vxh_set_fused.argtypes = [vp, i]
+        L.vxh_advance.argtypes = [vp, d, i]
+        L.vxh_advance_fields.argtypes = [vp, d, i]
+        L.vxh_sync.argtypes = [vp]
+        L.vxh_stage_moments.argtypes = [vp]
+        L.vxh_stage_poisson.argtypes = [vp]
+        L.vxh_stage_vlasov.argtypes = [vp, d, i]
+        L.vxh_stage_vlasov_sub.argtypes = [vp, d, i, i]
+        L.vxh_stage_fields.argtypes = [vp, d, i]
+        L.vxh_time_advance.argtypes = [vp, d, i, i, ctypes.POINTER(ctypes.c_float)]
+        L.vxh_get.restype = i
+        L.vxh_get.argtypes = [vp, ctypes.c_char_p, i, vp, i]
+        _host = L
+    return _host
+
+
+class Run:
+    """One SolverManager (SolverManager.hpp) for a scenario on one GPU."""
+
+    def __init__(self, params, device=0, fused=True):
+        L = host_lib()
+        self.params = params
+        self.h = L.vxh_create(ctypes.byref(params), device)
+        if not self.h:
+            raise VeritasError(L.vxh_create_error().decode())
+        self.set_fused(fused)
+
+    def _ck(self, rc):
+        if rc != 0:
+            raise VeritasError(host_lib().vxh_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            host_lib().vxh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_fused(self, on):
+        self._ck(host_lib().vxh_set_fused(self.h, int(bool(on))))
+
+    def calc_dt(self, cfl=None):
+        out = ctypes.c_double()
+        self._ck(host_lib().vxh_calc_dt(self.h, self.params.cfl if cfl is None else cfl,
+                                        ctypes.byref(out)))
+        return out.value
+
+    def advance(self, dt, nsteps=1):
+        self._ck(host_lib().vxh_advance(self.h, dt, nsteps))
+
+    def advance_fields(self, dt, nsteps=1):
+        self._ck(host_lib().vxh_advance_fields(self.h, dt, nsteps))
+
+    def sync(self):
+        self._ck(host_lib().vxh_sync(self.h))
+
+    def stage_moments(self):
+        self._ck(host_lib().vxh_stage_moments(self.h))
+
+    def stage_poisson(self):
+        self._ck(host_lib().vxh_stage_poisson(self.h))
+
+    def stage_vlasov(self, dt, step):
+        self._ck(host_lib().vxh_stage_vlasov(self.h, dt, step))
+
+    def stage_vlasov_sub(self, dt, step, sub):
+        self._ck(host_lib().vxh_stage_vlasov_sub(self.h, dt, step, sub))
+
+    def stage_fields(self, dt, step):
+        self._ck(host_lib().vxh_stage_fields(self.h, dt, step))
+
+    def time_advance(self, dt, warmup, steps):
+        ms = ctypes.c_float()
+        self._ck(host_lib().vxh_time_advance(self.h, dt, warmup, steps, ctypes.byref(ms)))
+        return ms.value
+
+    def get(self, what, ghosts=0):
+        L = host_lib()
+        n = L.vxh_get(self.h, what.encode(), ghosts, None, 0)
+        if n == -2:
+            raise KeyError(what)
+        if n < 0:
+            self._ck(-1)
+        out = np.empty(n, dtype=np.float64)
+        self._ck(0 if L.vxh_get(self.h, what.encode(), ghosts, out.ctypes.data, n) == n else -1)
+        if what in PLANES or what[:3] in ("FxH", "FpH"):
+            p = self.params
+            return out.reshape(p.nx + 2 * ghosts, p.np + 2 * ghosts)
+        return out
+
+    def scalars(self):
+        s = self.get("scalars")
+        return dict(Ex0=s[0], time=s[1], dx=s[2], dp=s[3], pmin=s[4], mass=s[5], charge=s[6],
+                    n_prepad=int(s[7]))

@@ -1,0 +1,210 @@
+// Host-side classes of the MI355X framework.  They keep the reference's class
+// interface (SolverManager.hpp, Mesh.hpp, Level.hpp, Rectangle.hpp,
+// EMSolver.hpp, BoundaryCondition.hpp, Settings.hpp) so a driver written
+// against the reference reads the same, while all numerics run in the HIP
+// kernels behind the C ABI of include/veritas_amd.h.
+#pragma once
+
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../../include/veritas_amd.h"
+#include "../scenario.h"
+
+namespace vxh {
+
+class Rectangle;
+class Level;
+class Mesh;
+class EMFieldSolver;
+class SolverManager;
+
+// thrown when a C-ABI call fails (the reference exits; a library reports)
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+// Settings.hpp:4-31
+struct Input {
+    double minEfficiency = 0.6, dx = 0.5, k = 0.01;
+    double refinementCriteria = 1e-8, cfl = 0.9, sizeWeight = 0.0;
+    double preLength = 2, postLength = 2;
+    unsigned int nx = 76 * 4, r = 2, Lfinest = 5;
+    std::vector<double> tempEM = {0.0};
+};
+struct Particles {
+    std::vector<double> mass = {9.10938291e-31};
+    std::vector<double> charge = {-1.60217657e-19};
+    std::vector<unsigned int> np = {50};
+    std::vector<double> dp = {0.1};
+    std::vector<double> pmin = {0.1};
+};
+struct Output {
+    bool time = true, rectangleData = true, charge = true, energy = true, potential = true;
+    bool EFieldLongitudinal = true, EFieldTransverse = true, BFieldTransverse = true,
+         AFieldSquared = true;
+    int precision = 15;
+};
+
+// Settings.hpp:33-65.  The user hooks (InitialDistribution, GetBY, GetBZ,
+// RefinementOverride) are the scenario's.
+class Settings {
+public:
+    Output output;
+    double dx, time, minEfficiency, preLength, postLength, refinementCriteria, cfl, sizeWeight;
+    std::vector<double> m, q, dp, fMax, pmin;
+    unsigned int x_size_finest, x_size, refinementRatio;
+    int maxDepth, quadratureDepth;
+    std::vector<unsigned int> p_size, p_size_finest;
+    vx::Scenario scn;
+
+    Settings(const Input &grid, const Particles &particles, const Output &out,
+             const vx::Scenario &scenario);
+    explicit Settings(const vx::Scenario &scenario);
+    double GetDp(int level, int particleType) const;
+    double GetDx(int level) const;
+    int GetXSize(int level) const;
+    int GetPSize(int level, int particleType) const;
+    double GetMass(int i) const { return m.at(i); }
+    double GetCharge(int i) const { return q.at(i); }
+    double GetfMax(int i) const { return fMax.at(i); }
+    void UpdateTime(int step, double dt);
+    void DetermineMaximum();
+    double InitialDistribution(double x, double p, int particleType) const {
+        return scn.initial(x, p, particleType);
+    }
+    double GetBY(double x, double t) const { return scn.by(x, t); }
+    double GetBZ(double x, double t) const { return scn.bz(x, t); }
+
+private:
+    void finish();
+};
+
+// Shared device context (one GPU, one stream)
+struct Device {
+    vx_ctx *ctx = nullptr;
+    explicit Device(int device);
+    ~Device();
+    void check(int rc, const char *what) const;
+};
+
+// Rectangle.hpp:6-80: one patch whose state lives in HBM.
+class Rectangle {
+public:
+    int n_x, n_p, x_pos, p_pos;
+    double relativeToBottom;
+    Rectangle(int n_x, int n_p, int x_pos, int p_pos, int depth, Settings &settings,
+              std::shared_ptr<Device> dev, bool up, bool down, bool left, bool right,
+              int particleType);
+    ~Rectangle();
+    void SetFieldSolver(EMFieldSolver *solver) { EMSolver = solver; }
+    void InitializeDistribution();
+    void FCTTimeStep(double timestep, int step, int subStep);
+    void FCTStageWalls(double timestep, int step);  // fused whole stage (walls only)
+    void CalculateRhoAndJ();
+    void UpdateSameLevelBoundaries(int val = 1);
+    void UpdateCornerPoints(int val = 1);
+    bool AllWalls() const { return up && down && left && right; }
+    const vx_patch &desc() const { return P; }
+    std::vector<double> Download(const double *plane, int ghosts) const;
+    std::vector<double> DownloadSlot(int slot, int ghosts) const { return Download(P.f[slot], ghosts); }
+    const double *Plane(const std::string &name) const;
+
+private:
+    int depth, particleType;
+    bool up, down, left, right;
+    Settings &settings;
+    std::shared_ptr<Device> dev;
+    EMFieldSolver *EMSolver = nullptr;
+    vx_patch P{};
+    std::vector<void *> allocations;
+    double *alloc_plane();
+};
+
+// BoundaryCondition.hpp:7-11: the domain-edge neighbour whose same-level value
+// is always 0.  On the device it is the zero ghost fill (vx_ghost_zero).
+class BoundaryCondition {
+public:
+    double GetValueFromSameLevel(int, int, int = 1) const { return 0.0; }
+};
+
+// Level.hpp:4-23
+class Level {
+public:
+    std::vector<std::shared_ptr<Rectangle>> rectangles;
+    Level(int particleType, int depth, Settings &settings);
+    void FCTTimeStep(double timestep, int step, int subStep);
+    void PushData(int updateType, int val = 1);
+    int depth;
+
+private:
+    int particleType;
+    Settings &settings;
+};
+
+// Mesh.hpp:4-41
+class Mesh {
+public:
+    int particleType;
+    std::vector<std::unique_ptr<Level>> levels;  // levels[0] is the finest
+    Mesh(int particleType, Settings &settings, std::shared_ptr<Device> dev);
+    void SetFieldSolver(EMFieldSolver *solver);
+    void Advance(double timeStep, int step);
+    void PushData(int val = 1);
+    void PushBoundaryC();
+    bool fused = true;  // single walled patch: one fused call per stage
+
+private:
+    Settings &settings;
+    std::shared_ptr<Device> dev;
+};
+
+// EMSolver.hpp:9-49
+class EMFieldSolver {
+public:
+    EMFieldSolver(Settings &settings, const std::vector<std::shared_ptr<Mesh>> &meshes,
+                  std::shared_ptr<Device> dev);
+    ~EMFieldSolver();
+    void AssembleRhoAndJ();
+    void UpdatePotential();
+    void RGKStep(int step, double timestep);
+    void EnforceChargeNeutralization();
+    double EstimateCFLBound();
+    const vx_fields &fields() const { return fl; }
+    const vx_field_state &state() const { return fs; }
+    void RebuildMomentPlan();
+
+private:
+    Settings &settings;
+    std::vector<std::shared_ptr<Mesh>> meshes;
+    std::shared_ptr<Device> dev;
+    vx_field_state fs{};
+    vx_fields fl{};
+    vx_moment_plan *plan = nullptr;
+    std::vector<void *> allocations;
+    double *alloc(size_t n);
+};
+
+// SolverManager.hpp:6-19
+class SolverManager {
+public:
+    explicit SolverManager(Settings &settings, int device = 0);
+    void Advance(double timeStep);
+    void AdvanceFields(double timeStep);
+    double CalculateDt(double cfl) { return cfl * EMSolver->EstimateCFLBound(); }
+    // pieces of Advance for stage-by-stage checks
+    void StageMoments() { EMSolver->AssembleRhoAndJ(); }
+    void StagePoisson() { EMSolver->UpdatePotential(); }
+    void StageVlasov(double dt, int step);
+    void StageVlasovSub(double dt, int step, int sub);
+    void StageFields(double dt, int step);
+    void SetFused(bool on);
+    std::shared_ptr<Device> dev;
+    std::vector<std::shared_ptr<Mesh>> meshes;
+    std::shared_ptr<EMFieldSolver> EMSolver;
+    Settings &settings;
+};
+
+}  // namespace vxh

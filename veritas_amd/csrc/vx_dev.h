@@ -1,0 +1,74 @@
+// Device helpers shared by the FCT, moment and field kernels.  Each follows
+// the reference expression for expression (same operands, same association)
+// because the FP64 path is bit-compared with the CPU oracle; the library is
+// compiled with -ffp-contract=off so no multiply-add is fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "vx_consts.h"
+
+namespace vxd {
+
+// Rectangle.hpp:111-126
+__device__ __forceinline__ double valmax(double a, double b) { return a > b ? a : b; }
+__device__ __forceinline__ double valmin(double a, double b) { return a < b ? a : b; }
+// std::min / std::max as used by the limiter-C sync (Rectangle.cpp:1646-1829)
+__device__ __forceinline__ double stdmin(double a, double b) { return (b < a) ? b : a; }
+
+// Rectangle.hpp:86-88: lower p-face of cell j (pmin + dp * (j + p_pos))
+__device__ __forceinline__ double momentum(double pmin, double dp, int j, int p_pos) {
+    return pmin + dp * ((double)j + (double)p_pos);
+}
+
+// Rectangle.hpp:181-183 with mc2inv = (m_inv*c_inv)*(m_inv*c_inv)
+__device__ __forceinline__ double gamma_(double p, double a2, double mc2inv) {
+    return sqrt(1.0 + ((p * p) + a2) * mc2inv);
+}
+
+// Rectangle.cpp:979-1030 (two-candidate WENO, upwind side takes the larger
+// mapped weight); identical values to GetWenoEdgeValue (Rectangle.cpp:943-977).
+__device__ __forceinline__ double weno(double f1, double f2, double f3, double f4, bool right) {
+    double fL = (1.0 / 6) * (-f1 + 5 * f2 + 2 * f3);
+    double fR = (1.0 / 6) * (2 * f2 + 5 * f3 - f4);
+    double AL = f1 - 2 * f2 + f3;
+    double BL = f3 - f1;
+    double AR = f2 - 2 * f3 + f4;
+    double BR = f4 - f2;
+    double bL = 4.0 / 3 * (AL * AL) + 0.5 * AL * BL + 0.25 * (BL * BL);
+    double bR = 4.0 / 3 * (AR * AR) - 0.5 * AR * BR + 0.25 * (BR * BR);
+    const double mm = 1.0e-10;
+    double oL = 0.5 / ((mm + bL) * (mm + bL));
+    double oR = 0.5 / ((mm + bR) * (mm + bR));
+    double wL = oL / (oL + oR);
+    double wR = oR / (oL + oR);
+    double wL0 = wL * (0.75 + wL * (wL - 0.5));
+    double wR0 = wR * (0.75 + wR * (wR - 0.5));
+    double s = wL0 + wR0;
+    // right: the larger weight goes to fL; left: the smaller one does
+    bool takeL = right ? (wL0 > wR0) : (wL0 < wR0);
+    double w = (takeL ? wL0 : wR0) / s;
+    double fwL = w;
+    double fwR = 1 - w;
+    return fwL * fL + fwR * fR;
+}
+
+// EMSolver.cpp:558-585: plain two-candidate WENO for the face potentials
+__device__ __forceinline__ double face_weno(double f1, double f2, double f3, double f4) {
+    double fL = (1.0 / 6) * (-f1 + 5 * f2 + 2 * f3);
+    double fR = (1.0 / 6) * (2 * f2 + 5 * f3 - f4);
+    double AL = f1 - 2 * f2 + f3;
+    double BL = f3 - f1;
+    double AR = f2 - 2 * f3 + f4;
+    double BR = f4 - f2;
+    double bL = 4.0 / 3 * (AL * AL) + 0.5 * AL * BL + 0.25 * (BL * BL);
+    double bR = 4.0 / 3 * (AR * AR) - 0.5 * AR * BR + 0.25 * (BR * BR);
+    const double mm = 1.0e-10;
+    double oL = 0.5 / ((mm + bL) * (mm + bL));
+    double oR = 0.5 / ((mm + bR) * (mm + bR));
+    double wL = oL / (oL + oR);
+    double wR = oR / (oL + oR);
+    return wL * fL + wR * fR;
+}
+
+}  // namespace vxd

@@ -1,0 +1,295 @@
+// p-moments: charge and current density (Rectangle::CalculateRhoAndJ,
+// Rectangle.cpp:157-282) and their assembly on the finest grid
+// (Level.cpp:19-62, Mesh.cpp:52-56, EMSolver.cpp:104-122).
+//
+// The reference sums over p sequentially (j = 0, 1, ..., n_p-1) for each x
+// column and each finest sub-cell k.  To reproduce those bits one wavefront
+// owns one column: its 64 lanes evaluate the summands of 64 consecutive p-cells
+// in parallel (interpolants, gamma factors, logarithms) and stage them in LDS;
+// lane 0 then adds them in order.  4096 columns give 4096 independent chains,
+// which keeps every CU busy while each chain stays in reference order.
+#include <cstring>
+#include <vector>
+
+#include "vx_dev.h"
+#include "vx_internal.h"
+#include "vx_math.h"
+
+using namespace vxi;
+using namespace vxd;
+
+namespace {
+
+constexpr int MAX_RTB = 4;  // coarsest level of a 3-level hierarchy at ratio 2
+
+// Rectangle.cpp:76-89
+__constant__ double kInterp[12] = {
+    0.104166666666667, -0.708333333333334, 0.708333333333334, -0.104166666666667,
+    0.117647058823529, 0.029411764705882,  0.029411764705882, 0.117647058823529,
+    -0.083333333333333, 0.166666666666667, -0.166666666666667, 0.083333333333334};
+
+struct MomArgs {
+    vx_patch p;
+    vx_fields fl;
+    double c1, c2, c3;     // m_inv*c_inv, 1/c1, 1/48
+    double inv_rtb;       // 1.0 / relativeToBottom
+    double dpq, jscale;   // dp*q and -q*q/mass
+    double rel[3 * MAX_RTB];  // interpolatCoefsREL (Rectangle.cpp:103-110)
+};
+
+// GetInterpolantsREL(f1..f5)[k] + correction (Rectangle.cpp:139-155, 195-204)
+__device__ __forceinline__ double rel_value(const MomArgs &A, const double *g, int pitch, int i,
+                                            int j, int k, double corr) {
+    double f1 = g[pidx(pitch, i - 2, j)], f2 = g[pidx(pitch, i - 1, j)];
+    double f3 = g[pidx(pitch, i, j)], f4 = g[pidx(pitch, i + 1, j)], f5 = g[pidx(pitch, i + 2, j)];
+    f5 -= f3; f4 -= f3; f2 -= f3; f1 -= f3;
+    double a1 = kInterp[0] * f1 + kInterp[1] * f2 + kInterp[2] * f4 + kInterp[3] * f5;
+    double a2 = kInterp[4] * f1 + kInterp[5] * f2 + kInterp[6] * f4 + kInterp[7] * f5;
+    double a3 = kInterp[8] * f1 + kInterp[9] * f2 + kInterp[10] * f4 + kInterp[11] * f5;
+    return A.rel[3 * k] * a1 + A.rel[3 * k + 1] * a2 + A.rel[3 * k + 2] * a3 + f3 + corr;
+}
+
+// correction making the sub-cell values average to the cell value
+__device__ double rel_correction(const MomArgs &A, const double *g, int pitch, int i, int j) {
+    int n = A.p.rtb;
+    double te = 0.0;
+    for (int k = 0; k < n; k++) te += rel_value(A, g, pitch, i, j, k, 0.0);
+    return g[pidx(pitch, i, j)] - A.inv_rtb * te;
+}
+
+// EMFieldSolver::GetCellAverageASquared (EMSolver.hpp:57-64)
+__device__ __forceinline__ double cell_a2(const vx_fields &fl, long i) {
+    i += fl.n_prepad;
+    i = i > -1 ? i : 0;
+    i = i < fl.n_e ? i : (fl.n_e - 1);
+    double ay = fl.Ay1[i], az = fl.Az1[i];
+    return (ay * ay) + (az * az);
+}
+
+// LDS exchange inside one wavefront: order the lanes' LDS writes before the
+// other lanes' reads (no workgroup barrier is needed; waves are independent).
+__device__ __forceinline__ void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One wavefront per x column; block = 4 columns.  RTB = relativeToBottom.
+template <int RTB>
+__global__ __launch_bounds__(256) void k_moments(MomArgs A) {
+    const vx_patch &P = A.p;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int i = blockIdx.x * 4 + wave;
+    constexpr int n = RTB;
+    __shared__ double termR[4][RTB][64];
+    __shared__ double termJ[4][RTB][64];
+    __shared__ double hbuf[4][RTB][68];
+    if (i >= P.n_x) return;  // whole wavefront exits together
+    const int pitch = P.pitch;
+    const double *g = P.f[1];
+    double accR[RTB], accJ[RTB];
+    double asq[RTB];
+    for (int k = 0; k < n; k++) {
+        accR[k] = 0.0;
+        accJ[k] = 0.0;
+        asq[k] = P.q * P.q * cell_a2(A.fl, (long)(i + P.x_pos) * n + k);
+    }
+    for (int jb = 0; jb < P.n_p; jb += 64) {
+        const int j = jb + lane;
+        // h(j') = Gamma(Momentum(j')) + c1*Momentum(j') for j' in [jb-1, jb+67)
+        for (int k = 0; k < n; k++) {
+            for (int e = lane; e < 68; e += 64) {
+                int jj = jb - 1 + e;
+                double pm = momentum(P.pmin, P.dp, jj, P.p_pos);
+                hbuf[wave][k][e] = gamma_(pm, asq[k], P.mc2inv) + A.c1 * pm;
+            }
+        }
+        wave_sync();
+        if (j < P.n_p) {
+            bool skip = P.nested && P.nested[pidx(pitch, i, j)];
+            for (int k = 0; k < n; k++) {
+                double tr = 0.0, tj = 0.0;
+                if (!skip) {
+                    double tm, t0, tp;
+                    if (n == 1) {
+                        // relativeToBottom == 1: the interpolant coefficients are
+                        // exactly 0 and the correction exactly 0, so the sub-cell
+                        // value is the cell value itself.
+                        tm = g[pidx(pitch, i, j - 1)];
+                        t0 = g[pidx(pitch, i, j)];
+                        tp = g[pidx(pitch, i, j + 1)];
+                    } else {
+                        tm = rel_value(A, g, pitch, i, j - 1, k, rel_correction(A, g, pitch, i, j - 1));
+                        t0 = rel_value(A, g, pitch, i, j, k, rel_correction(A, g, pitch, i, j));
+                        tp = rel_value(A, g, pitch, i, j + 1, k, rel_correction(A, g, pitch, i, j + 1));
+                    }
+                    // h indices: j-1 -> lane, j -> lane+1, j+1 -> lane+2, j+2 -> lane+3
+                    double hm1 = hbuf[wave][k][lane], h0 = hbuf[wave][k][lane + 1];
+                    double h1 = hbuf[wave][k][lane + 2], h2 = hbuf[wave][k][lane + 3];
+                    double gm = A.c2 * vx_log(h1 / h0);
+                    double gm1 = A.c2 * vx_log(h0 / hm1);
+                    double gp1 = A.c2 * vx_log(h2 / h1);
+                    tr = t0;
+                    tj = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
+                }
+                termR[wave][k][lane] = tr;
+                termJ[wave][k][lane] = skip ? 0.0 : tj;
+            }
+        }
+        wave_sync();
+        if (lane == 0) {
+            int cnt = P.n_p - jb < 64 ? P.n_p - jb : 64;
+            for (int k = 0; k < n; k++) {
+                double r = accR[k], q = accJ[k];
+                for (int e = 0; e < cnt; e++) {
+                    bool sk = P.nested && P.nested[pidx(pitch, i, jb + e)];
+                    if (!sk) {
+                        r += termR[wave][k][e];
+                        q += termJ[wave][k][e];
+                    }
+                }
+                accR[k] = r;
+                accJ[k] = q;
+            }
+        }
+        wave_sync();
+    }
+    if (lane == 0) {
+        for (int k = 0; k < n; k++) {
+            P.chargeR[(long)i * n + k] = accR[k] * A.dpq;
+            P.currentR[(long)i * n + k] = accJ[k] * A.jscale;
+        }
+    }
+}
+
+struct MomentSrc {
+    const double *chargeR, *currentR;
+    int start, count;  // finest-grid span [start, start + count)
+};
+
+// Assembly in the reference order: per species, levels finest first, one
+// patch per finest index and level (patches of a level do not overlap).
+__global__ void k_assemble(const MomentSrc *src, const int *level_start, int n_species,
+                           int n_levels, int n_f, double *charge, double *J,
+                           double *charges_species) {
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n_f) return;
+    double jsum = 0.0;  // EMSolver::J, accumulated over species and levels
+    double ch = 0.0;
+    for (int s = 0; s < n_species; s++) {
+        double cs_ = 0.0;  // charges[s][x]
+        for (int l = 0; l < n_levels; l++) {
+            double cl = 0.0, jl = 0.0;  // chargeL / currentL of level l
+            int a = level_start[s * n_levels + l], b = level_start[s * n_levels + l + 1];
+            for (int q = a; q < b; q++) {
+                int off = x - src[q].start;
+                if (off >= 0 && off < src[q].count) {
+                    cl += src[q].chargeR[off];
+                    jl += src[q].currentR[off];
+                }
+            }
+            cs_ += cl;
+            jsum += jl;
+        }
+        if (charges_species) charges_species[(long)s * n_f + x] = cs_;
+        ch += cs_;
+    }
+    charge[x] = ch;
+    J[x] = jsum;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
+    int rc = validate_patch(ctx, p);
+    if (rc) return rc;
+    if (p->rtb > MAX_RTB) return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d > %d", p->rtb, MAX_RTB);
+    MomArgs A;
+    A.p = *p;
+    A.fl = *fl;
+    double m_inv = 1 / p->m;
+    A.c1 = m_inv * vx_c_inv;
+    A.c2 = 1 / A.c1;
+    A.c3 = 1 / 48.0;
+    double rtb = (double)p->rtb;
+    A.inv_rtb = 1.0 / rtb;
+    A.dpq = p->dp * p->q;
+    A.jscale = -p->q * p->q / p->m;
+    for (int i = 0; i < 3 * MAX_RTB; i++) A.rel[i] = 0.0;
+    for (int i = 0; i < p->rtb; i++) {
+        double tl = -0.5 + i / rtb;
+        double tr = -0.5 + (i + 1.0) / rtb;
+        A.rel[3 * i] = (tl + tr) * 0.5;
+        A.rel[3 * i + 1] = (tl * tl + tl * tr + tr * tr) / 3.0 - (1.0 / 12);
+        A.rel[3 * i + 2] = (tl * tl * tl + tl * tl * tr + tl * tr * tr + tr * tr * tr) * 0.25;
+    }
+    unsigned g = (unsigned)((p->n_x + 3) / 4);
+    switch (p->rtb) {
+        case 1: k_moments<1><<<g, 256, 0, ctx->stream>>>(A); break;
+        case 2: k_moments<2><<<g, 256, 0, ctx->stream>>>(A); break;
+        case 4: k_moments<4><<<g, 256, 0, ctx->stream>>>(A); break;
+        default: return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d not in {1,2,4}", p->rtb);
+    }
+    return check_launch(ctx, "k_moments");
+}
+
+int vx_moment_plan_create(vx_ctx *ctx, const vx_patch *patches, const vx_moment_layout *lay,
+                          int32_t n_f, vx_moment_plan **out) {
+    if (!out) return fail(ctx, VX_EINVAL, "null plan pointer");
+    *out = nullptr;
+    if (!lay || lay->n_species < 1 || lay->n_levels < 1 || n_f < 1)
+        return fail(ctx, VX_EINVAL, "bad moment layout");
+    int ngroups = lay->n_species * lay->n_levels;
+    int npatch = lay->level_start[ngroups];
+    std::vector<MomentSrc> src(npatch > 0 ? npatch : 1);
+    for (int q = 0; q < npatch; q++) {
+        src[q].chargeR = patches[q].chargeR;
+        src[q].currentR = patches[q].currentR;
+        src[q].start = patches[q].x_pos * patches[q].rtb;
+        src[q].count = patches[q].n_x * patches[q].rtb;
+    }
+    size_t sb = sizeof(MomentSrc) * src.size();
+    size_t bytes = sb + sizeof(int) * (ngroups + 1);
+    vx_moment_plan *pl = new vx_moment_plan;
+    pl->n_species = lay->n_species;
+    pl->n_levels = lay->n_levels;
+    pl->n_f = n_f;
+    if (hipMalloc(&pl->tables, bytes) != hipSuccess) {
+        delete pl;
+        return fail(ctx, VX_ENOMEM, "moment plan tables");
+    }
+    std::vector<char> host(bytes);
+    std::memcpy(host.data(), src.data(), sb);
+    std::memcpy(host.data() + sb, lay->level_start, sizeof(int) * (ngroups + 1));
+    hipError_t e = hipMemcpy(pl->tables, host.data(), bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hipFree(pl->tables);
+        delete pl;
+        return fail(ctx, VX_EHIP, "moment plan copy: %s", hipGetErrorString(e));
+    }
+    pl->src_off = 0;
+    pl->ls_off = sb;
+    *out = pl;
+    return VX_OK;
+}
+
+void vx_moment_plan_destroy(vx_moment_plan *pl) {
+    if (!pl) return;
+    hipFree(pl->tables);
+    delete pl;
+}
+
+int vx_moments_assemble(vx_ctx *ctx, const vx_moment_plan *pl, double *charge, double *J,
+                        double *charges_species) {
+    if (!pl) return fail(ctx, VX_EINVAL, "null moment plan");
+    const MomentSrc *dsrc = (const MomentSrc *)((char *)pl->tables + pl->src_off);
+    const int *dls = (const int *)((char *)pl->tables + pl->ls_off);
+    k_assemble<<<(unsigned)((pl->n_f + 255) / 256), 256, 0, ctx->stream>>>(
+        dsrc, dls, pl->n_species, pl->n_levels, pl->n_f, charge, J, charges_species);
+    return check_launch(ctx, "k_assemble");
+}
+
+}  // extern "C"
