@@ -1,0 +1,132 @@
+"""Benchmark: phase-space cell-updates/s of the Vlasov-Maxwell time advance.
+
+Workload (one GPU): the relativistic two-stream configuration at its full size,
+Nx = Np = 4096 (16.8 M phase-space cells), single unrefined level, FP64, walls
+at the domain edges, dt = 0.4 x the CFL bound.  One step = one full 6-stage
+Runge-Kutta step of SolverManager::Advance (moments, Poisson, FCT Vlasov stage,
+transverse-field stage, six times).  A cell-update is one phase-space cell
+advanced by one full step.
+
+Prints one JSON line with the roofline of the dominant kernel (its average
+launch time measured live with HIP events on the library stream) and the CPU
+baseline (the FP64 CPU oracle on this host's cores, a bounded sample of the
+same workload).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode fused|plain]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12           # MI355X HBM3E, B/s
+BYTES_PER_UPDATE = 216.0    # algorithmic FP64 bytes per cell-update (stage s: 8(s+1) + 8)
+
+
+def kernel_bytes_per_cell(kernel, step):
+    """Bytes per cell that one launch of `kernel` at RK stage `step` must move
+    through its interface, each array touched once."""
+    s = step
+    if kernel == "k_sub0":
+        # read f_s, write FxH_s/FpH_s and FxDS/FpDS, read FxH/FpH of stages < s,
+        # FxL0/FpL0 written at stage 0 and read afterwards
+        return 8 + 16 + 16 + 16 * s + 16
+    if kernel == "k_stage":
+        # store-flux fused stage: read f_s and f^n, read FxH/FpH of stages < s,
+        # write FxH_s/FpH_s (except the last stage) and f_{s+1}
+        return 8 + 8 + 16 * s + (16 if s < 5 else 0) + 8
+    raise ValueError(kernel)
+
+
+def cpu_baseline(params, steps):
+    """Time the CPU oracle (test infrastructure, used here only as the
+    baseline) on `steps` steps of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from _oracle import OracleRun
+    o = OracleRun(params)
+    dt = o.calc_dt(0.4)
+    t0 = time.perf_counter()
+    o.advance(dt, steps)
+    el = time.perf_counter() - t0
+    o.close()
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": params.nx * params.np * steps / el, "unit": "cell-updates/s",
+            "cores": cores, "kind": "port",
+            "sample": f"{steps} steps of the {params.nx}x{params.np} two-stream workload "
+                      f"(FP64 CPU restatement, OpenMP, {el:.1f} s; initialisation excluded)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--nx", type=int, default=4096)
+    ap.add_argument("--np", type=int, default=4096)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1", file=sys.stderr)
+        sys.exit(2)
+
+    import veritas_amd as va
+    params = va.two_stream(nx=a.nx, np=a.np)
+    run = va.Run(params, fused=(a.mode == "fused"))
+    dt = run.calc_dt(0.4)
+    kernel = "k_stage" if (a.mode == "fused" and va.has_fused_stage()) else "k_sub0"
+
+    run.advance(dt, a.warmup)
+    run.sync()
+    run.timing_begin(kernel)
+    t0 = time.perf_counter()
+    ms = run.time_advance(dt, 0, a.steps)
+    wall = time.perf_counter() - t0
+    kms, nlaunch = run.timing_end()
+
+    cells = params.nx * params.np
+    value = cells * a.steps / (ms / 1e3)
+    # dominant kernel: algorithmic bytes per launch averaged over the six stages
+    per_cell = sum(kernel_bytes_per_cell(kernel, s) for s in range(6)) / 6.0
+    bytes_per_launch = per_cell * cells
+    avg_launch_s = (kms / nlaunch) / 1e3 if nlaunch else float("nan")
+    achieved = bytes_per_launch / avg_launch_s
+    out = {
+        "metric": "phase-space cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": ms / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded two-stream initial condition)",
+        "config": {"workload": f"relativistic two-stream, single level, {params.nx}x{params.np} "
+                               f"phase-space cells, FP64, dt=0.4xCFL",
+                   "nx": params.nx, "np": params.np, "mode": a.mode,
+                   "hbm_frac_216B": value * BYTES_PER_UPDATE / HBM_PEAK,
+                   "wall_s": wall},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved / 1e9,
+                     "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                     "traffic": None, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "avg_launch_ms": kms / nlaunch if nlaunch else None,
+                     "launches": nlaunch, "kernel_share": kms / ms if ms else None},
+    }
+    run.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(params, a.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

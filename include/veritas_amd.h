@@ -162,6 +162,10 @@ int vx_cfl_bound(vx_ctx *ctx, const vx_field_state *fs, int n_species, const dou
 /* ---- timing helpers (HIP events on the context stream) ---- */
 int vx_event_record(vx_ctx *ctx, int slot);
 int vx_event_elapsed_ms(vx_ctx *ctx, int slot_a, int slot_b, float *ms);
+/* Bracket every launch of the named kernel (e.g. "k_sub0") with events until
+ * vx_timing_end, which synchronises and returns the summed device time. */
+int vx_timing_begin(vx_ctx *ctx, const char *kernel);
+int vx_timing_end(vx_ctx *ctx, double *total_ms, int *launches);
 
 #ifdef __cplusplus
 }

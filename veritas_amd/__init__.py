@@ -35,6 +35,12 @@ def build():
     return _b()
 
 
+def has_fused_stage():
+    """True when the library's walled-patch stage is one fused kernel."""
+    L = ctypes.CDLL(KERNEL_LIB)
+    return hasattr(L, "vx_fct_stage_fused_available") and L.vx_fct_stage_fused_available() == 1
+
+
 def host_lib():
     """Load libveritas.so (and through it libvx_kernels.so); raise if absent."""
     global _host
@@ -61,6 +67,8 @@ def host_lib():
         L.vxh_stage_vlasov_sub.argtypes = [vp, d, i, i]
         L.vxh_stage_fields.argtypes = [vp, d, i]
         L.vxh_time_advance.argtypes = [vp, d, i, i, ctypes.POINTER(ctypes.c_float)]
+        L.vxh_timing_begin.argtypes = [vp, ctypes.c_char_p]
+        L.vxh_timing_end.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i)]
         L.vxh_get.restype = i
         L.vxh_get.argtypes = [vp, ctypes.c_char_p, i, vp, i]
         _host = L
@@ -127,6 +135,16 @@ class Run:
         ms = ctypes.c_float()
         self._ck(host_lib().vxh_time_advance(self.h, dt, warmup, steps, ctypes.byref(ms)))
         return ms.value
+
+    def timing_begin(self, kernel):
+        """Bracket every launch of `kernel` (e.g. "k_sub0") with HIP events on
+        the library stream until timing_end()."""
+        self._ck(host_lib().vxh_timing_begin(self.h, kernel.encode()))
+
+    def timing_end(self):
+        ms, n = ctypes.c_double(), ctypes.c_int()
+        self._ck(host_lib().vxh_timing_end(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def get(self, what, ghosts=0):
         L = host_lib()

@@ -110,6 +110,16 @@ int vxh_time_advance(void *h, double dt, int warmup, int steps, float *ms) {
     });
 }
 
+int vxh_timing_begin(void *h, const char *kernel) {
+    return guarded(h, [&](Run *r) { r->sm->dev->check(vx_timing_begin(r->sm->dev->ctx, kernel), "timing"); });
+}
+
+int vxh_timing_end(void *h, double *total_ms, int *launches) {
+    return guarded(h, [&](Run *r) {
+        r->sm->dev->check(vx_timing_end(r->sm->dev->ctx, total_ms, launches), "timing");
+    });
+}
+
 // Named arrays, same names and layouts as the oracle's C API.
 int vxh_get(void *h, const char *what, int ghosts, double *out, int n) {
     Run *r = (Run *)h;

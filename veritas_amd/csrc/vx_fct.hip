@@ -314,7 +314,10 @@ int vx_fct_sub0(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fi
     A.step = step;
     rk_coefficients(step, dt, A.a, A.ns, A.aSum);
     dim3 grid((unsigned)((p->n_p + 4 + TJ - 1) / TJ), (unsigned)((p->n_x + 4 + TI - 1) / TI));
-    k_sub0<<<grid, 256, 0, ctx->stream>>>(A);
+    {
+        LaunchTimer lt(ctx, "k_sub0");
+        k_sub0<<<grid, 256, 0, ctx->stream>>>(A);
+    }
     rc = check_launch(ctx, "k_sub0");
     if (rc) return rc;
     double a[6], aSum;

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <string>
+#include <vector>
 
 #include "../../include/veritas_amd.h"
 
@@ -14,6 +15,10 @@ struct vx_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     std::string err;
+    // per-launch timing of one named kernel (vx_timing_begin / vx_timing_end)
+    std::string timed_kernel;
+    std::vector<hipEvent_t> tev;  // pairs: start, stop
+    size_t tcount = 0;
 };
 
 struct vx_moment_plan {
@@ -42,6 +47,29 @@ inline int check_hip(vx_ctx *ctx, hipError_t e, const char *what) {
 inline int check_launch(vx_ctx *ctx, const char *what) {
     return check_hip(ctx, hipGetLastError(), what);
 }
+
+// Events around one launch when `name` is the kernel being timed.
+struct LaunchTimer {
+    vx_ctx *ctx;
+    hipEvent_t stop = nullptr;
+    LaunchTimer(vx_ctx *c, const char *name) : ctx(c) {
+        if (ctx->timed_kernel.empty() || ctx->timed_kernel != name) return;
+        size_t k = 2 * ctx->tcount;
+        if (k + 2 > ctx->tev.size()) {
+            for (int e = 0; e < 64; e++) {
+                hipEvent_t ev;
+                if (hipEventCreate(&ev) != hipSuccess) return;
+                ctx->tev.push_back(ev);
+            }
+        }
+        if (hipEventRecord(ctx->tev[k], ctx->stream) != hipSuccess) return;
+        stop = ctx->tev[k + 1];
+        ctx->tcount++;
+    }
+    ~LaunchTimer() {
+        if (stop) (void)hipEventRecord(stop, ctx->stream);
+    }
+};
 
 // Plane index for element (i, j) of a patch plane.
 __host__ __device__ inline long pidx(int pitch, int i, int j) {

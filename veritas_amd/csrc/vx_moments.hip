@@ -266,7 +266,7 @@ int vx_moment_plan_create(vx_ctx *ctx, const vx_patch *patches, const vx_moment_
     std::memcpy(host.data() + sb, lay->level_start, sizeof(int) * (ngroups + 1));
     hipError_t e = hipMemcpy(pl->tables, host.data(), bytes, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(pl->tables);
+        (void)hipFree(pl->tables);
         delete pl;
         return fail(ctx, VX_EHIP, "moment plan copy: %s", hipGetErrorString(e));
     }
@@ -278,7 +278,7 @@ int vx_moment_plan_create(vx_ctx *ctx, const vx_patch *patches, const vx_moment_
 
 void vx_moment_plan_destroy(vx_moment_plan *pl) {
     if (!pl) return;
-    hipFree(pl->tables);
+    (void)hipFree(pl->tables);
     delete pl;
 }
 

@@ -27,11 +27,12 @@ int vx_ctx_create(int device, vx_ctx **out) {
 
 void vx_ctx_destroy(vx_ctx *ctx) {
     if (!ctx) return;
-    hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
     for (auto &e : ctx->ev)
-        if (e) hipEventDestroy(e);
-    hipStreamDestroy(ctx->stream);
+        if (e) (void)hipEventDestroy(e);
+    for (auto &e : ctx->tev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
@@ -54,7 +55,7 @@ int vx_malloc(vx_ctx *ctx, size_t bytes, void **dptr) {
 
 int vx_free(vx_ctx *ctx, void *dptr) {
     if (!dptr) return VX_OK;
-    hipStreamSynchronize(ctx->stream);
+    (void)hipStreamSynchronize(ctx->stream);
     return check_hip(ctx, hipFree(dptr), "hipFree");
 }
 
@@ -96,6 +97,29 @@ int vx_event_elapsed_ms(vx_ctx *ctx, int a, int b, float *ms) {
     int rc = check_hip(ctx, hipEventSynchronize(ctx->ev[b]), "event sync");
     if (rc) return rc;
     return check_hip(ctx, hipEventElapsedTime(ms, ctx->ev[a], ctx->ev[b]), "event elapsed");
+}
+
+int vx_timing_begin(vx_ctx *ctx, const char *kernel) {
+    ctx->timed_kernel = kernel ? kernel : "";
+    ctx->tcount = 0;
+    return VX_OK;
+}
+
+int vx_timing_end(vx_ctx *ctx, double *total_ms, int *launches) {
+    int rc = vx_sync(ctx);
+    if (rc) return rc;
+    double tot = 0.0;
+    for (size_t k = 0; k < ctx->tcount; k++) {
+        float ms = 0.0f;
+        rc = check_hip(ctx, hipEventElapsedTime(&ms, ctx->tev[2 * k], ctx->tev[2 * k + 1]), "timing");
+        if (rc) return rc;
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (int)ctx->tcount;
+    ctx->timed_kernel.clear();
+    ctx->tcount = 0;
+    return VX_OK;
 }
 
 int32_t vx_plane_pitch(int32_t n_p) { return ((n_p + 4 + 7) / 8) * 8; }
