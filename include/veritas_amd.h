@@ -75,6 +75,9 @@ typedef struct vx_fields {
     const double *E;   /* EMFieldSolver::GetEfield(i), i in [0, n_f) (periodic) */
     const double *a2;  /* EMFieldSolver::a_squared, n_f + 1 faces */
     const double *Ay1, *Az1; /* transverse potentials, RK slot 1, n_e cells */
+    const double *E0, *a20;  /* E and a_squared as they were at RK stage 0 of this
+                                step (the fused stage recomputes the stage-0
+                                low-order fluxes from them) */
 } vx_fields;
 
 /* Plane geometry helper: pitch (doubles) for a patch with n_p cells. */
@@ -97,6 +100,17 @@ int vx_fct_sub3(vx_ctx *ctx, const vx_patch *p);
  * (Mesh::Advance, Mesh.cpp:64-89, for one patch).  Equivalent to
  * sub0 + ghost(2) + sub1 + sub2 + ghost(1) [+ sub3 at step 5]. */
 int vx_fct_stage_walls(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl);
+
+/* The same stage as one fused kernel (register streaming along x, wavefront
+ * shuffles along p); store-flux design: recomputes the stage-0 low-order
+ * fluxes from f[0] and the stage-0 fields, stores FxH/FpH of this stage, and
+ * writes f_{s+1} into `fout`.  `fin` is f_s (p->f[1]); `fout` must alias
+ * neither `fin` nor p->f[0]; ghost rings of all three planes must be zero.
+ * Bit-identical to the sub-step kernels.  Call vx_fct_stage_fused_available()
+ * to check that the library provides it. */
+int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
+                       const double *fin, double *fout);
+int vx_fct_stage_fused_available(void);
 
 /* Ghost fill of slot `slot` of a patch whose neighbours are all the domain-edge
  * sentinel (BoundaryCondition::GetValueFromSameLevel returns 0,

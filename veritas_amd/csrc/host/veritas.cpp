@@ -141,7 +141,7 @@ Rectangle::Rectangle(int n_x_, int n_p_, int x_pos_, int p_pos_, int depth_, Set
     P.cc = vx_cs * vx_cs * P.m;
     double m_inv = 1 / P.m;
     P.mc2inv = (m_inv * vx_c_inv) * (m_inv * vx_c_inv);
-    for (auto &f : P.f) f = alloc_plane();
+    for (int k = 0; k < 3; k++) P.f[k] = buf[k] = alloc_plane();
     for (int k = 0; k < 6; k++) {
         P.FxH[k] = alloc_plane();
         P.FpH[k] = alloc_plane();
@@ -203,6 +203,7 @@ void Rectangle::InitializeDistribution() {
 
 void Rectangle::FCTTimeStep(double timestep, int step, int subStep) {
     vx_ctx *c = dev->ctx;
+    UnaliasSlots();
     switch (subStep) {
         case 0: dev->check(vx_fct_sub0(c, &P, step, timestep, &EMSolver->fields()), "fct sub0"); break;
         case 1: dev->check(vx_fct_sub1(c, &P), "fct sub1"); break;
@@ -213,7 +214,38 @@ void Rectangle::FCTTimeStep(double timestep, int step, int subStep) {
 }
 
 void Rectangle::FCTStageWalls(double timestep, int step) {
+    UnaliasSlots();
     dev->check(vx_fct_stage_walls(dev->ctx, &P, step, timestep, &EMSolver->fields()), "fct stage");
+}
+
+// f[1] = f_s is read, f_{s+1} goes to the plane that holds neither f^n nor
+// f_s; at stage 5 f^n and f_s become the same plane (f[0] <- f[1],
+// Rectangle.cpp:1614-1622, without a copy).
+void Rectangle::FCTStageFused(double timestep, int step) {
+    double *out = nullptr;
+    for (double *b : buf)
+        if (b != P.f[0] && b != P.f[1]) {
+            out = b;
+            break;
+        }
+    dev->check(vx_fct_stage_fused(dev->ctx, &P, step, timestep, &EMSolver->fields(), P.f[1], out),
+               "fct stage fused");
+    P.f[1] = out;
+    if (step == 5) P.f[0] = out;
+    for (double *b : buf)
+        if (b != P.f[0] && b != P.f[1]) P.f[2] = b;
+}
+
+void Rectangle::UnaliasSlots() {
+    if (P.f[0] != P.f[1]) return;
+    double *spare[2];
+    int n = 0;
+    for (double *b : buf)
+        if (b != P.f[0]) spare[n++] = b;
+    P.f[1] = spare[0];
+    P.f[2] = spare[1];
+    dev->check(vx_memcpy_d2d(dev->ctx, P.f[1], P.f[0], sizeof(double) * vx_plane_elems(n_x, n_p)),
+               "unalias slots");
 }
 
 void Rectangle::CalculateRhoAndJ() {
@@ -309,7 +341,7 @@ void Mesh::SetFieldSolver(EMFieldSolver *solver) {
 void Mesh::Advance(double timeStep, int step) {
     if (fused && levels.size() == 1 && levels[0]->rectangles.size() == 1 &&
         levels[0]->rectangles[0]->AllWalls()) {
-        levels[0]->rectangles[0]->FCTStageWalls(timeStep, step);
+        levels[0]->rectangles[0]->FCTStageFused(timeStep, step);
         return;
     }
     for (int i = (int)levels.size() - 1; i > -1; i--) levels[i]->FCTTimeStep(timeStep, step, 0);
@@ -367,6 +399,9 @@ EMFieldSolver::EMFieldSolver(Settings &s, const std::vector<std::shared_ptr<Mesh
     fs.Az = alloc((size_t)8 * fs.n_e);
     fs.a2 = alloc((size_t)N + 1);
     fs.work = alloc(vx_field_work_elems(N));
+    double *E0 = alloc(N), *a20 = alloc((size_t)N + 1);
+    fl.E0 = E0;
+    fl.a20 = a20;
     fl.n_f = N;
     fl.n_prepad = fs.n_prepad;
     fl.n_e = fs.n_e;
@@ -418,6 +453,11 @@ void EMFieldSolver::AssembleRhoAndJ() {
 
 void EMFieldSolver::UpdatePotential() { dev->check(vx_poisson(dev->ctx, &fs), "poisson"); }
 
+void EMFieldSolver::SnapshotStage0Fields() {
+    dev->check(vx_memcpy_d2d(dev->ctx, (void *)fl.E0, fs.E, sizeof(double) * fs.n_f), "E0");
+    dev->check(vx_memcpy_d2d(dev->ctx, (void *)fl.a20, fs.a2, sizeof(double) * (fs.n_f + 1)), "a2_0");
+}
+
 // EMSolver.cpp:194-202; the hooks are evaluated at x = 0 and the stage time
 void EMFieldSolver::RGKStep(int step, double timestep) {
     double by0 = settings.GetBY(0, settings.time);
@@ -461,6 +501,7 @@ void SolverManager::Advance(double timeStep) {
     for (int i = 0; i < 6; i++) {
         EMSolver->AssembleRhoAndJ();
         EMSolver->UpdatePotential();
+        if (i == 0) EMSolver->SnapshotStage0Fields();
         for (auto &m : meshes) m->Advance(timeStep, i);
         settings.UpdateTime(i, timeStep);
         EMSolver->RGKStep(i, timeStep);
@@ -476,6 +517,7 @@ void SolverManager::AdvanceFields(double timeStep) {
 }
 
 void SolverManager::StageVlasov(double dt, int step) {
+    if (step == 0) EMSolver->SnapshotStage0Fields();
     for (auto &m : meshes) m->Advance(dt, step);
 }
 

@@ -102,7 +102,9 @@ public:
     void SetFieldSolver(EMFieldSolver *solver) { EMSolver = solver; }
     void InitializeDistribution();
     void FCTTimeStep(double timestep, int step, int subStep);
-    void FCTStageWalls(double timestep, int step);  // fused whole stage (walls only)
+    void FCTStageWalls(double timestep, int step);  // sub-step kernels, whole stage (walls only)
+    void FCTStageFused(double timestep, int step);  // one fused kernel per stage (walls only)
+    void UnaliasSlots();  // give f[0] and f[1] distinct planes (before sub-step kernels)
     void CalculateRhoAndJ();
     void UpdateSameLevelBoundaries(int val = 1);
     void UpdateCornerPoints(int val = 1);
@@ -119,6 +121,7 @@ private:
     std::shared_ptr<Device> dev;
     EMFieldSolver *EMSolver = nullptr;
     vx_patch P{};
+    double *buf[3] = {nullptr, nullptr, nullptr};  // physical planes behind f[0..2]
     std::vector<void *> allocations;
     double *alloc_plane();
 };
@@ -175,6 +178,7 @@ public:
     const vx_fields &fields() const { return fl; }
     const vx_field_state &state() const { return fs; }
     void RebuildMomentPlan();
+    void SnapshotStage0Fields();  // E and A^2 of RK stage 0 for the fused stage
 
 private:
     Settings &settings;

@@ -39,24 +39,43 @@ struct PoissonArgs {
 // Work layout (doubles): b | x | r | c | y | tmp | partials
 __host__ __device__ inline size_t wk_off(int n, int which) { return (size_t)which * n; }
 
-// exclusive prefix sum of v[0..n) into out[0..n) (out may alias v), deterministic
+constexpr int PW = PT / 64;  // wavefronts in the Poisson workgroup
+
+__device__ __forceinline__ double wave_incl_scan(double v) {
+    const int lane = threadIdx.x & 63;
+    for (int off = 1; off < 64; off <<= 1) {
+        double t = __shfl_up(v, off, 64);
+        if (lane >= off) v += t;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// exclusive prefix sum of v[0..n) into out[0..n) (out may alias v); each
+// thread owns a contiguous chunk; fixed (deterministic) order
 __device__ void block_exclusive_scan(const double *v, double *out, int n, double *lds) {
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int chunk = (n + PT - 1) / PT;
     const int a = t * chunk, b = min(n, a + chunk);
     double s = 0.0;
     for (int i = a; i < b; i++) s += v[i];
-    lds[t] = s;
+    double inc = wave_incl_scan(s);
+    if (lane == 63) lds[w] = inc;
     __syncthreads();
-    // Hillis-Steele inclusive scan over PT partials
-    for (int off = 1; off < PT; off <<= 1) {
-        double add = t >= off ? lds[t - off] : 0.0;
-        __syncthreads();
-        lds[t] += add;
-        __syncthreads();
+    if (t == 0) {
+        double run = 0.0;
+        for (int k = 0; k < PW; k++) {
+            double tot = lds[k];
+            lds[PW + k] = run;
+            run += tot;
+        }
     }
-    double run = t > 0 ? lds[t - 1] : 0.0;
     __syncthreads();
+    double run = lds[PW + w] + (inc - s);
     for (int i = a; i < b; i++) {
         double vi = v[i];
         out[i] = run;
@@ -65,21 +84,24 @@ __device__ void block_exclusive_scan(const double *v, double *out, int n, double
     __syncthreads();
 }
 
-// block sum of v[lo..hi), deterministic order (chunks then a tree over threads)
+// block sum of v[lo..hi), deterministic order
 __device__ double block_sum(const double *v, int lo, int hi, double *lds) {
-    const int t = threadIdx.x;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int n = hi - lo;
     const int chunk = (n + PT - 1) / PT;
     const int a = lo + t * chunk, b = min(hi, a + chunk);
     double s = 0.0;
     for (int i = a; i < b; i++) s += v[i];
-    lds[t] = s;
+    s = wave_sum(s);
+    if (lane == 0) lds[w] = s;
     __syncthreads();
-    for (int off = PT / 2; off > 0; off >>= 1) {
-        if (t < off) lds[t] += lds[t + off];
-        __syncthreads();
+    if (t == 0) {
+        double tot = 0.0;
+        for (int k = 0; k < PW; k++) tot += lds[k];
+        lds[2 * PW] = tot;
     }
-    double r = lds[0];
+    __syncthreads();
+    double r = lds[2 * PW];
     __syncthreads();
     return r;
 }

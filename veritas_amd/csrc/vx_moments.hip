@@ -74,91 +74,108 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One wavefront per x column; block = 4 columns.  RTB = relativeToBottom.
-template <int RTB>
+// A wavefront owns C x columns (NCH = C * RTB finest sub-columns, the
+// reference's accumulation chains) and walks p in chunks of 64.  Per chunk the
+// 64 lanes (one per p-cell) evaluate the summands of every chain in parallel
+// and stage them in LDS; then lane ch (< NCH) adds the 64 charge summands of
+// chain ch in p order and lane 32 + ch the current summands.  Skipped cells
+// (nested, or past n_p) contribute -0.0, which is an exact identity for
+// addition, so every chain performs the reference's additions in its order.
+template <int RTB, int C>
 __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
+    constexpr int NCH = C * RTB;
+    static_assert(NCH <= 32, "chains must fit in half a wavefront");
     const vx_patch &P = A.p;
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int i = blockIdx.x * 4 + wave;
-    constexpr int n = RTB;
-    __shared__ double termR[4][RTB][64];
-    __shared__ double termJ[4][RTB][64];
-    __shared__ double hbuf[4][RTB][68];
-    if (i >= P.n_x) return;  // whole wavefront exits together
+    const int i0 = (blockIdx.x * 4 + wave) * C;
+    __shared__ double S[4][NCH][68];   // sub-cell values, p index jb-1+e
+    __shared__ double H[4][68];        // Gamma(p)+c1 p for the chain being prepared
+    __shared__ double Lb[4][NCH][68];  // c2 log(h(j+1)/h(j)), p index jb-1+e
+    __shared__ double TR[4][NCH][65], TJ[4][NCH][65];
+    if (i0 >= P.n_x) return;  // uniform per wavefront
     const int pitch = P.pitch;
     const double *g = P.f[1];
-    double accR[RTB], accJ[RTB];
-    double asq[RTB];
-    for (int k = 0; k < n; k++) {
-        accR[k] = 0.0;
-        accJ[k] = 0.0;
-        asq[k] = P.q * P.q * cell_a2(A.fl, (long)(i + P.x_pos) * n + k);
+    double asq[NCH];
+    for (int ch = 0; ch < NCH; ch++) {
+        int c = ch / RTB, k = ch % RTB;
+        int i = i0 + c < P.n_x ? i0 + c : P.n_x - 1;
+        asq[ch] = P.q * P.q * cell_a2(A.fl, (long)(i + P.x_pos) * RTB + k);
     }
+    double acc = 0.0;  // lanes < NCH: charge chain; lanes 32..32+NCH-1: current chain
     for (int jb = 0; jb < P.n_p; jb += 64) {
-        const int j = jb + lane;
-        // h(j') = Gamma(Momentum(j')) + c1*Momentum(j') for j' in [jb-1, jb+67)
-        for (int k = 0; k < n; k++) {
-            for (int e = lane; e < 68; e += 64) {
-                int jj = jb - 1 + e;
-                double pm = momentum(P.pmin, P.dp, jj, P.p_pos);
-                hbuf[wave][k][e] = gamma_(pm, asq[k], P.mc2inv) + A.c1 * pm;
-            }
-        }
-        wave_sync();
-        if (j < P.n_p) {
-            bool skip = P.nested && P.nested[pidx(pitch, i, j)];
-            for (int k = 0; k < n; k++) {
-                double tr = 0.0, tj = 0.0;
-                if (!skip) {
-                    double tm, t0, tp;
-                    if (n == 1) {
-                        // relativeToBottom == 1: the interpolant coefficients are
-                        // exactly 0 and the correction exactly 0, so the sub-cell
-                        // value is the cell value itself.
-                        tm = g[pidx(pitch, i, j - 1)];
-                        t0 = g[pidx(pitch, i, j)];
-                        tp = g[pidx(pitch, i, j + 1)];
+        // sub-cell values for p in [jb-1, jb+65)
+        for (int ch = 0; ch < NCH; ch++) {
+            int c = ch / RTB, k = ch % RTB;
+            int i = i0 + c;
+            for (int e = lane; e < 66; e += 64) {
+                int j = jb - 1 + e;
+                double v = 0.0;
+                if (i < P.n_x && j < P.n_p + 2) {
+                    if (RTB == 1) {
+                        // interpolant coefficients and correction are exactly 0 for
+                        // relativeToBottom == 1: the sub-cell value is the cell value
+                        v = g[pidx(pitch, i, j)];
                     } else {
-                        tm = rel_value(A, g, pitch, i, j - 1, k, rel_correction(A, g, pitch, i, j - 1));
-                        t0 = rel_value(A, g, pitch, i, j, k, rel_correction(A, g, pitch, i, j));
-                        tp = rel_value(A, g, pitch, i, j + 1, k, rel_correction(A, g, pitch, i, j + 1));
+                        v = rel_value(A, g, pitch, i, j, k, rel_correction(A, g, pitch, i, j));
                     }
-                    // h indices: j-1 -> lane, j -> lane+1, j+1 -> lane+2, j+2 -> lane+3
-                    double hm1 = hbuf[wave][k][lane], h0 = hbuf[wave][k][lane + 1];
-                    double h1 = hbuf[wave][k][lane + 2], h2 = hbuf[wave][k][lane + 3];
-                    double gm = A.c2 * vx_log(h1 / h0);
-                    double gm1 = A.c2 * vx_log(h0 / hm1);
-                    double gp1 = A.c2 * vx_log(h2 / h1);
-                    tr = t0;
-                    tj = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
                 }
-                termR[wave][k][lane] = tr;
-                termJ[wave][k][lane] = skip ? 0.0 : tj;
+                S[wave][ch][e] = v;
             }
         }
-        wave_sync();
-        if (lane == 0) {
-            int cnt = P.n_p - jb < 64 ? P.n_p - jb : 64;
-            for (int k = 0; k < n; k++) {
-                double r = accR[k], q = accJ[k];
-                for (int e = 0; e < cnt; e++) {
-                    bool sk = P.nested && P.nested[pidx(pitch, i, jb + e)];
-                    if (!sk) {
-                        r += termR[wave][k][e];
-                        q += termJ[wave][k][e];
-                    }
-                }
-                accR[k] = r;
-                accJ[k] = q;
+        // logarithm terms, computed once per distinct A^2 among the chains
+        for (int ch = 0; ch < NCH; ch++) {
+            if (ch > 0 && asq[ch] == asq[ch - 1]) {
+                for (int e = lane; e < 66; e += 64) Lb[wave][ch][e] = Lb[wave][ch - 1][e];
+                continue;
             }
+            for (int e = lane; e < 67; e += 64) {
+                double pm = momentum(P.pmin, P.dp, jb - 1 + e, P.p_pos);
+                H[wave][e] = gamma_(pm, asq[ch], P.mc2inv) + A.c1 * pm;
+            }
+            wave_sync();
+            for (int e = lane; e < 66; e += 64) Lb[wave][ch][e] = A.c2 * vx_log(H[wave][e + 1] / H[wave][e]);
+            wave_sync();
+        }
+        wave_sync();
+        // summands of p-cell j = jb + lane for every chain (Rectangle.cpp:214-253)
+        const int j = jb + lane;
+        const bool live = j < P.n_p;
+        for (int ch = 0; ch < NCH; ch++) {
+            int c = ch / RTB;
+            int i = i0 + c;
+            bool skip = !live || i >= P.n_x || (P.nested && P.nested[pidx(pitch, i, j)]);
+            double tr = -0.0, tj = -0.0;
+            if (!skip) {
+                double tm = S[wave][ch][lane], t0 = S[wave][ch][lane + 1], tp = S[wave][ch][lane + 2];
+                double gm = Lb[wave][ch][lane + 1], gm1 = Lb[wave][ch][lane], gp1 = Lb[wave][ch][lane + 2];
+                tr = t0;
+                tj = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
+            }
+            TR[wave][ch][lane] = tr;
+            TJ[wave][ch][lane] = tj;
+        }
+        wave_sync();
+        // ordered accumulation
+        if ((lane & 31) < NCH) {
+            const double *row = (lane < 32) ? &TR[wave][lane][0] : &TJ[wave][lane - 32][0];
+            double a = acc;
+            for (int e = 0; e < 64; e += 8) {
+                double v0 = row[e], v1 = row[e + 1], v2 = row[e + 2], v3 = row[e + 3];
+                double v4 = row[e + 4], v5 = row[e + 5], v6 = row[e + 6], v7 = row[e + 7];
+                a += v0; a += v1; a += v2; a += v3; a += v4; a += v5; a += v6; a += v7;
+            }
+            acc = a;
         }
         wave_sync();
     }
-    if (lane == 0) {
-        for (int k = 0; k < n; k++) {
-            P.chargeR[(long)i * n + k] = accR[k] * A.dpq;
-            P.currentR[(long)i * n + k] = accJ[k] * A.jscale;
+    if ((lane & 31) < NCH) {
+        int ch = lane & 31;
+        int c = ch / RTB, k = ch % RTB;
+        int i = i0 + c;
+        if (i < P.n_x) {
+            if (lane < 32) P.chargeR[(long)i * RTB + k] = acc * A.dpq;
+            else P.currentR[(long)i * RTB + k] = acc * A.jscale;
         }
     }
 }
@@ -226,11 +243,11 @@ int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
         A.rel[3 * i + 1] = (tl * tl + tl * tr + tr * tr) / 3.0 - (1.0 / 12);
         A.rel[3 * i + 2] = (tl * tl * tl + tl * tl * tr + tl * tr * tr + tr * tr * tr) * 0.25;
     }
-    unsigned g = (unsigned)((p->n_x + 3) / 4);
+    LaunchTimer lt(ctx, "k_moments");
     switch (p->rtb) {
-        case 1: k_moments<1><<<g, 256, 0, ctx->stream>>>(A); break;
-        case 2: k_moments<2><<<g, 256, 0, ctx->stream>>>(A); break;
-        case 4: k_moments<4><<<g, 256, 0, ctx->stream>>>(A); break;
+        case 1: k_moments<1, 4><<<(unsigned)((p->n_x + 15) / 16), 256, 0, ctx->stream>>>(A); break;
+        case 2: k_moments<2, 2><<<(unsigned)((p->n_x + 7) / 8), 256, 0, ctx->stream>>>(A); break;
+        case 4: k_moments<4, 1><<<(unsigned)((p->n_x + 3) / 4), 256, 0, ctx->stream>>>(A); break;
         default: return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d not in {1,2,4}", p->rtb);
     }
     return check_launch(ctx, "k_moments");

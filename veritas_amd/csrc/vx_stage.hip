@@ -1,0 +1,316 @@
+// Fused FCT stage for one patch whose four edges are domain walls (a single
+// unrefined level): Rectangle::FCTTimeStep sub-steps 0-2 with the zero ghost
+// fills between them (Mesh::Advance, Mesh.cpp:64-89; Rectangle.cpp:1255-1613)
+// in one kernel, bit-identical to the sub-step kernels and the CPU oracle.
+//
+// Design (register streaming, no LDS):
+//  * A wavefront owns 64 consecutive p-cells, lane l <-> j = jw - 3 + l.  Every
+//    quantity of the stage depends on p-neighbours at most 3 cells away, so
+//    lanes 3..60 produce exact results (58 outputs per wavefront) and p
+//    neighbours are read from other lanes with cross-lane shuffles.
+//  * The wavefront marches along x over a strip of TI output rows.  At march
+//    row r it loads f_s(r) and f^n(r) (one coalesced 512-byte row each) and
+//    advances a software pipeline held in registers:
+//        x-face r-1 : WENO value and high-order flux FxH_s
+//        column r   : WENO value; column r-1: high-order flux FpH_s
+//        x-face r, column r: low-order fluxes of stage 0 (recomputed from f^n
+//                     and the stage-0 fields instead of stored)
+//        cell r-1   : low-order predictor f2
+//        x-face r-1, column r-1: antidiffusive fluxes FxDS/FpDS
+//        cell r-2   : Zalesak R+-
+//        cell r-3   : limiter C on its faces and the corrected value f_{s+1}
+//  * Store-flux design: FxH_s/FpH_s are written once (owned faces) for the
+//    later stages' RK combinations instead of recomputing the WENO fluxes of
+//    every earlier stage from stored stage values.
+// Every quantity is computed only inside the reference's loop range and is 0
+// outside it, as the reference's never-written array entries are.
+#include "vx_dev.h"
+#include "vx_internal.h"
+
+using namespace vxi;
+using namespace vxd;
+
+namespace {
+
+constexpr int HALO = 3;
+constexpr int OUTW = 64 - 2 * HALO;  // p-cells produced per wavefront
+
+struct StageArgs {
+    vx_patch p;
+    const double *fin;  // f_s (stage input), ghost ring zero
+    double *fout;       // f_{s+1}; only interior cells are written
+    const double *E, *a2;    // fields of this stage (finest grid)
+    const double *E0, *a20;  // fields of stage 0 (for the low-order fluxes)
+    int n_f;
+    int step, ns;
+    double a[6], aSum;
+    int TI;       // output rows per strip
+    int nstrip;   // strips in x
+};
+
+__device__ __forceinline__ double shdn(double v, int d) { return __shfl_down(v, d, 64); }
+__device__ __forceinline__ double shup(double v, int d) { return __shfl_up(v, d, 64); }
+
+__device__ __forceinline__ double ldp(const double *pl, int pitch, int nx, int np, int i, int j) {
+    return (i >= -2 && i < nx + 2 && j >= -2 && j < np + 2) ? pl[pidx(pitch, i, j)] : 0.0;
+}
+
+// q^2 A^2 at x-face i (Rectangle.cpp:1280, 1296-1297; EMSolver.cpp:133-135)
+__device__ __forceinline__ double face_as(const StageArgs &A, const double *a2, int i) {
+    long fi = (long)A.p.rtb * (long)(i + A.p.x_pos);
+    long c = fi < 0 ? 0 : (fi > A.n_f ? A.n_f : fi);
+    return A.p.q * A.p.q * a2[c];
+}
+
+// q * Rectangle::GetEfield(i) (Rectangle.cpp:1055-1067, 1298)
+__device__ __forceinline__ double col_qE(const StageArgs &A, const double *E, int i) {
+    long j = (long)A.p.rtb * (long)(i + A.p.x_pos);
+    double t = 0.0;
+    for (int k = 0; k < A.p.rtb; k++) {
+        long m = (j + k) % A.n_f;
+        if (m < 0) m += A.n_f;
+        t += E[m];
+    }
+    t *= (1.0 / (double)A.p.rtb);
+    return A.p.q * t;
+}
+
+__global__ __launch_bounds__(256) void k_stage(StageArgs A) {
+    const vx_patch &P = A.p;
+    const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
+    const int lane = threadIdx.x & 63;
+    const int strip = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (strip >= A.nstrip) return;  // uniform per wavefront
+    const int i0 = strip * A.TI;
+    const int i1 = min(i0 + A.TI, nx);       // output rows [i0, i1)
+    const bool last_strip = i0 + A.TI >= nx;
+    const int jw = blockIdx.x * OUTW;
+    const int j = jw - HALO + lane;
+    const bool out_lane = lane >= HALO && lane < 64 - HALO && j < np;
+    // faces this wavefront stores (each valid face has exactly one owner)
+    const bool own_row_x = (lane >= HALO && lane < 64 - HALO && j < np) ||
+                           (j == np && lane >= HALO && lane < 65 - HALO) ||
+                           (jw == 0 && j == -1);
+    const bool own_row_p = (lane >= HALO && lane < 64 - HALO && j < np) ||
+                           (j == np && lane >= HALO && lane < 65 - HALO);
+    const bool store_flux = A.step < 5;
+
+    // reference loop ranges in j (Rectangle.cpp:1279-1394) for this lane
+    const bool jr_ex = j >= -1 && j < np + 1;   // ex, fx, FxH, FxL rows
+    const bool jr_ep = j >= -1 && j < np + 2;   // ep
+    const bool jr_fp = j >= 0 && j < np + 1;    // fp, FpH, FpL p-faces
+    const bool jr_R = j >= -1 && j < np + 1;    // R
+    const bool jr_ns = j >= -2 && j < np + 2;   // FxDS/FpDS
+    // update loop ranges (walls on all sides: xm = 1, xp = n_x, pm = 1, pp = n_p)
+    const int xm = P.left ? 1 : 0, xp = P.right ? nx : nx + 1;
+    const int pm = P.down ? 1 : 0, pp = P.up ? np : np + 1;
+
+    const double mc2 = P.mc2inv;
+    const double dpcc = P.dp_inv * P.cc;
+    const double ccdx = P.cc * P.dx_inv;
+    const double w3 = 1 / 48.0;
+    const double pj = momentum(P.pmin, P.dp, j, P.p_pos);
+    const double pj2 = pj * pj;
+
+    const double *fin = A.fin, *f0p = P.f[0];
+    const int r0 = i0 - HALO;
+
+    // pipeline state (names: _mK = row/face r-K)
+    double fs_m3 = 0, fs_m2 = 0, fs_m1 = 0, fs_0 = 0;
+    double f0_m3 = 0, f0_m2 = 0, f0_m1 = 0, f0_0 = 0;
+    double ep_m2 = 0, ep_m1 = 0, ep_0 = 0, fp_m2 = 0, fp_m1 = 0, fp_0 = 0;
+    double FxL_m1 = 0, FxL_0 = 0, FpL_m1 = 0, FpL_0 = 0;
+    double f2_m3 = 0, f2_m2 = 0, f2_m1 = 0;
+    double FxD_m3 = 0, FxD_m2 = 0, FxD_m1 = 0;
+    double FpD_m3 = 0, FpD_m2 = 0, FpD_m1 = 0;
+    double Rp_m4 = 0, Rp_m3 = 0, Rp_m2 = 0, Rm_m4 = 0, Rm_m3 = 0, Rm_m2 = 0;
+
+    // Gamma(p_j, q^2 A^2 at x-face i), current fields and stage-0 fields
+    double as_m1 = face_as(A, A.a2, r0 - 1), as_0 = face_as(A, A.a2, r0);
+    double G_m1 = gamma_(pj, as_m1, mc2);
+    double G_0 = (as_0 == as_m1) ? G_m1 : gamma_(pj, as_0, mc2);
+    double as0_0 = face_as(A, A.a20, r0);
+    double G0_0 = (as0_0 == as_0) ? G_0 : gamma_(pj, as0_0, mc2);
+
+    for (int r = r0; r <= i1 + HALO - 1; r++) {
+        // ---- ingest row r
+        fs_0 = ldp(fin, pitch, nx, np, r, j);
+        f0_0 = ldp(f0p, pitch, nx, np, r, j);
+        double as_p1 = face_as(A, A.a2, r + 1);
+        double G_p1 = (as_p1 == as_0) ? G_0 : gamma_(pj, as_p1, mc2);
+        double as0_p1 = face_as(A, A.a20, r + 1);
+        double G0_p1 = (as0_p1 == as_p1) ? G_p1 : gamma_(pj, as0_p1, mc2);
+
+        // ---- x-face r-1: ex, fx (Rectangle.cpp:1278-1293), FxH_s (1313-1334)
+        const int ix = r - 1;
+        double Gd = shdn(G_m1, 1);  // Gamma at p-face j+1
+        double ex = (ix >= -1 && ix < nx + 2 && jr_ex) ? dpcc * (Gd - G_m1) : 0.0;
+        double fx = (ix >= 0 && ix < nx + 1 && jr_ex) ? weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0) : 0.0;
+        double ex_u = shdn(ex, 1), ex_d = shup(ex, 1);
+        double fx_u = shdn(fx, 1), fx_d = shup(fx, 1);
+        const bool vFx = ix >= 0 && ix < nx + 1 && jr_ex;
+        double FxH = vFx ? P.dx_inv * (fx * ex + w3 * (fx_u - fx_d) * (ex_u - ex_d)) : 0.0;
+
+        // ---- column r: ep, fp (Rectangle.cpp:1294-1312)
+        double qE = (r >= -1 && r < nx + 1) ? col_qE(A, A.E, r) : 0.0;
+        ep_0 = (r >= -1 && r < nx + 1 && jr_ep) ? qE - ccdx * (G_p1 - G_0) : 0.0;
+        double s2 = shup(fs_0, 2), s1 = shup(fs_0, 1), s3 = shdn(fs_0, 1);
+        fp_0 = (r >= -1 && r < nx + 1 && jr_fp) ? weno(s2, s1, fs_0, s3, ep_0 > 0.0) : 0.0;
+        // ---- column r-1: FpH_s (Rectangle.cpp:1353-1375)
+        const bool vFp = ix >= -1 && ix < nx + 1 && jr_fp;
+        double FpH = vFp ? P.dp_inv * (fp_m1 * ep_m1 + w3 * (fp_0 - fp_m2) * (ep_0 - ep_m2)) : 0.0;
+        if (store_flux) {
+            const bool own_x = ix >= i0 && (ix < i0 + A.TI || last_strip);
+            const bool own_c = (ix >= i0 || i0 == 0) && (ix < i0 + A.TI || last_strip);
+            if (vFx && own_x && own_row_x) P.FxH[A.step][pidx(pitch, ix, j)] = FxH;
+            if (vFp && own_c && own_row_p) P.FpH[A.step][pidx(pitch, ix, j)] = FpH;
+        }
+
+        // ---- low-order fluxes of stage 0 at x-face r and column r
+        // (Rectangle.cpp:1335-1352, 1376-1394 evaluated with the stage-0 fields)
+        double G0d = shdn(G0_0, 1);
+        double ex0 = dpcc * (G0d - G0_0);
+        FxL_0 = (r >= 0 && r < nx + 1 && jr_ex) ? P.dx_inv * ((ex0 > 0.0 ? f0_m1 : f0_0) * ex0) : 0.0;
+        double qE0 = (r >= -1 && r < nx + 1) ? col_qE(A, A.E0, r) : 0.0;
+        double ep0 = qE0 - ccdx * (G0_p1 - G0_0);
+        double f0_jm1 = shup(f0_0, 1);
+        FpL_0 = (r >= -1 && r < nx + 1 && jr_fp) ? P.dp_inv * ((ep0 > 0.0 ? f0_jm1 : f0_0) * ep0) : 0.0;
+
+        // ---- cell r-1: low-order predictor (Rectangle.cpp:1518-1534, gather form)
+        {
+            const int i = r - 1;
+            double FpL_up = shdn(FpL_m1, 1);  // FpL0 at (i, j+1)
+            double t = 0.0;
+            if (i >= 0 && i < nx && j >= 0 && j < np) {
+                t = f0_m1;
+                const bool in0 = i >= xm && i < xp && j >= pm && j < pp;
+                if (in0) {
+                    t += A.aSum * FxL_m1;
+                    t += A.aSum * FpL_m1;
+                }
+                if (i >= xm && i < xp && j + 1 >= pm && j + 1 < pp) t -= A.aSum * FpL_up;
+                if (i + 1 >= xm && i + 1 < xp && j >= pm && j < pp) t -= A.aSum * FxL_0;
+            }
+            f2_m1 = t;  // ghost cells: zero fill of the walls (BoundaryCondition.cpp:6-8)
+        }
+
+        // ---- x-face r-1 and column r-1: RK flux combination (Rectangle.cpp:1396-1517)
+        {
+            double xLS = A.aSum * FxL_m1, pLS = A.aSum * FpL_m1;
+            const bool nsx = ix >= -2 && ix < nx + 2 && jr_ns;
+            double sx = A.a[0] * (A.step == 0 ? FxH : ((vFx && nsx) ? P.FxH[0][pidx(pitch, ix, j)] : 0.0));
+            double sp = A.a[0] * (A.step == 0 ? FpH : ((vFp && nsx) ? P.FpH[0][pidx(pitch, ix, j)] : 0.0));
+            for (int k = 1; k < A.ns; k++) {
+                double hx = (k == A.step) ? FxH : ((vFx && nsx) ? P.FxH[k][pidx(pitch, ix, j)] : 0.0);
+                double hp = (k == A.step) ? FpH : ((vFp && nsx) ? P.FpH[k][pidx(pitch, ix, j)] : 0.0);
+                sx = sx + A.a[k] * hx;
+                sp = sp + A.a[k] * hp;
+            }
+            FxD_m1 = sx - xLS;
+            FpD_m1 = sp - pLS;
+        }
+
+        // ---- cell r-2: Zalesak R+- (Rectangle.cpp:1538-1579)
+        {
+            const int i = r - 2;
+            double FpDu = shdn(FpD_m2, 1);  // FpDS at (i, j+1)
+            double f0u = shdn(f0_m2, 1), f0d = shup(f0_m2, 1);
+            double f2u = shdn(f2_m2, 1), f2d = shup(f2_m2, 1);
+            double rp = 0.0, rm = 0.0;
+            if (i >= -1 && i < nx + 1 && jr_R) {
+                double Pp = valmax(0.0, FxD_m2) - valmin(0.0, FxD_m1) + valmax(0.0, FpD_m2) - valmin(0.0, FpDu);
+                double Pm = valmax(0.0, FxD_m1) - valmin(0.0, FxD_m2) + valmax(0.0, FpDu) - valmin(0.0, FpD_m2);
+                double wMax = valmax(valmax(f0_m2, f2_m2), valmax(valmax(f0_m1, f2_m1),
+                              valmax(valmax(f0_m3, f2_m3), valmax(valmax(f0u, f2u), valmax(f0d, f2d)))));
+                double wMin = valmin(valmin(f0_m2, f2_m2), valmin(valmin(f0_m1, f2_m1),
+                              valmin(valmin(f0_m3, f2_m3), valmin(valmin(f0u, f2u), valmin(f0d, f2d)))));
+                double Qm = -wMin + f2_m2;
+                double Qp = wMax - f2_m2;
+                rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
+                rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
+            }
+            Rp_m2 = rp;
+            Rm_m2 = rm;
+        }
+
+        // ---- cell r-3: limiter C on its faces (Rectangle.cpp:1581-1594) and the
+        // corrected value (Rectangle.cpp:1596-1612, gather form)
+        {
+            const int i = r - 3;
+            double Rpd = shup(Rp_m3, 1), Rmd = shup(Rm_m3, 1);  // R at (i, j-1)
+            const bool jint = j >= 0 && j < np;
+            double cxl = (i >= 1 && i < nx && jint)
+                             ? (FxD_m3 > 0.0 ? valmin(Rp_m3, Rm_m4) : valmin(Rp_m4, Rm_m3)) : 1.0;
+            double cxr = (i + 1 >= 1 && i + 1 < nx && jint)
+                             ? (FxD_m2 > 0.0 ? valmin(Rp_m2, Rm_m3) : valmin(Rp_m3, Rm_m2)) : 1.0;
+            double cpl = (i >= 1 && i < nx && jint)
+                             ? (FpD_m3 > 0.0 ? valmin(Rp_m3, Rmd) : valmin(Rpd, Rm_m3)) : 1.0;
+            double cpu = shdn(cpl, 1);      // C at (i, j+1), computed by the next lane
+            double FpDu = shdn(FpD_m3, 1);  // FpDS at (i, j+1)
+            if (out_lane && i >= i0 && i < i1) {
+                double t = f2_m3;
+                if (i >= xm && i < xp && j >= pm && j < pp) {
+                    t += cxl * FxD_m3;
+                    t += cpl * FpD_m3;
+                }
+                if (i >= xm && i < xp && j + 1 >= pm && j + 1 < pp) t -= cpu * FpDu;
+                if (i + 1 >= xm && i + 1 < xp && j >= pm && j < pp) t -= cxr * FxD_m2;
+                A.fout[pidx(pitch, i, j)] = t;
+            }
+        }
+
+        // ---- shift the pipeline by one row
+        fs_m3 = fs_m2; fs_m2 = fs_m1; fs_m1 = fs_0;
+        f0_m3 = f0_m2; f0_m2 = f0_m1; f0_m1 = f0_0;
+        as_0 = as_p1; G_m1 = G_0; G_0 = G_p1; G0_0 = G0_p1;
+        ep_m2 = ep_m1; ep_m1 = ep_0; fp_m2 = fp_m1; fp_m1 = fp_0;
+        FxL_m1 = FxL_0; FpL_m1 = FpL_0;
+        f2_m3 = f2_m2; f2_m2 = f2_m1;
+        FxD_m3 = FxD_m2; FxD_m2 = FxD_m1;
+        FpD_m3 = FpD_m2; FpD_m2 = FpD_m1;
+        Rp_m4 = Rp_m3; Rp_m3 = Rp_m2; Rm_m4 = Rm_m3; Rm_m3 = Rm_m2;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_fct_stage_fused_available(void) { return 1; }
+
+int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
+                       const double *fin, double *fout) {
+    int rc = validate_patch(ctx, p);
+    if (rc) return rc;
+    if (step < 0 || step > 5 || !fl || !fl->E0 || !fl->a20)
+        return fail(ctx, VX_EINVAL, "stage_fused: bad step %d or missing stage-0 fields", step);
+    if (!(p->left && p->right && p->down && p->up))
+        return fail(ctx, VX_EINVAL, "stage_fused: patch must touch all four domain edges");
+    if (fin == p->f[0] && step != 0)
+        return fail(ctx, VX_EINVAL, "stage_fused: input aliases f^n after stage 0");
+    if (fout == fin || fout == p->f[0])
+        return fail(ctx, VX_EINVAL, "stage_fused: output must not alias the inputs");
+    StageArgs A;
+    A.p = *p;
+    A.fin = fin;
+    A.fout = fout;
+    A.E = fl->E;
+    A.a2 = fl->a2;
+    A.E0 = fl->E0;
+    A.a20 = fl->a20;
+    A.n_f = fl->n_f;
+    A.step = step;
+    A.ns = step + 1;
+    for (int k = 0; k < 6; k++) A.a[k] = 0.0;
+    for (int k = 0; k < A.ns; k++) A.a[k] = vx::rk_a[step][k] * dt;
+    A.aSum = A.a[0];
+    for (int k = 1; k < A.ns; k++) A.aSum = A.aSum + A.a[k];
+    A.TI = p->n_x >= 512 ? 64 : (p->n_x >= 128 ? 32 : 16);
+    A.nstrip = (p->n_x + A.TI - 1) / A.TI;
+    dim3 grid((unsigned)((p->n_p + OUTW - 1) / OUTW), (unsigned)((A.nstrip + 3) / 4));
+    LaunchTimer lt(ctx, "k_stage");
+    k_stage<<<grid, 256, 0, ctx->stream>>>(A);
+    return check_launch(ctx, "k_stage");
+}
+
+}  // extern "C"
