@@ -229,6 +229,208 @@ __global__ __launch_bounds__(PT) void k_poisson(PoissonArgs P) {
     for (int i = t; i < N; i += PT) F.E[i] = efield(F.phi, N, F.field_coef, ex0n, i);
 }
 
+// Register/LDS variant for n_f <= 4 * PT: thread t owns the CH consecutive
+// entries [t*CH, t*CH + CH); neighbours (residual, convolution) come from LDS.
+constexpr int SMALL_MAX = 4 * PT;
+
+// exclusive scan of per-thread totals; returns this thread's offset
+__device__ double block_offsets(double tot, double *part) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    double inc = wave_incl_scan(tot);
+    if (lane == 63) part[w] = inc;
+    __syncthreads();
+    if (t == 0) {
+        double run = 0.0;
+        for (int k = 0; k < PW; k++) {
+            double v = part[k];
+            part[PW + k] = run;
+            run += v;
+        }
+    }
+    __syncthreads();
+    double off = part[PW + w] + (inc - tot);
+    __syncthreads();
+    return off;
+}
+
+__device__ double block_total(double v, double *part) {
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    v = wave_sum(v);
+    if (lane == 0) part[w] = v;
+    __syncthreads();
+    if (t == 0) {
+        double tot = 0.0;
+        for (int k = 0; k < PW; k++) tot += part[k];
+        part[2 * PW] = tot;
+    }
+    __syncthreads();
+    double r = part[2 * PW];
+    __syncthreads();
+    return r;
+}
+
+template <int CH>
+__device__ void approx_small(const PoissonArgs &P, const double *in, double *out, double *yb,
+                             double *part) {
+    const int N = P.fs.n_f;
+    const int base = threadIdx.x * CH;
+    double loc = 0.0;
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        if (base + k >= 1 && base + k < N) loc += in[k];
+    double tail = block_total(loc, part);
+    double rp[CH], S[CH];
+    double tot = 0.0;
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        int i = base + k;
+        rp[k] = (i == 0) ? -tail : (i < N ? in[k] : 0.0);
+        tot += rp[k];
+    }
+    double run = block_offsets(tot, part);
+    double ssum = 0.0;
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        S[k] = run;
+        run += rp[k];
+        if (base + k < N) ssum += S[k];
+    }
+    double d0 = block_total(ssum, part) * P.invN;
+    double D[CH];
+    tot = 0.0;
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        int i = base + k;
+        D[k] = (i == 0 || i >= N) ? 0.0 : d0 - S[k];
+        tot += D[k];
+    }
+    run = block_offsets(tot, part);
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        run += D[k];
+        if (base + k < N) yb[base + k] = run;
+    }
+    __syncthreads();
+    double cp[CH];
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        int i = base + k;
+        double v = 0.0;
+        if (i < N) {
+            v = P.g[0] * yb[i];
+            for (int m = 1; m < NG; m++) {
+                int ip = i + m, im = i - m;
+                ip -= ip >= N ? N : 0;
+                im += im < 0 ? N : 0;
+                v += P.g[m] * (yb[ip] + yb[im]);
+            }
+        }
+        cp[k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) part[0] = cp[0];
+    __syncthreads();
+    double c0 = part[0];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        int i = base + k;
+        out[k] = (i > 0) ? cp[k] - c0 : cp[k];
+        if (i < N) yb[i] = out[k];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        out[0] = in[0] - (((P.w1 * yb[1] + P.w2 * yb[2]) + P.w2 * yb[N - 2]) + P.w1 * yb[N - 1]);
+    __syncthreads();
+}
+
+template <int CH>
+__device__ void residual_small(const PoissonArgs &P, const double *b, const double *x, double *r,
+                               double *xb) {
+    const int N = P.fs.n_f;
+    const int base = threadIdx.x * CH;
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        if (base + k < N) xb[base + k] = x[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        int i = base + k;
+        if (i >= N) {
+            r[k] = 0.0;
+            continue;
+        }
+        double sh = b[k], sl = 0.0;
+        auto sub = [&](double a, double xv) {
+            double p = a * xv;
+            double e = fma(a, xv, -p);
+            double tt = sh - p;
+            double bv = tt - sh;
+            double err = (sh - (tt - bv)) + (-p - bv);
+            sl = sl + (err - e);
+            sh = tt;
+        };
+        if (i == 0) {
+            sub(1.0, xb[0]); sub(P.w1, xb[1]); sub(P.w2, xb[2]); sub(P.w2, xb[N - 2]); sub(P.w1, xb[N - 1]);
+        } else {
+            const double cf[5] = {P.w2, P.w1, P.w3, P.w1, P.w2};
+            for (int d = -2; d <= 2; d++) {
+                int jj = i + d;
+                jj = jj < 0 ? jj + N : (jj >= N ? jj - N : jj);
+                if (jj == 0) continue;
+                sub(cf[d + 2], xb[jj]);
+            }
+        }
+        r[k] = sh + sl;
+    }
+    __syncthreads();
+}
+
+template <int CH>
+__global__ __launch_bounds__(PT) void k_poisson_small(PoissonArgs P) {
+    const vx_field_state &F = P.fs;
+    const int N = F.n_f;
+    const int t = threadIdx.x, base = t * CH;
+    __shared__ double wb[SMALL_MAX];  // neighbour exchange (residual and convolution phases never overlap)
+    __shared__ double part[2 * PW + 2];
+    double *xb = wb, *yb = wb;
+    double b[CH], x[CH], r[CH], c[CH];
+    const double te = vx_eps0_inv;
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        int i = base + k;
+        double v = 0.0;
+        if (i < N) {
+            v = te * (F.charge[i] + F.neutral[i]);
+            v *= F.dx2;
+        }
+        b[k] = v;
+    }
+    approx_small<CH>(P, b, x, yb, part);
+    for (int it = 0; it < 3; it++) {
+        residual_small<CH>(P, b, x, r, xb);
+        approx_small<CH>(P, r, c, yb, part);
+#pragma unroll
+        for (int k = 0; k < CH; k++) x[k] = x[k] + c[k];
+    }
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        if (base + k < N) {
+            xb[base + k] = x[k];
+            F.phi[base + k] = x[k];
+        }
+    __syncthreads();
+    double ex0 = *F.Ex0;
+    double e_m1 = efield(xb, N, F.field_coef, ex0, -1);
+    double e_0 = efield(xb, N, F.field_coef, ex0, 0);
+    double ex0n = ex0 + -(e_m1 + e_0) * 0.5;
+    __syncthreads();
+    if (t == 0) *F.Ex0 = ex0n;
+#pragma unroll
+    for (int k = 0; k < CH; k++)
+        if (base + k < N) F.E[base + k] = efield(xb, N, F.field_coef, ex0n, base + k);
+}
+
 // ---------------------------------------------------------------- Maxwell
 struct MaxArgs {
     vx_field_state fs;
@@ -361,7 +563,15 @@ int vx_poisson(vx_ctx *ctx, const vx_field_state *fs) {
         pw *= rho;
     }
     P.invN = 1.0 / fs->n_f;
-    k_poisson<<<1, PT, 0, ctx->stream>>>(P);
+    LaunchTimer lt(ctx, "k_poisson");
+    int ch = (fs->n_f + PT - 1) / PT;
+    if (fs->n_f <= SMALL_MAX) {
+        if (ch <= 1) k_poisson_small<1><<<1, PT, 0, ctx->stream>>>(P);
+        else if (ch <= 2) k_poisson_small<2><<<1, PT, 0, ctx->stream>>>(P);
+        else k_poisson_small<4><<<1, PT, 0, ctx->stream>>>(P);
+    } else {
+        k_poisson<<<1, PT, 0, ctx->stream>>>(P);
+    }
     return check_launch(ctx, "k_poisson");
 }
 

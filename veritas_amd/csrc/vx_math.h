@@ -39,16 +39,20 @@ VX_HD double vx_log(double x) {
     const double ln2_lo = 1.9082149292705877e-10;  // ln 2 - ln2_hi
     uint64_t b = vx_double_to_bits(x);
     int e = (int)((b >> 52) & 0x7ff);
-    if (!(x > 0.0) || e == 0x7ff || e == 0) {
-        // zero, negative, subnormal, inf, nan: not produced by the method;
-        // give the IEEE-style answers without claiming accuracy.
+    int kbias = 1023;
+    if (!(x > 0.0) || e == 0x7ff) {
+        // zero, negative, inf, nan: not produced by the method; give the
+        // IEEE-style answers without claiming accuracy.
         if (x == 0.0) return -vx_bits_to_double(0x7ff0000000000000ULL);
         if (x < 0.0 || x != x) return vx_bits_to_double(0x7ff8000000000000ULL);
-        if (e == 0x7ff) return x;
-        // subnormal: scale up by 2^54 and correct
-        return vx_log(x * 18014398509481984.0) - 54.0 * 0.6931471805599453;
+        return x;
     }
-    int k = e - 1023;
+    if (e == 0) {  // subnormal: scale by 2^54 (exact) and correct the exponent
+        b = vx_double_to_bits(x * 18014398509481984.0);
+        e = (int)((b >> 52) & 0x7ff);
+        kbias = 1023 + 54;
+    }
+    int k = e - kbias;
     // mantissa in [1, 2)
     double m = vx_bits_to_double((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL);
     if (m > 1.4142135623730951) {  // move to [sqrt(1/2), sqrt(2))

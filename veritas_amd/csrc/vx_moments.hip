@@ -103,24 +103,34 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         asq[ch] = P.q * P.q * cell_a2(A.fl, (long)(i + P.x_pos) * RTB + k);
     }
     double acc = 0.0;  // lanes < NCH: charge chain; lanes 32..32+NCH-1: current chain
+    // sub-cell value of chain ch at p-cell j (Rectangle.cpp:183-204, 258-274)
+    auto subcell = [&](int ch, int j) -> double {
+        int c = ch / RTB, k = ch % RTB;
+        int i = i0 + c;
+        if (i >= P.n_x || j >= P.n_p + 2) return 0.0;
+        if (RTB == 1) {
+            // interpolant coefficients and correction are exactly 0 for
+            // relativeToBottom == 1: the sub-cell value is the cell value
+            return g[pidx(pitch, i, j)];
+        }
+        return rel_value(A, g, pitch, i, j, k, rel_correction(A, g, pitch, i, j));
+    };
+    // chunk prefetch: values for p in [jb-1, jb+65) held in registers one
+    // chunk ahead so the global-memory latency overlaps the ordered sums
+    double pre[NCH][2];
+    for (int ch = 0; ch < NCH; ch++) {
+        pre[ch][0] = subcell(ch, -1 + lane);
+        pre[ch][1] = lane < 2 ? subcell(ch, 63 + lane) : 0.0;
+    }
     for (int jb = 0; jb < P.n_p; jb += 64) {
-        // sub-cell values for p in [jb-1, jb+65)
         for (int ch = 0; ch < NCH; ch++) {
-            int c = ch / RTB, k = ch % RTB;
-            int i = i0 + c;
-            for (int e = lane; e < 66; e += 64) {
-                int j = jb - 1 + e;
-                double v = 0.0;
-                if (i < P.n_x && j < P.n_p + 2) {
-                    if (RTB == 1) {
-                        // interpolant coefficients and correction are exactly 0 for
-                        // relativeToBottom == 1: the sub-cell value is the cell value
-                        v = g[pidx(pitch, i, j)];
-                    } else {
-                        v = rel_value(A, g, pitch, i, j, k, rel_correction(A, g, pitch, i, j));
-                    }
-                }
-                S[wave][ch][e] = v;
+            S[wave][ch][lane] = pre[ch][0];
+            if (lane < 2) S[wave][ch][64 + lane] = pre[ch][1];
+        }
+        if (jb + 64 < P.n_p) {
+            for (int ch = 0; ch < NCH; ch++) {
+                pre[ch][0] = subcell(ch, jb + 63 + lane);
+                pre[ch][1] = lane < 2 ? subcell(ch, jb + 127 + lane) : 0.0;
             }
         }
         // logarithm terms, computed once per distinct A^2 among the chains

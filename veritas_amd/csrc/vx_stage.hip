@@ -75,6 +75,7 @@ __device__ __forceinline__ double col_qE(const StageArgs &A, const double *E, in
     return A.p.q * t;
 }
 
+template <int STEP>
 __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     const vx_patch &P = A.p;
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
@@ -93,14 +94,13 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
                            (jw == 0 && j == -1);
     const bool own_row_p = (lane >= HALO && lane < 64 - HALO && j < np) ||
                            (j == np && lane >= HALO && lane < 65 - HALO);
-    const bool store_flux = A.step < 5;
+    constexpr bool store_flux = STEP < 5;
 
     // reference loop ranges in j (Rectangle.cpp:1279-1394) for this lane
     const bool jr_ex = j >= -1 && j < np + 1;   // ex, fx, FxH, FxL rows
     const bool jr_ep = j >= -1 && j < np + 2;   // ep
     const bool jr_fp = j >= 0 && j < np + 1;    // fp, FpH, FpL p-faces
     const bool jr_R = j >= -1 && j < np + 1;    // R
-    const bool jr_ns = j >= -2 && j < np + 2;   // FxDS/FpDS
     // update loop ranges (walls on all sides: xm = 1, xp = n_x, pm = 1, pp = n_p)
     const int xm = P.left ? 1 : 0, xp = P.right ? nx : nx + 1;
     const int pm = P.down ? 1 : 0, pp = P.up ? np : np + 1;
@@ -130,16 +130,44 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     double G_m1 = gamma_(pj, as_m1, mc2);
     double G_0 = (as_0 == as_m1) ? G_m1 : gamma_(pj, as_0, mc2);
     double as0_0 = face_as(A, A.a20, r0);
-    double G0_0 = (as0_0 == as_0) ? G_0 : gamma_(pj, as0_0, mc2);
+    double G0_0 = (STEP == 0 || as0_0 == as_0) ? G_0 : gamma_(pj, as0_0, mc2);
+
+    // software prefetch: rows r and r+1 are in flight before iteration r, the
+    // stored fluxes of x-face / column r-1 one iteration ahead
+    double pfs_a = ldp(fin, pitch, nx, np, r0, j), pf0_a = ldp(f0p, pitch, nx, np, r0, j);
+    double pfs_b = ldp(fin, pitch, nx, np, r0 + 1, j), pf0_b = ldp(f0p, pitch, nx, np, r0 + 1, j);
+    constexpr int NH = STEP > 0 ? STEP : 1;  // stored flux planes read by this stage
+    double hx[NH], hp[NH];
+    auto load_h = [&](int face) {
+        const bool vx_ = face >= 0 && face < nx + 1 && jr_ex;
+        const bool vp_ = face >= -1 && face < nx + 1 && jr_fp;
+#pragma unroll
+        for (int k = 0; k < NH; k++) {
+            hx[k] = (k < STEP && vx_) ? P.FxH[k][pidx(pitch, face, j)] : 0.0;
+            hp[k] = (k < STEP && vp_) ? P.FpH[k][pidx(pitch, face, j)] : 0.0;
+        }
+    };
+    load_h(r0 - 1);
 
     for (int r = r0; r <= i1 + HALO - 1; r++) {
         // ---- ingest row r
-        fs_0 = ldp(fin, pitch, nx, np, r, j);
-        f0_0 = ldp(f0p, pitch, nx, np, r, j);
+        fs_0 = pfs_a;
+        f0_0 = pf0_a;
+        pfs_a = pfs_b;
+        pf0_a = pf0_b;
+        pfs_b = ldp(fin, pitch, nx, np, r + 2, j);
+        pf0_b = ldp(f0p, pitch, nx, np, r + 2, j);
+        double hxc[NH], hpc[NH];
+#pragma unroll
+        for (int k = 0; k < NH; k++) {
+            hxc[k] = hx[k];
+            hpc[k] = hp[k];
+        }
+        load_h(r);
         double as_p1 = face_as(A, A.a2, r + 1);
         double G_p1 = (as_p1 == as_0) ? G_0 : gamma_(pj, as_p1, mc2);
         double as0_p1 = face_as(A, A.a20, r + 1);
-        double G0_p1 = (as0_p1 == as_p1) ? G_p1 : gamma_(pj, as0_p1, mc2);
+        double G0_p1 = (STEP == 0 || as0_p1 == as_p1) ? G_p1 : gamma_(pj, as0_p1, mc2);
 
         // ---- x-face r-1: ex, fx (Rectangle.cpp:1278-1293), FxH_s (1313-1334)
         const int ix = r - 1;
@@ -162,8 +190,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         if (store_flux) {
             const bool own_x = ix >= i0 && (ix < i0 + A.TI || last_strip);
             const bool own_c = (ix >= i0 || i0 == 0) && (ix < i0 + A.TI || last_strip);
-            if (vFx && own_x && own_row_x) P.FxH[A.step][pidx(pitch, ix, j)] = FxH;
-            if (vFp && own_c && own_row_p) P.FpH[A.step][pidx(pitch, ix, j)] = FpH;
+            if (vFx && own_x && own_row_x) P.FxH[STEP][pidx(pitch, ix, j)] = FxH;
+            if (vFp && own_c && own_row_p) P.FpH[STEP][pidx(pitch, ix, j)] = FpH;
         }
 
         // ---- low-order fluxes of stage 0 at x-face r and column r
@@ -197,14 +225,14 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         // ---- x-face r-1 and column r-1: RK flux combination (Rectangle.cpp:1396-1517)
         {
             double xLS = A.aSum * FxL_m1, pLS = A.aSum * FpL_m1;
-            const bool nsx = ix >= -2 && ix < nx + 2 && jr_ns;
-            double sx = A.a[0] * (A.step == 0 ? FxH : ((vFx && nsx) ? P.FxH[0][pidx(pitch, ix, j)] : 0.0));
-            double sp = A.a[0] * (A.step == 0 ? FpH : ((vFp && nsx) ? P.FpH[0][pidx(pitch, ix, j)] : 0.0));
-            for (int k = 1; k < A.ns; k++) {
-                double hx = (k == A.step) ? FxH : ((vFx && nsx) ? P.FxH[k][pidx(pitch, ix, j)] : 0.0);
-                double hp = (k == A.step) ? FpH : ((vFp && nsx) ? P.FpH[k][pidx(pitch, ix, j)] : 0.0);
-                sx = sx + A.a[k] * hx;
-                sp = sp + A.a[k] * hp;
+            double sx = A.a[0] * (STEP == 0 ? FxH : hxc[0]);
+            double sp = A.a[0] * (STEP == 0 ? FpH : hpc[0]);
+#pragma unroll
+            for (int k = 1; k <= STEP; k++) {
+                double hxk = (k == STEP) ? FxH : hxc[k < NH ? k : 0];
+                double hpk = (k == STEP) ? FpH : hpc[k < NH ? k : 0];
+                sx = sx + A.a[k] * hxk;
+                sp = sp + A.a[k] * hpk;
             }
             FxD_m1 = sx - xLS;
             FpD_m1 = sp - pLS;
@@ -309,7 +337,14 @@ int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     A.nstrip = (p->n_x + A.TI - 1) / A.TI;
     dim3 grid((unsigned)((p->n_p + OUTW - 1) / OUTW), (unsigned)((A.nstrip + 3) / 4));
     LaunchTimer lt(ctx, "k_stage");
-    k_stage<<<grid, 256, 0, ctx->stream>>>(A);
+    switch (step) {
+        case 0: k_stage<0><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 1: k_stage<1><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 2: k_stage<2><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 3: k_stage<3><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 4: k_stage<4><<<grid, 256, 0, ctx->stream>>>(A); break;
+        default: k_stage<5><<<grid, 256, 0, ctx->stream>>>(A); break;
+    }
     return check_launch(ctx, "k_stage");
 }
 
