@@ -21,6 +21,24 @@ __device__ __forceinline__ double momentum(double pmin, double dp, int j, int p_
     return pmin + dp * ((double)j + (double)p_pos);
 }
 
+// Rectangle::GetEfield(i) (Rectangle.cpp:1055-1067): mean of the finest-grid E
+// over the rtb finest cells of patch column i.  EMFieldSolver::GetEfield
+// (EMSolver.cpp:137-154) wraps each stencil index once, which equals the
+// periodic index for finest indices in [-n_f, 2 n_f); callers check that the
+// patch lies inside the finest grid, so rtb * (i + x_pos) + k stays there for
+// the columns i in [-2, n_x + 2) the stage reads.
+__device__ __forceinline__ double patch_E(const double *E, int n_f, int rtb, int x_pos, int i) {
+    const int j = rtb * (i + x_pos);
+    double t = 0.0;
+    for (int k = 0; k < rtb; k++) {
+        int m = j + k;
+        m = m < 0 ? m + n_f : (m >= n_f ? m - n_f : m);
+        t += E[m];
+    }
+    t *= (1.0 / (double)rtb);
+    return t;
+}
+
 // Rectangle.hpp:181-183 with mc2inv = (m_inv*c_inv)*(m_inv*c_inv)
 __device__ __forceinline__ double gamma_(double p, double a2, double mc2inv) {
     return sqrt(1.0 + ((p * p) + a2) * mc2inv);

@@ -43,15 +43,7 @@ __device__ __forceinline__ double face_as(const Sub0Args &A, int i) {
 // Rectangle::GetEfield(i) (Rectangle.cpp:1055-1067); E is periodic on the
 // finest grid (EMSolver.cpp:137-154 wraps every stencil index).
 __device__ __forceinline__ double patch_E(const vx_patch &p, const vx_fields &fl, int i) {
-    long j = (long)p.rtb * (long)(i + p.x_pos);
-    double t = 0.0;
-    for (int k = 0; k < p.rtb; k++) {
-        long m = (j + k) % fl.n_f;
-        if (m < 0) m += fl.n_f;
-        t += fl.E[m];
-    }
-    t *= (1.0 / (double)p.rtb);
-    return t;
+    return vxd::patch_E(fl.E, fl.n_f, p.rtb, p.x_pos, i);
 }
 
 // Sub-step 0: fluxes of stage `step`, RK combination.  One 256-thread block
@@ -77,8 +69,8 @@ __global__ __launch_bounds__(256) void k_sub0(Sub0Args A) {
     }
     if (tid < TI + 3) AS[tid] = face_as(A, i0 - 1 + tid);
     if (tid >= 64 && tid < 64 + TI + 2) {
-        int r = tid - 64;
-        EM[r] = P.q * patch_E(P, A.fl, i0 - 1 + r);
+        int r = tid - 64, i = i0 - 1 + r;
+        EM[r] = (i >= -1 && i < nx + 1) ? P.q * patch_E(P, A.fl, i) : 0.0;  // columns ep reads
     }
     __syncthreads();
     for (int e = tid; e < (TI + 3) * (TJ + 3); e += 256) {
@@ -308,6 +300,7 @@ int vx_fct_sub0(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fi
     int rc = validate_patch(ctx, p);
     if (rc) return rc;
     if (step < 0 || step > 5 || !fl) return fail(ctx, VX_EINVAL, "sub0: bad step %d", step);
+    if ((rc = validate_span(ctx, p, fl->n_f))) return rc;
     Sub0Args A;
     A.p = *p;
     A.fl = *fl;

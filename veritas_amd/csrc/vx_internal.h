@@ -85,4 +85,13 @@ inline int validate_patch(vx_ctx *ctx, const vx_patch *p) {
     return VX_OK;
 }
 
+// A patch lies inside the finest grid of n_f cells (Mesh::updateHierarchy only
+// builds such patches); the stage kernels wrap finest indices once on that basis.
+inline int validate_span(vx_ctx *ctx, const vx_patch *p, int n_f) {
+    if (n_f < 1 || p->x_pos < 0 || (long)p->rtb * ((long)p->x_pos + (long)p->n_x) > (long)n_f)
+        return fail(ctx, VX_EINVAL, "patch x range [%d, %d) x %d outside the finest grid of %d cells",
+                    p->x_pos, p->x_pos + p->n_x, p->rtb, n_f);
+    return VX_OK;
+}
+
 }  // namespace vxi

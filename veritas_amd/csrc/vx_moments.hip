@@ -234,6 +234,8 @@ int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
     int rc = validate_patch(ctx, p);
     if (rc) return rc;
     if (p->rtb > MAX_RTB) return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d > %d", p->rtb, MAX_RTB);
+    if (!fl) return fail(ctx, VX_EINVAL, "moments: null fields");
+    if ((rc = validate_span(ctx, p, fl->n_f))) return rc;
     MomArgs A;
     A.p = *p;
     A.fl = *fl;

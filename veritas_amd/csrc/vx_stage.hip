@@ -48,8 +48,21 @@ struct StageArgs {
     int nstrip;   // strips in x
 };
 
-__device__ __forceinline__ double shdn(double v, int d) { return __shfl_down(v, d, 64); }
-__device__ __forceinline__ double shup(double v, int d) { return __shfl_up(v, d, 64); }
+// Whole-wavefront lane shifts by one lane (DPP wave_shl:1 / wave_shr:1, plain
+// VALU moves instead of LDS permutes): shdn gives lane l the value of lane
+// l + 1, shup the value of lane l - 1.  The end lane (63, resp. 0) receives 0;
+// it is a halo lane, as are its neighbours that consume it.
+template <int CTRL>
+__device__ __forceinline__ double wave_shift1(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+    const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
+    return __hiloint2double(rhi, rlo);
+}
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+__device__ __forceinline__ double shdn1(double v) { return wave_shift1<DPP_WAVE_SHL1>(v); }
+__device__ __forceinline__ double shup1(double v) { return wave_shift1<DPP_WAVE_SHR1>(v); }
 
 __device__ __forceinline__ double ldp(const double *pl, int pitch, int nx, int np, int i, int j) {
     return (i >= -2 && i < nx + 2 && j >= -2 && j < np + 2) ? pl[pidx(pitch, i, j)] : 0.0;
@@ -57,22 +70,14 @@ __device__ __forceinline__ double ldp(const double *pl, int pitch, int nx, int n
 
 // q^2 A^2 at x-face i (Rectangle.cpp:1280, 1296-1297; EMSolver.cpp:133-135)
 __device__ __forceinline__ double face_as(const StageArgs &A, const double *a2, int i) {
-    long fi = (long)A.p.rtb * (long)(i + A.p.x_pos);
-    long c = fi < 0 ? 0 : (fi > A.n_f ? A.n_f : fi);
+    int fi = A.p.rtb * (i + A.p.x_pos);
+    int c = fi < 0 ? 0 : (fi > A.n_f ? A.n_f : fi);
     return A.p.q * A.p.q * a2[c];
 }
 
 // q * Rectangle::GetEfield(i) (Rectangle.cpp:1055-1067, 1298)
 __device__ __forceinline__ double col_qE(const StageArgs &A, const double *E, int i) {
-    long j = (long)A.p.rtb * (long)(i + A.p.x_pos);
-    double t = 0.0;
-    for (int k = 0; k < A.p.rtb; k++) {
-        long m = (j + k) % A.n_f;
-        if (m < 0) m += A.n_f;
-        t += E[m];
-    }
-    t *= (1.0 / (double)A.p.rtb);
-    return A.p.q * t;
+    return A.p.q * patch_E(E, A.n_f, A.p.rtb, A.p.x_pos, i);
 }
 
 template <int STEP>
@@ -80,7 +85,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     const vx_patch &P = A.p;
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
     const int lane = threadIdx.x & 63;
-    const int strip = blockIdx.y * 4 + (threadIdx.x >> 6);
+    // the wavefront index is uniform: row loop, row ranges and field reads stay scalar
+    const int strip = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (strip >= A.nstrip) return;  // uniform per wavefront
     const int i0 = strip * A.TI;
     const int i1 = min(i0 + A.TI, nx);       // output rows [i0, i1)
@@ -171,18 +177,18 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
 
         // ---- x-face r-1: ex, fx (Rectangle.cpp:1278-1293), FxH_s (1313-1334)
         const int ix = r - 1;
-        double Gd = shdn(G_m1, 1);  // Gamma at p-face j+1
+        double Gd = shdn1(G_m1);  // Gamma at p-face j+1
         double ex = (ix >= -1 && ix < nx + 2 && jr_ex) ? dpcc * (Gd - G_m1) : 0.0;
         double fx = (ix >= 0 && ix < nx + 1 && jr_ex) ? weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0) : 0.0;
-        double ex_u = shdn(ex, 1), ex_d = shup(ex, 1);
-        double fx_u = shdn(fx, 1), fx_d = shup(fx, 1);
+        double ex_u = shdn1(ex), ex_d = shup1(ex);
+        double fx_u = shdn1(fx), fx_d = shup1(fx);
         const bool vFx = ix >= 0 && ix < nx + 1 && jr_ex;
         double FxH = vFx ? P.dx_inv * (fx * ex + w3 * (fx_u - fx_d) * (ex_u - ex_d)) : 0.0;
 
         // ---- column r: ep, fp (Rectangle.cpp:1294-1312)
         double qE = (r >= -1 && r < nx + 1) ? col_qE(A, A.E, r) : 0.0;
         ep_0 = (r >= -1 && r < nx + 1 && jr_ep) ? qE - ccdx * (G_p1 - G_0) : 0.0;
-        double s2 = shup(fs_0, 2), s1 = shup(fs_0, 1), s3 = shdn(fs_0, 1);
+        double s1 = shup1(fs_0), s2 = shup1(s1), s3 = shdn1(fs_0);
         fp_0 = (r >= -1 && r < nx + 1 && jr_fp) ? weno(s2, s1, fs_0, s3, ep_0 > 0.0) : 0.0;
         // ---- column r-1: FpH_s (Rectangle.cpp:1353-1375)
         const bool vFp = ix >= -1 && ix < nx + 1 && jr_fp;
@@ -196,18 +202,18 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
 
         // ---- low-order fluxes of stage 0 at x-face r and column r
         // (Rectangle.cpp:1335-1352, 1376-1394 evaluated with the stage-0 fields)
-        double G0d = shdn(G0_0, 1);
+        double G0d = shdn1(G0_0);
         double ex0 = dpcc * (G0d - G0_0);
         FxL_0 = (r >= 0 && r < nx + 1 && jr_ex) ? P.dx_inv * ((ex0 > 0.0 ? f0_m1 : f0_0) * ex0) : 0.0;
         double qE0 = (r >= -1 && r < nx + 1) ? col_qE(A, A.E0, r) : 0.0;
         double ep0 = qE0 - ccdx * (G0_p1 - G0_0);
-        double f0_jm1 = shup(f0_0, 1);
+        double f0_jm1 = shup1(f0_0);
         FpL_0 = (r >= -1 && r < nx + 1 && jr_fp) ? P.dp_inv * ((ep0 > 0.0 ? f0_jm1 : f0_0) * ep0) : 0.0;
 
         // ---- cell r-1: low-order predictor (Rectangle.cpp:1518-1534, gather form)
         {
             const int i = r - 1;
-            double FpL_up = shdn(FpL_m1, 1);  // FpL0 at (i, j+1)
+            double FpL_up = shdn1(FpL_m1);  // FpL0 at (i, j+1)
             double t = 0.0;
             if (i >= 0 && i < nx && j >= 0 && j < np) {
                 t = f0_m1;
@@ -241,9 +247,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         // ---- cell r-2: Zalesak R+- (Rectangle.cpp:1538-1579)
         {
             const int i = r - 2;
-            double FpDu = shdn(FpD_m2, 1);  // FpDS at (i, j+1)
-            double f0u = shdn(f0_m2, 1), f0d = shup(f0_m2, 1);
-            double f2u = shdn(f2_m2, 1), f2d = shup(f2_m2, 1);
+            double FpDu = shdn1(FpD_m2);  // FpDS at (i, j+1)
+            double f0u = shdn1(f0_m2), f0d = shup1(f0_m2);
+            double f2u = shdn1(f2_m2), f2d = shup1(f2_m2);
             double rp = 0.0, rm = 0.0;
             if (i >= -1 && i < nx + 1 && jr_R) {
                 double Pp = valmax(0.0, FxD_m2) - valmin(0.0, FxD_m1) + valmax(0.0, FpD_m2) - valmin(0.0, FpDu);
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         // corrected value (Rectangle.cpp:1596-1612, gather form)
         {
             const int i = r - 3;
-            double Rpd = shup(Rp_m3, 1), Rmd = shup(Rm_m3, 1);  // R at (i, j-1)
+            double Rpd = shup1(Rp_m3), Rmd = shup1(Rm_m3);  // R at (i, j-1)
             const bool jint = j >= 0 && j < np;
             double cxl = (i >= 1 && i < nx && jint)
                              ? (FxD_m3 > 0.0 ? valmin(Rp_m3, Rm_m4) : valmin(Rp_m4, Rm_m3)) : 1.0;
@@ -273,8 +279,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
                              ? (FxD_m2 > 0.0 ? valmin(Rp_m2, Rm_m3) : valmin(Rp_m3, Rm_m2)) : 1.0;
             double cpl = (i >= 1 && i < nx && jint)
                              ? (FpD_m3 > 0.0 ? valmin(Rp_m3, Rmd) : valmin(Rpd, Rm_m3)) : 1.0;
-            double cpu = shdn(cpl, 1);      // C at (i, j+1), computed by the next lane
-            double FpDu = shdn(FpD_m3, 1);  // FpDS at (i, j+1)
+            double cpu = shdn1(cpl);      // C at (i, j+1), computed by the next lane
+            double FpDu = shdn1(FpD_m3);  // FpDS at (i, j+1)
             if (out_lane && i >= i0 && i < i1) {
                 double t = f2_m3;
                 if (i >= xm && i < xp && j >= pm && j < pp) {
@@ -312,6 +318,7 @@ int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     if (rc) return rc;
     if (step < 0 || step > 5 || !fl || !fl->E0 || !fl->a20)
         return fail(ctx, VX_EINVAL, "stage_fused: bad step %d or missing stage-0 fields", step);
+    if ((rc = validate_span(ctx, p, fl->n_f))) return rc;
     if (!(p->left && p->right && p->down && p->up))
         return fail(ctx, VX_EINVAL, "stage_fused: patch must touch all four domain edges");
     if (fin == p->f[0] && step != 0)
