@@ -42,6 +42,18 @@ def kernel_bytes_per_cell(kernel, step):
     raise ValueError(kernel)
 
 
+def measured_traffic(kernel, nx, np_):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-kernel calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic_two_stream_{nx}.md")
+    if nx != np_ or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(params, steps):
     """Time the CPU oracle (test infrastructure, used here only as the
     baseline) on `steps` steps of the same workload."""
@@ -116,7 +128,8 @@ def main():
                    "wall_s": wall},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                     "traffic": None, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "traffic": measured_traffic(kernel, params.nx, params.np),
+                     "algorithmic_bytes_per_launch": bytes_per_launch,
                      "avg_launch_ms": kms / nlaunch if nlaunch else None,
                      "launches": nlaunch, "kernel_share": kms / ms if ms else None},
     }

@@ -46,6 +46,7 @@ struct StageArgs {
     double a[6], aSum;
     int TI;       // output rows per strip
     int nstrip;   // strips in x
+    int nbx;      // tiles along p
 };
 
 // Whole-wavefront lane shifts by one lane (DPP wave_shl:1 / wave_shr:1, plain
@@ -86,12 +87,20 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
     const int lane = threadIdx.x & 63;
     // the wavefront index is uniform: row loop, row ranges and field reads stay scalar
-    const int strip = blockIdx.y * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // XCD-aware tile order: the dispatcher hands workgroup w to XCD w mod 8, so
+    // renumber workgroups so that each XCD receives a contiguous run of tiles;
+    // p-neighbouring tiles then march over the same rows on the same XCD and
+    // share the cache lines of their overlapping halos in that XCD's L2.
+    const int nwg = gridDim.x, w = blockIdx.x;
+    const int xcd = w & 7, q = nwg >> 3, rem = nwg & 7;
+    const int tile = xcd * q + min(xcd, rem) + (w >> 3);
+    const int bx = tile % A.nbx, by = tile / A.nbx;
+    const int strip = by * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (strip >= A.nstrip) return;  // uniform per wavefront
     const int i0 = strip * A.TI;
     const int i1 = min(i0 + A.TI, nx);       // output rows [i0, i1)
     const bool last_strip = i0 + A.TI >= nx;
-    const int jw = blockIdx.x * OUTW;
+    const int jw = bx * OUTW;
     const int j = jw - HALO + lane;
     const bool out_lane = lane >= HALO && lane < 64 - HALO && j < np;
     // faces this wavefront stores (each valid face has exactly one owner)
@@ -342,7 +351,8 @@ int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     for (int k = 1; k < A.ns; k++) A.aSum = A.aSum + A.a[k];
     A.TI = p->n_x >= 512 ? 64 : (p->n_x >= 128 ? 32 : 16);
     A.nstrip = (p->n_x + A.TI - 1) / A.TI;
-    dim3 grid((unsigned)((p->n_p + OUTW - 1) / OUTW), (unsigned)((A.nstrip + 3) / 4));
+    A.nbx = (p->n_p + OUTW - 1) / OUTW;
+    dim3 grid((unsigned)(A.nbx * ((A.nstrip + 3) / 4)));
     LaunchTimer lt(ctx, "k_stage");
     switch (step) {
         case 0: k_stage<0><<<grid, 256, 0, ctx->stream>>>(A); break;
