@@ -173,6 +173,83 @@ int vx_neutralize(vx_ctx *ctx, const vx_field_state *fs);
 int vx_cfl_bound(vx_ctx *ctx, const vx_field_state *fs, int n_species, const double *q,
                  const double *m, const double *dp_finest, double *out);
 
+/* ---- adaptive mesh: cell-task programs over a table of patches ----
+ *
+ * The reference fills ghost cells, restricts, synchronises the limiter C and
+ * substitutes coarse-fine fluxes with per-rectangle loops over neighbour
+ * pointers (Rectangle.cpp:314-614, 892-941, 1032-1253, 1625-1960; Level.cpp:
+ * 88-150; Mesh.cpp:91-130, 904-917).  Here the host walks the same loops once
+ * per hierarchy and records each cell or face operation as a task; a program
+ * is a list of tasks cut into batches so that no task in a batch reads or
+ * writes what another task of the same batch writes (the batches replay the
+ * reference's sequential order exactly).  Each batch is one launch. */
+typedef struct vx_patch_table vx_patch_table;
+/* Device copy of n patch descriptors; tasks name patches by index. */
+int vx_patch_table_create(vx_ctx *ctx, const vx_patch *patches, int32_t n, vx_patch_table **out);
+/* Refresh the descriptors (same count) after plane pointers changed. */
+int vx_patch_table_update(vx_ctx *ctx, vx_patch_table *t, const vx_patch *patches, int32_t n);
+void vx_patch_table_destroy(vx_patch_table *t);
+
+typedef struct vx_cell_task {
+    int32_t op;        /* VX_OP_* */
+    int32_t dst;       /* patch written */
+    int32_t di, dj;    /* patch-local cell or face of dst */
+    int32_t src;       /* patch read */
+    int32_t si, sj;    /* op-specific source coordinates (below) */
+    int32_t aux;       /* op-specific */
+} vx_cell_task;
+
+enum {
+    /* f slot `slot` of dst at (di, dj) = ... (mode VX_MODE_GHOST), or slots 0
+     * and 1 of dst = ... read from slot 1 of src (mode VX_MODE_TRANSFER) */
+    VX_OP_ZERO = 0,     /* 0 (domain-edge sentinel, BoundaryCondition.cpp:6-8) */
+    VX_OP_COPY = 1,     /* src at patch-local (si, sj) (GetValueFromSameLevel, Rectangle.cpp:307-312) */
+    VX_OP_COARSE = 2,   /* element aux (= k * r + l) of src's conservative interpolation of
+                           finest-level-of-src+1 cell (si, sj) (GetWenoValueFromCoarseLevel,
+                           Rectangle.cpp:343-415) */
+    VX_OP_RESTRICT = 3, /* mean of src's r x r cells under coarse cell (si, sj) given in the
+                           coarse level's coordinates (GetValueFromFinerLevel, Rectangle.cpp:314-327) */
+    /* limiter C (Rectangle.cpp:1625-1915); aux = t, the face type 0..3 */
+    VX_OP_C_SAME = 4,   /* one face: dst = neighbour (di, dj), src = caller (si, sj) (SetCFromSameLevel) */
+    VX_OP_C_DIFF = 5,   /* dst = coarse (di, dj) = (i_coarse, j_coarse), src = finer caller with
+                           (si, sj) = (i_caller, j_caller) (SetCFromDifferentLevel) */
+    VX_OP_C_UPDATE = 6, /* one face: min of both sides (UpdateCFromSameLevel) */
+    /* coarse face (di, dj) of dst takes the finer patch src's fluxes over the r
+     * faces starting at (si, sj) (CalculateFluxToCoarse*, Rectangle.hpp:132-178)
+     * and its antidiffusive flux is recombined; bit k of aux set: finer face k
+     * is itself a coarse-fine face, whose stored low-order flux is used */
+    VX_OP_FLUX_X = 7,
+    VX_OP_FLUX_P = 8
+};
+enum { VX_MODE_GHOST = 0, VX_MODE_TRANSFER = 1 };
+
+typedef struct vx_program vx_program;
+/* Upload tasks; batch_start[b] is the first task of batch b (n_batches + 1
+ * entries, the last = n_tasks). */
+int vx_program_create(vx_ctx *ctx, const vx_cell_task *tasks, int32_t n_tasks,
+                      const int32_t *batch_start, int32_t n_batches, vx_program **out);
+void vx_program_destroy(vx_program *prog);
+/* Ghost-fill / restriction / regrid-transfer / limiter-C programs.  r is the
+ * refinement ratio (2 or 4). */
+int vx_program_run(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int mode,
+                   int slot, int r);
+/* Coarse-fine flux program of RK stage `step` (run after vx_fct_sub0_fluxes
+ * of every patch and before vx_fct_sub0_predictor), finest coarse level first. */
+int vx_program_run_fluxes(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog,
+                          int step, double dt, int r, const vx_fields *fl);
+
+/* Split form of vx_fct_sub0 for hierarchies: the fluxes and RK combination
+ * (Rectangle.cpp:1269-1517), then the low-order predictor (1518-1534). */
+int vx_fct_sub0_fluxes(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl);
+int vx_fct_sub0_predictor(vx_ctx *ctx, const vx_patch *p, int step, double dt);
+
+/* Refinement flags of one patch (Rectangle::getError, Rectangle.cpp:866-890,
+ * ErrorEstimate, Rectangle.hpp:128-130): flags[(i - lo_x) * (hi_p - lo_p) +
+ * (j - lo_p)] = 1 where the error estimate of patch-local cell (i, j) in
+ * [lo_x, hi_x) x [lo_p, hi_p) exceeds `criteria` (uint8 device array). */
+int vx_flag_cells(vx_ctx *ctx, const vx_patch *p, const double *weights5, double criteria,
+                  int32_t lo_x, int32_t hi_x, int32_t lo_p, int32_t hi_p, uint8_t *flags);
+
 /* ---- timing helpers (HIP events on the context stream) ---- */
 int vx_event_record(vx_ctx *ctx, int slot);
 int vx_event_elapsed_ms(vx_ctx *ctx, int slot_a, int slot_b, float *ms);

@@ -102,6 +102,12 @@ Patch::Patch(int n_x_, int n_p_, int x_pos_, int p_pos_, int depth_, Settings &s
         FpH[k].assign(NSsz, 0.0);
     }
     is_nested.assign(NSsz, 0);
+    is_interpolated.assign(NSsz, 0);
+    is_ilb_x.assign(NSsz, 0);
+    is_ilb_p.assign(NSsz, 0);
+    finer_level.assign(NSsz, nullptr);
+    finer_level_x.assign(NSsz, nullptr);
+    finer_level_p.assign(NSsz, nullptr);
     ext_xm.assign(n_p / r + 2, nullptr);
     ext_xp.assign(n_p / r + 2, nullptr);
     ext_pm.assign(n_x / r, nullptr);
@@ -113,6 +119,23 @@ Patch::Patch(int n_x_, int n_p_, int x_pos_, int p_pos_, int depth_, Settings &s
     chargeR.assign((size_t)(n_x * rtb), 0.0);
     currentR.assign((size_t)(n_x * rtb), 0.0);
     energyR.assign((size_t)(n_p * rtb), 0.0);
+    // Rectangle.cpp:68-74 (depth 0 is the finest level)
+    errorWeights.assign(5, 0.0);
+    double fmx = s.fMax.at(sp);
+    errorWeights[0] = 0.5 / fmx * std::pow(r, -0.5 * depth);
+    errorWeights[1] = 0.5 / fmx * std::pow(r, -0.5 * depth);
+    errorWeights[2] = 0.5 / fmx * std::pow(r, -2.0 * depth);
+    errorWeights[3] = 0.5 / fmx * std::pow(r, -2.0 * depth);
+    errorWeights[4] = s.sizeWeight / fmx * std::pow(r, -s.maxDepth + depth);
+    // Rectangle.cpp:91-101 sub-cell averages for one refinement step
+    interpREF.assign(3 * (size_t)r, 0.0);
+    for (int i = 0; i < r; i++) {
+        double tl = -0.5 + i / (double)r;
+        double tr = -0.5 + (i + 1.0) / (double)r;
+        interpREF[3 * i] = (tl + tr) * 0.5;
+        interpREF[3 * i + 1] = (tl * tl + tl * tr + tr * tr) / 3.0 - (1.0 / 12);
+        interpREF[3 * i + 2] = (tl * tl * tl + tl * tl * tr + tl * tr * tr + tr * tr * tr) * 0.25;
+    }
     // Rectangle.cpp:103-110 sub-cell average coefficients of the cubic
     interpREL.assign(3 * (size_t)rtb, 0.0);
     for (int i = 0; i < rtb; i++) {
@@ -137,11 +160,7 @@ double Patch::GetValueFromSameLevel(int i, int j, int val) const {
     return f[val][NS(i - x_pos, j - p_pos)];
 }
 
-// Rectangle.cpp:76-89 interpolation matrix and 139-155 (REL variant)
-static const double kInterp[12] = {
-    0.104166666666667, -0.708333333333334, 0.708333333333334, -0.104166666666667,
-    0.117647058823529, 0.029411764705882,  0.029411764705882, 0.117647058823529,
-    -0.083333333333333, 0.166666666666667, -0.166666666666667, 0.083333333333334};
+// Rectangle.cpp:139-155 (REL variant; interpolation matrix in oracle.hpp)
 
 std::vector<double> Patch::GetInterpolantsREL(double f1, double f2, double f3, double f4,
                                               double f5) const {
@@ -357,22 +376,35 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
             for (int j = 0; j < np1; j++)
                 fp[NS(i, j)] = WenoNoMax(f1[NS(i, j - 2)], f1[NS(i, j - 1)], f1[NS(i, j)],
                                          f1[NS(i, j + 1)], ep[NS(i, j)] > 0.0);
-        // 1313-1394; coarse-fine faces (finer_level_x/p) arrive with AMR
+        // 1313-1394; faces on the edge of a finer patch take the finer
+        // patch's fluxes (CalculateFluxToCoarse*, Rectangle.hpp:132-178)
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < nx1; i++)
             for (int j = -1; j < np1; j++) {
-                double am = ex[NS(i, j)], ap1 = ex[NS(i, j + 1)], am1 = ex[NS(i, j - 1)];
-                double fm = fx[NS(i, j)], fp1 = fx[NS(i, j + 1)], fm1 = fx[NS(i, j - 1)];
-                FxH[step][NS(i, j)] = dx_inv * (fm * am + w3 * (fp1 - fm1) * (ap1 - am1));
-                FxL[step][NS(i, j)] = dx_inv * ((am > 0.0 ? f1[NS(i - 1, j)] : f1[NS(i, j)]) * am);
+                if (!is_ilb_x[NS(i, j)]) {
+                    double am = ex[NS(i, j)], ap1 = ex[NS(i, j + 1)], am1 = ex[NS(i, j - 1)];
+                    double fm = fx[NS(i, j)], fp1 = fx[NS(i, j + 1)], fm1 = fx[NS(i, j - 1)];
+                    FxH[step][NS(i, j)] = dx_inv * (fm * am + w3 * (fp1 - fm1) * (ap1 - am1));
+                    FxL[step][NS(i, j)] = dx_inv * ((am > 0.0 ? f1[NS(i - 1, j)] : f1[NS(i, j)]) * am);
+                } else {
+                    Patch *fine = finer_level_x[NS(i, j)];
+                    FxH[step][NS(i, j)] = fine->CalculateFluxToCoarseX(x_pos + i, p_pos + j);
+                    FxL[step][NS(i, j)] = fine->CalculateFluxToCoarseXL(x_pos + i, p_pos + j);
+                }
             }
 #pragma omp parallel for schedule(static)
         for (int i = -1; i < nx1; i++)
             for (int j = 0; j < np1; j++) {
-                double am = ep[NS(i, j)], ap1 = ep[NS(i + 1, j)], am1 = ep[NS(i - 1, j)];
-                double fm = fp[NS(i, j)], fp1 = fp[NS(i + 1, j)], fm1 = fp[NS(i - 1, j)];
-                FpH[step][NS(i, j)] = dp_inv * (fm * am + w3 * (fp1 - fm1) * (ap1 - am1));
-                FpL[step][NS(i, j)] = dp_inv * ((am > 0.0 ? f1[NS(i, j - 1)] : f1[NS(i, j)]) * am);
+                if (!is_ilb_p[NS(i, j)]) {
+                    double am = ep[NS(i, j)], ap1 = ep[NS(i + 1, j)], am1 = ep[NS(i - 1, j)];
+                    double fm = fp[NS(i, j)], fp1 = fp[NS(i + 1, j)], fm1 = fp[NS(i - 1, j)];
+                    FpH[step][NS(i, j)] = dp_inv * (fm * am + w3 * (fp1 - fm1) * (ap1 - am1));
+                    FpL[step][NS(i, j)] = dp_inv * ((am > 0.0 ? f1[NS(i, j - 1)] : f1[NS(i, j)]) * am);
+                } else {
+                    Patch *fine = finer_level_p[NS(i, j)];
+                    FpH[step][NS(i, j)] = fine->CalculateFluxToCoarseP(x_pos + i, p_pos + j);
+                    FpL[step][NS(i, j)] = fine->CalculateFluxToCoarsePL(x_pos + i, p_pos + j);
+                }
             }
         // 1396-1517: RK combination (a_k = coefficient * dt, aSum summed in order)
         double a[6] = {0, 0, 0, 0, 0, 0};
@@ -496,64 +528,6 @@ void Patch::UpdateSameLevelBoundaries(int val) {
                 g[NS(i * r + j, n_p + 1)] = s ? s->GetValueFromSameLevel(x_pos + i * r + j, p_pos + n_p + 1, val) : 0.0;
             }
         }
-}
-
-// Rectangle.cpp:1130-1214, same-level branches; coarse-level branches arrive
-// with the multi-level hierarchy.
-void Patch::UpdateCornerPoints(int val) {
-    std::vector<double> &g = f[val];
-    auto same = [&](Patch *s, int ii, int jj) {
-        return s ? s->GetValueFromSameLevel(ii, jj, val) : 0.0;
-    };
-    {
-        int i = -1;
-        Patch *s = ext_xm[i + 1];
-        if (same_xm[i + 1])
-            for (int j = r - 2; j < r; j++) {
-                g[NS(-1, i * r + j)] = same(s, x_pos - 1, p_pos + i * r + j);
-                g[NS(-2, i * r + j)] = same(s, x_pos - 2, p_pos + i * r + j);
-            }
-        else
-            throw std::runtime_error("coarse-level corner fill not restated yet");
-    }
-    {
-        int i = (int)same_xm.size() - 2;
-        Patch *s = ext_xm[i + 1];
-        if (same_xm[i + 1])
-            for (int j = 0; j < 2; j++) {
-                g[NS(-1, i * r + j)] = same(s, x_pos - 1, p_pos + i * r + j);
-                g[NS(-2, i * r + j)] = same(s, x_pos - 2, p_pos + i * r + j);
-            }
-        else
-            throw std::runtime_error("coarse-level corner fill not restated yet");
-    }
-    {
-        int i = -1;
-        Patch *s = ext_xp[i + 1];
-        if (same_xp[i + 1])
-            for (int j = r - 2; j < r; j++) {
-                g[NS(n_x, i * r + j)] = same(s, x_pos + n_x, p_pos + i * r + j);
-                g[NS(n_x + 1, i * r + j)] = same(s, x_pos + n_x + 1, p_pos + i * r + j);
-            }
-        else
-            throw std::runtime_error("coarse-level corner fill not restated yet");
-    }
-    {
-        int i = (int)same_xp.size() - 2;
-        Patch *s = ext_xp[i + 1];
-        if (same_xp[i + 1])
-            for (int j = 0; j < 2; j++) {
-                g[NS(n_x, i * r + j)] = same(s, x_pos + n_x, p_pos + i * r + j);
-                g[NS(n_x + 1, i * r + j)] = same(s, x_pos + n_x + 1, p_pos + i * r + j);
-            }
-        else
-            throw std::runtime_error("coarse-level corner fill not restated yet");
-    }
-}
-
-void Patch::UpdateDifferentLevelBoundaries(int val) {
-    (void)val;
-    throw std::runtime_error("coarse-fine ghost fill not restated yet");
 }
 
 // ---------------------------------------------------------------- Field
@@ -948,37 +922,19 @@ void Level::InterpolateEnergyToFinestMesh(std::vector<double> &energy) {
 
 void Level::PushData(int type, int val) {  // Level.cpp:88-126
     switch (type) {
-        case 0:
-            if (!rectangles.empty()) throw std::runtime_error("restriction not restated yet");
-            break;
+        case 0: for (auto &r : rectangles) r->UpdateInterriorPoints(val); break;
         case 1: for (auto &r : rectangles) r->UpdateSameLevelBoundaries(val); break;
         case 2: for (auto &r : rectangles) r->UpdateDifferentLevelBoundaries(val); break;
         case 3: for (auto &r : rectangles) r->UpdateCornerPoints(val); break;
-        default:
-            // 4-6: limiter-C sync.  With a single patch per level every
-            // neighbour is the domain-edge sentinel, which the oracle skips.
-            if (rectangles.size() > 1) throw std::runtime_error("C sync not restated yet");
-            break;
+        case 4: for (auto &r : rectangles) r->CalculateSameBoundaryC(); break;
+        case 5: for (auto &r : rectangles) r->CalculateDifferentBoundaryC(); break;
+        case 6: for (auto &r : rectangles) r->UpdateSameBoundaryC(); break;
     }
 }
 
 // ---------------------------------------------------------------- Mesh
-// Mesh.cpp:8-50 for a single level (maxDepth == 0); hierarchy building for
-// more levels arrives with the AMR restatement.
-Mesh::Mesh(int sp_, Settings &s) : settings(s), sp(sp_) {
-    if (s.maxDepth != 0) throw std::runtime_error("oracle: multi-level meshes not restated yet");
-    int xmax = s.GetXSize(s.maxDepth), pmax = s.GetPSize(s.maxDepth, sp);
-    // promoteHierarchyToMesh(true), Mesh.cpp:794-875
-    levels.push_back(std::unique_ptr<Level>(new Level(sp, 0, s)));
-    levels[0]->rectangles.push_back(std::unique_ptr<Patch>(
-        new Patch(xmax, pmax, 0, 0, 0, s, true, true, true, true, sp)));
-    for (auto &r : levels[0]->rectangles) r->InitializeDistribution();
-    PushData();
-    for (auto &lvl : levels)
-        for (auto &r : lvl->rectangles) r->FCTTimeStep(0, -1, 3);
-}
-
-void Mesh::SetFieldSolver(Field *fld) {
+void Mesh::SetFieldSolver(Field *fld) {  // Mesh.cpp:108-114
+    em = fld;
     for (auto &lvl : levels)
         for (auto &r : lvl->rectangles) r->em = fld;
 }

@@ -77,11 +77,8 @@ void vxo_stage_fields(void *h, double dt, int step) {
 
 // Named arrays.  Patch planes (base patch of species 0) are returned with
 // `ghosts` ghost layers (0..2), row-major over (x, p).
-int vxo_get(void *h, const char *what, int ghosts, double *out, int n) {
-    Run *r = (Run *)h;
-    orc::Field &em = *r->s->em;
-    orc::Patch &p = base(r);
-    std::string w(what);
+// Plane `what` of patch p, or -1 when the name is not a plane.
+static int patch_plane(orc::Patch &p, const std::string &w, int ghosts, double *out, int n) {
     if (w == "f0") return copy_plane(p, p.f[0], ghosts, out, n);
     if (w == "f1") return copy_plane(p, p.f[1], ghosts, out, n);
     if (w == "f2") return copy_plane(p, p.f[2], ghosts, out, n);
@@ -97,6 +94,20 @@ int vxo_get(void *h, const char *what, int ghosts, double *out, int n) {
         if (w == "FxL" + std::to_string(k)) return copy_plane(p, p.FxL[k], ghosts, out, n);
         if (w == "FpL" + std::to_string(k)) return copy_plane(p, p.FpL[k], ghosts, out, n);
     }
+    if (w == "nested") {
+        std::vector<double> v(p.is_nested.begin(), p.is_nested.end());
+        return copy_plane(p, v, ghosts, out, n);
+    }
+    return -1;
+}
+
+int vxo_get(void *h, const char *what, int ghosts, double *out, int n) {
+    Run *r = (Run *)h;
+    orc::Field &em = *r->s->em;
+    orc::Patch &p = base(r);
+    std::string w(what);
+    int m = patch_plane(p, w, ghosts, out, n);
+    if (m >= 0) return m;
     if (w == "phi") return copy_out(em.PHI, out, n);
     if (w == "charge") return copy_out(em.charge, out, n);
     if (w == "J") return copy_out(em.J, out, n);
@@ -124,6 +135,48 @@ int vxo_get(void *h, const char *what, int ghosts, double *out, int n) {
     }
     return -1;
 }
+
+// Patch layout of species sp: 9 ints per patch (level with 0 the finest,
+// x_pos, p_pos, n_x, n_p, up, down, left, right), levels finest first, patches
+// in the reference's order.  Returns the number of ints.
+int vxo_layout(void *h, int sp, int *out, int n) {
+    Run *r = (Run *)h;
+    std::vector<int> v;
+    auto &levels = r->s->meshes.at(sp)->levels;
+    for (unsigned l = 0; l < levels.size(); l++)
+        for (auto &p : levels[l]->rectangles) {
+            int e[9] = {(int)l, p->x_pos, p->p_pos, p->n_x, p->n_p, p->up, p->down, p->left, p->right};
+            v.insert(v.end(), e, e + 9);
+        }
+    if (out && n >= (int)v.size()) std::memcpy(out, v.data(), sizeof(int) * v.size());
+    return (int)v.size();
+}
+
+// Plane of patch `idx` of level `lvl` (0 = finest) of species sp.
+int vxo_patch_get(void *h, int sp, int lvl, int idx, const char *what, int ghosts, double *out, int n) {
+    Run *r = (Run *)h;
+    auto &levels = r->s->meshes.at(sp)->levels;
+    if (lvl < 0 || lvl >= (int)levels.size() || idx < 0 || idx >= (int)levels[lvl]->rectangles.size())
+        return -2;
+    return patch_plane(*levels[lvl]->rectangles[idx], what, ghosts, out, n);
+}
+
+// Flags of hierarchy level lvl (0 = coarsest) from the current data
+// (Mesh::getError, Mesh.cpp:212-295), as (i, j) pairs.  Returns 2 * count.
+int vxo_flags(void *h, int sp, int lvl, int *out, int n) {
+    Run *r = (Run *)h;
+    std::vector<orc::coords> fl;
+    r->s->meshes.at(sp)->getError(lvl, false, fl);
+    if (out && n >= 2 * (int)fl.size())
+        for (size_t k = 0; k < fl.size(); k++) {
+            out[2 * k] = fl[k].first;
+            out[2 * k + 1] = fl[k].second;
+        }
+    return 2 * (int)fl.size();
+}
+
+// SolverManager::reGrid (SolverManager.cpp:48-58)
+void vxo_regrid(void *h) { ((Run *)h)->s->reGrid(); }
 
 // Standalone pieces for unit tests.
 double vxo_weno_nomax(double f1, double f2, double f3, double f4, int right) {

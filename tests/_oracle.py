@@ -45,6 +45,14 @@ def lib():
         L.vxo_face_weno.argtypes = [d, d, d, d]
         L.vxo_k1e.restype = ctypes.c_double
         L.vxo_k1e.argtypes = [ctypes.c_double]
+        i = ctypes.c_int
+        L.vxo_layout.restype = i
+        L.vxo_layout.argtypes = [vp, i, vp, i]
+        L.vxo_patch_get.restype = i
+        L.vxo_patch_get.argtypes = [vp, i, i, i, ctypes.c_char_p, i, vp, i]
+        L.vxo_flags.restype = i
+        L.vxo_flags.argtypes = [vp, i, i, vp, i]
+        L.vxo_regrid.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -102,3 +110,29 @@ class OracleRun:
         x = np.empty_like(b)
         lib().vxo_poisson_approx(self.h, b.ctypes.data, x.ctypes.data)
         return x
+
+    # ---- adaptive mesh (same accessors as veritas_amd.Run)
+    def layout(self, sp=0):
+        n = lib().vxo_layout(self.h, sp, None, 0)
+        a = (ctypes.c_int * max(n, 1))()
+        lib().vxo_layout(self.h, sp, a, n)
+        return np.array(a[:n], dtype=np.int64).reshape(-1, 9)
+
+    def patch(self, level, idx, what, ghosts=0, sp=0):
+        n = lib().vxo_patch_get(self.h, sp, level, idx, what.encode(), ghosts, None, 0)
+        if n < 0:
+            raise KeyError((level, idx, what))
+        out = np.empty(n, dtype=np.float64)
+        lib().vxo_patch_get(self.h, sp, level, idx, what.encode(), ghosts, out.ctypes.data, n)
+        lay = self.layout(sp)
+        row = lay[lay[:, 0] == level][idx]
+        return out.reshape(int(row[3]) + 2 * ghosts, int(row[4]) + 2 * ghosts)
+
+    def flags(self, level, sp=0):
+        n = lib().vxo_flags(self.h, sp, level, None, 0)
+        a = (ctypes.c_int * max(n, 1))()
+        lib().vxo_flags(self.h, sp, level, a, n)
+        return np.array(a[:n], dtype=np.int64).reshape(-1, 2)
+
+    def regrid(self):
+        lib().vxo_regrid(self.h)

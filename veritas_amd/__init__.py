@@ -13,7 +13,7 @@ import os
 
 import numpy as np
 
-from .params import ScenarioParams, landau, two_stream  # noqa: F401
+from .params import ScenarioParams, landau, laser, two_stream  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "lib")
@@ -71,6 +71,13 @@ def host_lib():
         L.vxh_timing_end.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i)]
         L.vxh_get.restype = i
         L.vxh_get.argtypes = [vp, ctypes.c_char_p, i, vp, i]
+        L.vxh_layout.restype = i
+        L.vxh_layout.argtypes = [vp, i, vp, i]
+        L.vxh_patch_get.restype = i
+        L.vxh_patch_get.argtypes = [vp, i, i, i, ctypes.c_char_p, i, vp, i]
+        L.vxh_flags.restype = i
+        L.vxh_flags.argtypes = [vp, i, i, vp, i]
+        L.vxh_regrid.argtypes = [vp]
         _host = L
     return _host
 
@@ -164,3 +171,44 @@ class Run:
         s = self.get("scalars")
         return dict(Ex0=s[0], time=s[1], dx=s[2], dp=s[3], pmin=s[4], mass=s[5], charge=s[6],
                     n_prepad=int(s[7]))
+
+    # ---- adaptive mesh
+    def layout(self, sp=0):
+        """Patches as rows (level with 0 the finest, x_pos, p_pos, n_x, n_p,
+        up, down, left, right), levels finest first, in the reference's order."""
+        L = host_lib()
+        n = L.vxh_layout(self.h, sp, None, 0)
+        if n < 0:
+            self._ck(-1)
+        a = (ctypes.c_int * max(n, 1))()
+        L.vxh_layout(self.h, sp, a, n)
+        return np.array(a[:n], dtype=np.int64).reshape(-1, 9)
+
+    def patch(self, level, idx, what, ghosts=0, sp=0):
+        """Plane `what` of patch idx of `level` (0 = finest) as (n_x, n_p)."""
+        L = host_lib()
+        n = L.vxh_patch_get(self.h, sp, level, idx, what.encode(), ghosts, None, 0)
+        if n == -2:
+            raise KeyError((level, idx, what))
+        if n < 0:
+            self._ck(-1)
+        out = np.empty(n, dtype=np.float64)
+        self._ck(0 if L.vxh_patch_get(self.h, sp, level, idx, what.encode(), ghosts, out.ctypes.data, n) == n
+                 else -1)
+        lay = self.layout(sp)
+        row = lay[lay[:, 0] == level][idx]
+        return out.reshape(int(row[3]) + 2 * ghosts, int(row[4]) + 2 * ghosts)
+
+    def flags(self, level, sp=0):
+        """Refinement flags of hierarchy level `level` (0 = coarsest) as (i, j) rows."""
+        L = host_lib()
+        n = L.vxh_flags(self.h, sp, level, None, 0)
+        if n < 0:
+            self._ck(-1)
+        a = (ctypes.c_int * max(n, 1))()
+        L.vxh_flags(self.h, sp, level, a, n)
+        return np.array(a[:n], dtype=np.int64).reshape(-1, 2)
+
+    def regrid(self):
+        """SolverManager::reGrid (SolverManager.cpp:48-58)."""
+        self._ck(host_lib().vxh_regrid(self.h))

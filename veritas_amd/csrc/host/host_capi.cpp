@@ -120,6 +120,65 @@ int vxh_timing_end(void *h, double *total_ms, int *launches) {
     });
 }
 
+// Patch layout of species sp, same encoding as the oracle's vxo_layout:
+// 9 ints per patch (level with 0 the finest, x_pos, p_pos, n_x, n_p, up,
+// down, left, right), levels finest first.
+int vxh_layout(void *h, int sp, int *out, int n) {
+    Run *r = (Run *)h;
+    try {
+        std::vector<int> v;
+        auto &levels = r->sm->meshes.at(sp)->levels;
+        for (unsigned l = 0; l < levels.size(); l++)
+            for (auto &p : levels[l]->rectangles) {
+                int e[9] = {(int)l, p->x_pos, p->p_pos, p->n_x, p->n_p, p->up, p->down, p->left, p->right};
+                v.insert(v.end(), e, e + 9);
+            }
+        if (out && n >= (int)v.size()) std::memcpy(out, v.data(), sizeof(int) * v.size());
+        return (int)v.size();
+    } catch (std::exception &e) {
+        r->err = e.what();
+        return -1;
+    }
+}
+
+// Plane of patch idx of level lvl (0 = finest) of species sp; -2 unknown name or patch
+int vxh_patch_get(void *h, int sp, int lvl, int idx, const char *what, int ghosts, double *out, int n) {
+    Run *r = (Run *)h;
+    try {
+        auto &levels = r->sm->meshes.at(sp)->levels;
+        if (lvl < 0 || lvl >= (int)levels.size() || idx < 0 || idx >= (int)levels[lvl]->rectangles.size())
+            return -2;
+        vxh::Rectangle &p = *levels[lvl]->rectangles[idx];
+        const double *pl = p.Plane(what);
+        if (!pl) return -2;
+        return copy_out(p.Download(pl, ghosts), out, n);
+    } catch (std::exception &e) {
+        r->err = e.what();
+        return -1;
+    }
+}
+
+// Refinement flags of hierarchy level lvl (0 = coarsest) from the current
+// data, as (i, j) pairs; returns 2 * count
+int vxh_flags(void *h, int sp, int lvl, int *out, int n) {
+    Run *r = (Run *)h;
+    try {
+        std::vector<vxh::coords> fl;
+        r->sm->meshes.at(sp)->getError(lvl, false, fl);
+        if (out && n >= 2 * (int)fl.size())
+            for (size_t k = 0; k < fl.size(); k++) {
+                out[2 * k] = fl[k].first;
+                out[2 * k + 1] = fl[k].second;
+            }
+        return 2 * (int)fl.size();
+    } catch (std::exception &e) {
+        r->err = e.what();
+        return -1;
+    }
+}
+
+int vxh_regrid(void *h) { return guarded(h, [&](Run *r) { r->sm->reGrid(); }); }
+
 // Named arrays, same names and layouts as the oracle's C API.
 int vxh_get(void *h, const char *what, int ghosts, double *out, int n) {
     Run *r = (Run *)h;

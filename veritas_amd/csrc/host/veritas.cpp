@@ -155,6 +155,31 @@ Rectangle::Rectangle(int n_x_, int n_p_, int x_pos_, int p_pos_, int depth_, Set
     P.Cx = alloc_plane();
     P.Cp = alloc_plane();
     P.nested = nullptr;
+    // adaptive-mesh links: sentinel neighbours, no finer patches
+    // (Rectangle.cpp:40-66); error weights (Rectangle.cpp:68-74)
+    const int r = s.refinementRatio;
+    const size_t NSsz = (size_t)(n_x + 4) * (n_p + 4);
+    ext_xm.assign(n_p / r + 2, nullptr);
+    ext_xp.assign(n_p / r + 2, nullptr);
+    ext_pm.assign(n_x / r, nullptr);
+    ext_pp.assign(n_x / r, nullptr);
+    same_xm.assign(n_p / r + 2, 1);
+    same_xp.assign(n_p / r + 2, 1);
+    same_pm.assign(n_x / r, 1);
+    same_pp.assign(n_x / r, 1);
+    finer_level.assign(NSsz, nullptr);
+    finer_level_x.assign(NSsz, nullptr);
+    finer_level_p.assign(NSsz, nullptr);
+    is_nested.assign(NSsz, 0);
+    is_ilb_x.assign(NSsz, 0);
+    is_ilb_p.assign(NSsz, 0);
+    is_interpolated.assign(NSsz, 0);
+    const double fmx = s.GetfMax(sp);
+    errorWeights[0] = 0.5 / fmx * std::pow(r, -0.5 * depth);
+    errorWeights[1] = 0.5 / fmx * std::pow(r, -0.5 * depth);
+    errorWeights[2] = 0.5 / fmx * std::pow(r, -2.0 * depth);
+    errorWeights[3] = 0.5 / fmx * std::pow(r, -2.0 * depth);
+    errorWeights[4] = s.sizeWeight / fmx * std::pow(r, -s.maxDepth + depth);
     void *c = nullptr;
     dev->check(vx_malloc(dev->ctx, sizeof(double) * n_x * P.rtb, &c), "alloc chargeR");
     allocations.push_back(c);
@@ -234,6 +259,7 @@ void Rectangle::FCTStageFused(double timestep, int step) {
     if (step == 5) P.f[0] = out;
     for (double *b : buf)
         if (b != P.f[0] && b != P.f[1]) P.f[2] = b;
+    slot_version++;
 }
 
 void Rectangle::UnaliasSlots() {
@@ -244,24 +270,13 @@ void Rectangle::UnaliasSlots() {
         if (b != P.f[0]) spare[n++] = b;
     P.f[1] = spare[0];
     P.f[2] = spare[1];
+    slot_version++;
     dev->check(vx_memcpy_d2d(dev->ctx, P.f[1], P.f[0], sizeof(double) * vx_plane_elems(n_x, n_p)),
                "unalias slots");
 }
 
 void Rectangle::CalculateRhoAndJ() {
     dev->check(vx_moments_patch(dev->ctx, &P, &EMSolver->fields()), "moments");
-}
-
-// With every neighbour the domain-edge sentinel, the same-level ghost copy and
-// the corner fill both write zeros (BoundaryCondition.cpp:6-8).
-void Rectangle::UpdateSameLevelBoundaries(int val) {
-    if (!AllWalls()) throw Error("same-level ghost copy between patches is not available yet");
-    dev->check(vx_ghost_zero(dev->ctx, &P, val), "ghost fill");
-}
-
-void Rectangle::UpdateCornerPoints(int val) {
-    if (!AllWalls()) throw Error("coarse-fine corner fill is not available yet");
-    (void)val;  // corners were zeroed with the sides by vx_ghost_zero
 }
 
 std::vector<double> Rectangle::Download(const double *plane, int g) const {
@@ -301,50 +316,46 @@ void Level::FCTTimeStep(double timestep, int step, int subStep) {
     for (auto &r : rectangles) r->FCTTimeStep(timestep, step, subStep);
 }
 
-// Level.cpp:88-126
-void Level::PushData(int updateType, int val) {
-    switch (updateType) {
-        case 1: for (auto &r : rectangles) r->UpdateSameLevelBoundaries(val); break;
-        case 3: for (auto &r : rectangles) r->UpdateCornerPoints(val); break;
-        case 4: case 5: case 6:
-            // limiter-C sync: with one walled patch every neighbour is the
-            // sentinel and only wall faces (never used) would change
-            for (auto &r : rectangles)
-                if (!r->AllWalls()) throw Error("limiter-C sync between patches is not available yet");
-            break;
-        default:
-            if (!rectangles.empty()) throw Error("multi-level PushData is not available yet");
-    }
-}
-
 // ---------------------------------------------------------------- Mesh
-Mesh::Mesh(int sp, Settings &s, std::shared_ptr<Device> d)
-    : particleType(sp), settings(s), dev(std::move(d)) {
-    if (s.maxDepth != 0) throw Error("refined hierarchies are not available yet (Lfinest must be 1)");
-    int xmax = s.GetXSize(s.maxDepth), pmax = s.GetPSize(s.maxDepth, sp);
-    // promoteHierarchyToMesh(true) for the base rectangle (Mesh.cpp:794-875)
-    levels.push_back(std::unique_ptr<Level>(new Level(sp, 0, s)));
-    levels[0]->rectangles.push_back(
-        std::make_shared<Rectangle>(xmax, pmax, 0, 0, 0, s, dev, true, true, true, true, sp));
-    for (auto &r : levels[0]->rectangles) r->InitializeDistribution();
-    PushData();
-    for (auto &l : levels)
-        for (auto &r : l->rectangles) r->FCTTimeStep(0, -1, 3);
-}
-
 void Mesh::SetFieldSolver(EMFieldSolver *solver) {
+    EMSolver = solver;
     for (auto &l : levels)
         for (auto &r : l->rectangles) r->SetFieldSolver(solver);
 }
 
-// Mesh.cpp:64-89
+void Mesh::Sub0(double timeStep, int step) {
+    for (auto &l : levels)
+        for (auto &r : l->rectangles) r->UnaliasSlots();
+    EnsureTable();
+    const vx_fields &fl = EMSolver->fields();
+    for (int i = (int)levels.size() - 1; i > -1; i--)
+        for (auto &r : levels[i]->rectangles)
+            dev->check(vx_fct_sub0_fluxes(dev->ctx, &r->desc(), step, timeStep, &fl), "fct sub0 fluxes");
+    for (unsigned i = 1; i < levels.size(); i++) levels[i]->RunFluxProgram(step, timeStep, fl);
+    for (int i = (int)levels.size() - 1; i > -1; i--)
+        for (auto &r : levels[i]->rectangles)
+            dev->check(vx_fct_sub0_predictor(dev->ctx, &r->desc(), step, timeStep), "fct sub0 predictor");
+}
+
+void Mesh::EnsureTable() {
+    int v = 0;
+    for (Rectangle *r : order) v += r->slot_version;
+    if (v != table_version) {
+        RefreshTable();
+        table_version = v;
+    }
+}
+
+// Mesh.cpp:64-89.  Sub-step 0 is split around the coarse-fine flux programs:
+// every patch's fluxes, then the coarse faces on finer patches' edges (finest
+// coarse level first), then the low-order predictors.
 void Mesh::Advance(double timeStep, int step) {
-    if (fused && levels.size() == 1 && levels[0]->rectangles.size() == 1 &&
-        levels[0]->rectangles[0]->AllWalls()) {
-        levels[0]->rectangles[0]->FCTStageFused(timeStep, step);
+    Rectangle *single = SinglePatch() ? levels.back()->rectangles[0].get() : nullptr;
+    if (fused && single && single->AllWalls()) {
+        single->FCTStageFused(timeStep, step);
         return;
     }
-    for (int i = (int)levels.size() - 1; i > -1; i--) levels[i]->FCTTimeStep(timeStep, step, 0);
+    Sub0(timeStep, step);
     PushData(2);
     for (int i = (int)levels.size() - 1; i > -1; i--) levels[i]->FCTTimeStep(timeStep, step, 1);
     PushBoundaryC();
@@ -356,6 +367,7 @@ void Mesh::Advance(double timeStep, int step) {
 
 // Mesh.cpp:91-106
 void Mesh::PushData(int val) {
+    EnsureTable();
     levels[0]->PushData(1, val);
     for (unsigned i = 1; i < levels.size(); i++) {
         levels[i]->PushData(0, val);
@@ -367,6 +379,7 @@ void Mesh::PushData(int val) {
 
 // Mesh.cpp:904-917
 void Mesh::PushBoundaryC() {
+    EnsureTable();
     for (auto &l : levels) l->PushData(4);
     for (auto &l : levels) l->PushData(5);
     for (auto &l : levels) l->PushData(6);
@@ -523,7 +536,9 @@ void SolverManager::StageVlasov(double dt, int step) {
 
 void SolverManager::StageVlasovSub(double dt, int step, int sub) {
     for (auto &m : meshes) {
-        for (int i = (int)m->levels.size() - 1; i > -1; i--) m->levels[i]->FCTTimeStep(dt, step, sub);
+        if (sub == 0) m->Sub0(dt, step);
+        else
+            for (int i = (int)m->levels.size() - 1; i > -1; i--) m->levels[i]->FCTTimeStep(dt, step, sub);
         if (sub == 0) m->PushData(2);
         if (sub == 1) m->PushBoundaryC();
         if (sub == 2) m->PushData(1);

@@ -5,6 +5,7 @@
 // kernels behind the C ABI of include/veritas_amd.h.
 #pragma once
 
+#include <cstdint>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -95,6 +96,21 @@ class Rectangle {
 public:
     int n_x, n_p, x_pos, p_pos;
     double relativeToBottom;
+    int depth, particleType;
+    bool up, down, left, right;
+    // adaptive-mesh links (Rectangle.hpp:7-11); a null neighbour is the
+    // domain-edge sentinel (BoundaryCondition)
+    std::vector<Rectangle *> ext_xm, ext_xp, ext_pm, ext_pp;
+    std::vector<char> same_xm, same_xp, same_pm, same_pp;
+    std::vector<Rectangle *> finer_level, finer_level_x, finer_level_p;
+    std::vector<char> is_nested, is_ilb_x, is_ilb_p, is_interpolated;
+    double errorWeights[5];
+    int table_index = -1;  // index in the mesh's device patch table
+    int NS(int i, int j) const { return (n_p + 4) * (i + 2) + 2 + j; }
+    void CalculateConnectivitySame(Rectangle *rect);                    // Rectangle.cpp:671-766
+    void CalculateConnectivityFromFiner(Rectangle *current, Rectangle *rect);  // 768-864
+    void getError(std::vector<std::pair<int, int>> &flagged);          // 866-890
+    void UploadNested();
     Rectangle(int n_x, int n_p, int x_pos, int p_pos, int depth, Settings &settings,
               std::shared_ptr<Device> dev, bool up, bool down, bool left, bool right,
               int particleType);
@@ -105,9 +121,8 @@ public:
     void FCTStageWalls(double timestep, int step);  // sub-step kernels, whole stage (walls only)
     void FCTStageFused(double timestep, int step);  // one fused kernel per stage (walls only)
     void UnaliasSlots();  // give f[0] and f[1] distinct planes (before sub-step kernels)
+    int slot_version = 0;  // bumped whenever the f plane pointers change
     void CalculateRhoAndJ();
-    void UpdateSameLevelBoundaries(int val = 1);
-    void UpdateCornerPoints(int val = 1);
     bool AllWalls() const { return up && down && left && right; }
     const vx_patch &desc() const { return P; }
     std::vector<double> Download(const double *plane, int ghosts) const;
@@ -115,12 +130,11 @@ public:
     const double *Plane(const std::string &name) const;
 
 private:
-    int depth, particleType;
-    bool up, down, left, right;
     Settings &settings;
     std::shared_ptr<Device> dev;
     EMFieldSolver *EMSolver = nullptr;
     vx_patch P{};
+    uint8_t *nested_d = nullptr;
     double *buf[3] = {nullptr, nullptr, nullptr};  // physical planes behind f[0..2]
     std::vector<void *> allocations;
     double *alloc_plane();
@@ -138,30 +152,61 @@ class Level {
 public:
     std::vector<std::shared_ptr<Rectangle>> rectangles;
     Level(int particleType, int depth, Settings &settings);
+    ~Level();
     void FCTTimeStep(double timestep, int step, int subStep);
+    // Level.cpp:88-126: 0 restriction, 1 same-level ghosts, 2 coarse-fine
+    // ghosts, 3 corners, 4-6 limiter-C sync; each a device task program
     void PushData(int updateType, int val = 1);
+    void RunFluxProgram(int step, double dt, const vx_fields &fl);
     int depth;
+    // set by the mesh when the hierarchy changes
+    Device *dev = nullptr;
+    vx_patch_table *table = nullptr;
+    vx_program *prog[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    vx_program *flux_prog = nullptr;  // coarse-fine faces of this level's patches
+    void FreePrograms();
 
 private:
     int particleType;
     Settings &settings;
 };
 
+typedef std::pair<int, int> coords;  // veritas.hpp:45-48
+typedef std::pair<coords, coords> rect;
+typedef std::vector<rect> level;
+
 // Mesh.hpp:4-41
 class Mesh {
 public:
     int particleType;
     std::vector<std::unique_ptr<Level>> levels;  // levels[0] is the finest
+    std::vector<level> hierarchy;                // hierarchy[0] is the coarsest
     Mesh(int particleType, Settings &settings, std::shared_ptr<Device> dev);
+    ~Mesh();
     void SetFieldSolver(EMFieldSolver *solver);
     void Advance(double timeStep, int step);
     void PushData(int val = 1);
     void PushBoundaryC();
+    void updateHierarchy(bool init = false);       // Mesh.cpp:132-210
+    void getError(int lvl, bool init, std::vector<coords> &flagged);  // Mesh.cpp:212-295
+    bool SinglePatch() const;
+    void Sub0(double timeStep, int step);  // sub-step 0 of every patch with the coarse-fine fluxes
     bool fused = true;  // single walled patch: one fused call per stage
 
 private:
     Settings &settings;
     std::shared_ptr<Device> dev;
+    EMFieldSolver *EMSolver = nullptr;
+    // device patch table and task programs of the current hierarchy (host/amr.cpp)
+    vx_patch_table *table = nullptr;
+    std::vector<Rectangle *> order;  // patches in table order
+    void promoteHierarchyToMesh(bool init);      // Mesh.cpp:794-875
+    void InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &levels_n);  // Mesh.cpp:116-130
+    void BuildPrograms();
+    void FreePrograms();
+    void RefreshTable();
+    void EnsureTable();  // refresh the device table if plane pointers moved
+    int table_version = -1;
 };
 
 // EMSolver.hpp:9-49
@@ -198,6 +243,7 @@ public:
     void Advance(double timeStep);
     void AdvanceFields(double timeStep);
     double CalculateDt(double cfl) { return cfl * EMSolver->EstimateCFLBound(); }
+    void reGrid();  // SolverManager.cpp:48-58
     // pieces of Advance for stage-by-stage checks
     void StageMoments() { EMSolver->AssembleRhoAndJ(); }
     void StagePoisson() { EMSolver->UpdatePotential(); }

@@ -45,6 +45,7 @@ struct Scenario {
     // laser
     double a0 = 0.5, n_over_nc = 0.01, ramp = 50.0, fwhm_cycles = 10.0;
     double lambda0 = 1.0e-6;  // laser wavelength (m) for the laser scenario
+    double gap = 0.0;         // vacuum gap (wavelengths) in the middle of the plateau
     // derived (filled by finalize)
     double omega_p = 0, unit_len = 0, unit_p = 0, norm = 0;
     double phases[64] = {0};
@@ -66,6 +67,12 @@ struct Scenario {
     }
 
     void finalize() {
+        if (kind == SCN_LASER) {
+            // plateau density given as a fraction of the critical density of
+            // the laser wavelength: n_c = eps0 m omega0^2 / e^2
+            double w0 = 2.0 * M_PI * vx_cs / lambda0;
+            n_ref = n_over_nc * vx_eps0 * mass * w0 * w0 / (e_charge * e_charge);
+        }
         omega_p = std::sqrt(n_ref * e_charge * e_charge / (vx_eps0 * mass));
         unit_len = vx_cs / omega_p;
         unit_p = mass * vx_cs;
@@ -110,6 +117,10 @@ struct Scenario {
             double start = 10.0 * lam;        // vacuum gap before the ramp
             double xr = (xn - start) / (ramp * lam);
             double prof = xr <= 0 ? 0.0 : (xr < 1 ? xr : 1.0);
+            if (gap > 0.0) {  // two slabs: vacuum of `gap` wavelengths mid-plateau
+                double mid = 0.5 * (start + ramp * lam + length);
+                if (std::fabs(xn - mid) < 0.5 * gap * lam) prof = 0.0;
+            }
             n = prof;
             fu = mj(u, 0.0);
         }
@@ -134,6 +145,8 @@ struct Scenario {
         double E = a0 * Ec * laser_env(t - x / vx_cs) * std::sin(w0 * t - k0 * x);
         return E / vx_cs;
     }
+    // the scenarios here define no refinement override (the hook returns false)
+    bool has_refinement_override() const { return false; }
     bool refinement_override(double x, double p, int depth, int species) const {
         (void)x; (void)p; (void)depth; (void)species;
         return false;
@@ -166,6 +179,7 @@ struct Scenario {
         s.ramp = c.ramp;
         s.fwhm_cycles = c.fwhm_cycles;
         s.lambda0 = c.lambda0;
+        s.gap = c.gap;
         s.finalize();
         s.pre_length = c.pre_length * s.unit_len;
         s.post_length = c.post_length * s.unit_len;

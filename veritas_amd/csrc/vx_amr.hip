@@ -1,0 +1,506 @@
+// Adaptive-mesh operations as cell-task programs (ghost fill, restriction,
+// regrid transfer, limiter-C synchronisation, coarse-fine fluxes) and the
+// refinement-flag kernel.
+//
+// The host records, once per hierarchy, every cell or face operation that the
+// reference's per-rectangle loops perform (Rectangle.cpp:314-614, 892-941,
+// 1032-1253, 1625-1960) as a vx_cell_task, in the reference's order, and cuts
+// the list into batches whose tasks touch disjoint data; one thread runs one
+// task, one launch runs one batch.  These are O(patch perimeter) operations;
+// the per-cell arithmetic keeps the reference's expressions so that the
+// results match the CPU oracle bit for bit.
+#include <cmath>
+#include <vector>
+
+#include "vx_dev.h"
+#include "vx_internal.h"
+
+using namespace vxi;
+using namespace vxd;
+
+struct vx_patch_table {
+    vx_ctx *ctx = nullptr;
+    vx_patch *d = nullptr;
+    int32_t n = 0;
+    uint64_t serial = 0;          // identity for the program bounds check
+    std::vector<vx_patch> host;   // shapes, for the bounds check
+};
+
+struct vx_program {
+    vx_ctx *ctx = nullptr;
+    vx_cell_task *d = nullptr;
+    std::vector<int32_t> batch;   // host copy of the batch starts
+    std::vector<vx_cell_task> host;
+    uint64_t checked_serial = 0;  // table this program was bounds-checked against
+    int checked_r = 0;
+};
+
+static uint64_t g_table_serial = 0;
+
+namespace {
+
+constexpr int MAXR = 4;
+
+// Rectangle.cpp:76-89
+__constant__ double kInterpA[12] = {
+    0.104166666666667, -0.708333333333334, 0.708333333333334, -0.104166666666667,
+    0.117647058823529, 0.029411764705882,  0.029411764705882, 0.117647058823529,
+    -0.083333333333333, 0.166666666666667, -0.166666666666667, 0.083333333333334};
+
+struct ProgArgs {
+    const vx_patch *pt;
+    const vx_cell_task *task;
+    int begin, end;
+    int mode, slot, r;
+    double inv_r2;           // 1.0 / std::pow(r, 2)      (Rectangle.cpp:389)
+    double r2;               // std::pow((double)r, 2.0)  (Rectangle.cpp:325)
+    double inv_rr;           // 1.0 / (r * r)             (Rectangle.hpp:141)
+    double ref[3 * MAXR];    // interpolatCoefsREF (Rectangle.cpp:91-101)
+    // flux programs
+    int step, ns;
+    double a[6], aSum;
+    const double *a2;
+    int n_f;
+};
+
+__device__ __forceinline__ double &at(double *pl, int pitch, int i, int j) { return pl[pidx(pitch, i, j)]; }
+__device__ __forceinline__ double get(const double *pl, int pitch, int i, int j) { return pl[pidx(pitch, i, j)]; }
+
+// GetInterpolantsREF (Rectangle.cpp:121-137), element k of r
+__device__ __forceinline__ void interp_ref(const ProgArgs &A, double f1, double f2, double f3, double f4,
+                                           double f5, double *out) {
+    f5 -= f3; f4 -= f3; f2 -= f3; f1 -= f3;
+    double a1 = kInterpA[0] * f1 + kInterpA[1] * f2 + kInterpA[2] * f4 + kInterpA[3] * f5;
+    double a2 = kInterpA[4] * f1 + kInterpA[5] * f2 + kInterpA[6] * f4 + kInterpA[7] * f5;
+    double a3 = kInterpA[8] * f1 + kInterpA[9] * f2 + kInterpA[10] * f4 + kInterpA[11] * f5;
+    for (int i = 0; i < A.r; i++)
+        out[i] = A.ref[3 * i] * a1 + A.ref[3 * i + 1] * a2 + A.ref[3 * i + 2] * a3 + f3;
+}
+
+// GetWenoValueFromCoarseLevel (Rectangle.cpp:343-415): element e = k * r + l
+// of the corrected r x r interpolants of the coarse cell holding (i, j).
+__device__ double coarse_element(const ProgArgs &A, const vx_patch &C, const double *g, int i, int j, int e) {
+    const int r = A.r, pitch = C.pitch;
+    const int ic = i / r - C.x_pos, jc = j / r - C.p_pos;
+    double temps[5][MAXR];
+    for (int k = -2; k < 3; k++)
+        interp_ref(A, get(g, pitch, ic - 2, jc + k), get(g, pitch, ic - 1, jc + k), get(g, pitch, ic, jc + k),
+                   get(g, pitch, ic + 1, jc + k), get(g, pitch, ic + 2, jc + k), temps[k + 2]);
+    double sum(0.0), want = 0.0;
+    for (int k = 0; k < r; k++) {
+        double part[MAXR];
+        interp_ref(A, temps[0][k], temps[1][k], temps[2][k], temps[3][k], temps[4][k], part);
+        for (int l = 0; l < r; l++) {
+            if (k * r + l == e) want = part[l];
+            sum += part[l];
+        }
+    }
+    double correction = get(g, pitch, ic, jc) - A.inv_r2 * sum;
+    return want + correction;
+}
+
+// GetValueFromFinerLevel (Rectangle.cpp:314-327)
+__device__ double finer_mean(const ProgArgs &A, const vx_patch &F, const double *g, int i, int j) {
+    const int r = A.r;
+    const int i0 = i * r - F.x_pos, j0 = j * r - F.p_pos;
+    double temp(0.0);
+    for (int k = 0; k < r; k++)
+        for (int l = 0; l < r; l++) temp += get(g, F.pitch, i0 + k, j0 + l);
+    temp /= A.r2;
+    return temp;
+}
+
+// std::min of the limiter-C sync (Rectangle.cpp:1646-1905)
+__device__ __forceinline__ double mn(double a, double b) { return stdmin(a, b); }
+
+// one limiter-C task (Rectangle.cpp:1625-1706, 1794-1830, 1879-1915)
+__device__ void c_task(const ProgArgs &A, const vx_cell_task &T) {
+    const vx_patch &N = A.pt[T.dst];  // neighbour (same level) or coarse patch
+    const vx_patch &Q = A.pt[T.src];  // caller
+    const int np_ = N.pitch, qp = Q.pitch;
+    const int di = T.di, dj = T.dj, si = T.si, sj = T.sj, t = T.aux;
+    if (T.op == VX_OP_C_SAME) {
+        double c;
+        if (t == 0)
+            c = get(N.FxDS, np_, di, dj) > 0.0 ? mn(get(Q.Rp, qp, si, sj), get(N.Rm, np_, di - 1, dj))
+                                               : mn(get(N.Rp, np_, di - 1, dj), get(Q.Rm, qp, si, sj));
+        else if (t == 1)
+            c = get(N.FxDS, np_, di, dj) > 0.0 ? mn(get(Q.Rm, qp, si - 1, sj), get(N.Rp, np_, di, dj))
+                                               : mn(get(N.Rm, np_, di, dj), get(Q.Rp, qp, si - 1, sj));
+        else if (t == 2)
+            c = get(N.FpDS, np_, di, dj) > 0.0 ? mn(get(Q.Rp, qp, si, sj), get(N.Rm, np_, di, dj - 1))
+                                               : mn(get(N.Rp, np_, di, dj - 1), get(Q.Rm, qp, si, sj));
+        else
+            c = get(N.FpDS, np_, di, dj) > 0.0 ? mn(get(Q.Rm, qp, si, sj - 1), get(N.Rp, np_, di, dj))
+                                               : mn(get(N.Rm, np_, di, dj), get(Q.Rp, qp, si, sj - 1));
+        double *Nc = t < 2 ? N.Cx : N.Cp, *Qc = t < 2 ? Q.Cx : Q.Cp;
+        at(Nc, np_, di, dj) = c;
+        at(Qc, qp, si, sj) = c;
+    } else if (T.op == VX_OP_C_UPDATE) {
+        double *Nc = t < 2 ? N.Cx : N.Cp, *Qc = t < 2 ? Q.Cx : Q.Cp;
+        double c = mn(get(Nc, np_, di, dj), get(Qc, qp, si, sj));
+        at(Nc, np_, di, dj) = c;
+        at(Qc, qp, si, sj) = c;
+    } else {  // VX_OP_C_DIFF: N is the coarse patch, Q the finer caller
+        const int r = A.r;
+        if (t < 2) {
+            double c = get(N.Cx, np_, di, dj);
+            for (int k = 0; k < r; k++) {
+                bool pos = get(Q.FxDS, qp, si, sj + k) > 0.0;
+                double v = t == 0 ? (pos ? get(Q.Rp, qp, si, sj + k) : get(Q.Rm, qp, si, sj + k))
+                                  : (pos ? get(Q.Rm, qp, si - 1, sj + k) : get(Q.Rp, qp, si - 1, sj + k));
+                c = mn(c, v);
+            }
+            bool posc = get(N.FxDS, np_, di, dj) > 0.0;
+            double vc = t == 0 ? (posc ? get(N.Rm, np_, di - 1, dj) : get(N.Rp, np_, di - 1, dj))
+                               : (posc ? get(N.Rp, np_, di, dj) : get(N.Rm, np_, di, dj));
+            c = mn(c, vc);
+            for (int k = 0; k < r; k++) at(Q.Cx, qp, si, sj + k) = c;
+            at(N.Cx, np_, di, dj) = c;
+        } else {
+            double c = get(N.Cp, np_, di, dj);
+            for (int k = 0; k < r; k++) {
+                bool pos = get(Q.FpDS, qp, si + k, sj) > 0.0;
+                double v = t == 2 ? (pos ? get(Q.Rp, qp, si + k, sj) : get(Q.Rm, qp, si + k, sj))
+                                  : (pos ? get(Q.Rm, qp, si + k, sj - 1) : get(Q.Rp, qp, si + k, sj - 1));
+                c = mn(c, v);
+            }
+            bool posc = get(N.FpDS, np_, di, dj) > 0.0;
+            double vc = t == 2 ? (posc ? get(N.Rm, np_, di, dj - 1) : get(N.Rp, np_, di, dj - 1))
+                               : (posc ? get(N.Rp, np_, di, dj) : get(N.Rm, np_, di, dj));
+            c = mn(c, vc);
+            for (int k = 0; k < r; k++) at(Q.Cp, qp, si + k, sj) = c;
+            at(N.Cp, np_, di, dj) = c;
+        }
+    }
+}
+
+// EMFieldSolver::GetASquared (EMSolver.cpp:133-135) at a patch's x-face
+__device__ __forceinline__ double face_a2(const ProgArgs &A, const vx_patch &P, int i) {
+    int fi = P.rtb * (i + P.x_pos);
+    int c = fi < 0 ? 0 : (fi > A.n_f ? A.n_f : fi);
+    return A.a2[c];
+}
+
+// RGKGetFluxXL / RGKGetFluxPL of the finer patch (Rectangle.cpp:1216-1253),
+// with their own association: dx_inv * f * am
+__device__ double lo_flux_x(const ProgArgs &A, const vx_patch &F, int i, int j) {
+    double as = F.q * F.q * face_a2(A, F, i);
+    double am = F.dp_inv * F.cc *
+                (gamma_(momentum(F.pmin, F.dp, j + 1, F.p_pos), as, F.mc2inv) -
+                 gamma_(momentum(F.pmin, F.dp, j, F.p_pos), as, F.mc2inv));
+    return F.dx_inv * (am > 0.0 ? get(F.f[1], F.pitch, i - 1, j) : get(F.f[1], F.pitch, i, j)) * am;
+}
+__device__ double lo_flux_p(const ProgArgs &A, const vx_patch &F, const double *E, int i, int j) {
+    double as_1 = F.q * F.q * face_a2(A, F, i);
+    double as_2 = F.q * F.q * face_a2(A, F, i + 1);
+    double Em = F.q * patch_E(E, A.n_f, F.rtb, F.x_pos, i);
+    double mom = momentum(F.pmin, F.dp, j, F.p_pos);
+    double am = Em - F.cc * F.dx_inv * (gamma_(mom, as_2, F.mc2inv) - gamma_(mom, as_1, F.mc2inv));
+    return F.dp_inv * (am > 0.0 ? get(F.f[1], F.pitch, i, j - 1) : get(F.f[1], F.pitch, i, j)) * am;
+}
+
+// coarse-fine flux task (Rectangle.cpp:1313-1394 else-branches, then the RK
+// combination of Rectangle.cpp:1396-1517 at that face)
+__device__ void flux_task(const ProgArgs &A, const double *E, const vx_cell_task &T) {
+    const vx_patch &K = A.pt[T.dst];
+    const vx_patch &F = A.pt[T.src];
+    const int r = A.r, kp = K.pitch, fp = F.pitch;
+    const bool xdir = T.op == VX_OP_FLUX_X;
+    double *const *KH = xdir ? K.FxH : K.FpH;
+    double *const *FH = xdir ? F.FxH : F.FpH;
+    double *KL = xdir ? K.FxL0 : K.FpL0;
+    const double *FL = xdir ? F.FxL0 : F.FpL0;
+    double h = 0.0;
+    for (int k = 0; k < r; k++) h += xdir ? get(FH[A.step], fp, T.si, T.sj + k) : get(FH[A.step], fp, T.si + k, T.sj);
+    h *= A.inv_rr;
+    at(KH[A.step], kp, T.di, T.dj) = h;
+    if (A.step == 0) {
+        double l = 0.0;
+        for (int k = 0; k < r; k++) {
+            int i = xdir ? T.si : T.si + k, j = xdir ? T.sj + k : T.sj;
+            double v;
+            if ((T.aux >> k) & 1) v = get(FL, fp, i, j);
+            else v = xdir ? lo_flux_x(A, F, i, j) : lo_flux_p(A, F, E, i, j);
+            l += v;
+        }
+        l *= A.inv_rr;
+        at(KL, kp, T.di, T.dj) = l;
+    }
+    double s = A.a[0] * get(KH[0], kp, T.di, T.dj);
+    for (int k = 1; k < A.ns; k++) s = s + A.a[k] * get(KH[k], kp, T.di, T.dj);
+    double ls = A.aSum * get(KL, kp, T.di, T.dj);
+    at(xdir ? K.FxDS : K.FpDS, kp, T.di, T.dj) = s - ls;
+}
+
+__global__ __launch_bounds__(256) void k_program(ProgArgs A, const double *E) {
+    const int t = A.begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.end) return;
+    const vx_cell_task T = A.task[t];
+    if (T.op >= VX_OP_C_SAME && T.op <= VX_OP_C_UPDATE) {
+        c_task(A, T);
+        return;
+    }
+    if (T.op == VX_OP_FLUX_X || T.op == VX_OP_FLUX_P) {
+        flux_task(A, E, T);
+        return;
+    }
+    const vx_patch &D = A.pt[T.dst];
+    const int rs = A.mode == VX_MODE_TRANSFER ? 1 : A.slot;
+    double v = 0.0;
+    if (T.op == VX_OP_COPY) {
+        const vx_patch &S = A.pt[T.src];
+        v = get(S.f[rs], S.pitch, T.si, T.sj);
+    } else if (T.op == VX_OP_COARSE) {
+        const vx_patch &S = A.pt[T.src];
+        v = coarse_element(A, S, S.f[rs], T.si, T.sj, T.aux);
+    } else if (T.op == VX_OP_RESTRICT) {
+        const vx_patch &S = A.pt[T.src];
+        v = finer_mean(A, S, S.f[rs], T.si, T.sj);
+    }
+    if (A.mode == VX_MODE_TRANSFER) {
+        at(D.f[0], D.pitch, T.di, T.dj) = v;
+        at(D.f[1], D.pitch, T.di, T.dj) = v;
+    } else {
+        at(D.f[A.slot], D.pitch, T.di, T.dj) = v;
+    }
+}
+
+// Rectangle::ErrorEstimate (Rectangle.hpp:128-130) > criteria
+__global__ __launch_bounds__(256) void k_flags(vx_patch P, double w0, double w1, double w2, double w3,
+                                               double w4, double crit, int lo_x, int hi_x, int lo_p,
+                                               int hi_p, uint8_t *flags) {
+    const int w = hi_p - lo_p;
+    const long n = (long)(hi_x - lo_x) * w;
+    const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const int i = lo_x + (int)(e / w), j = lo_p + (int)(e % w);
+    const double *g = P.f[1];
+    const int pt = P.pitch;
+    double c = get(g, pt, i, j);
+    double err = w0 * fabs(get(g, pt, i + 1, j) - get(g, pt, i - 1, j)) +
+                 w1 * fabs(get(g, pt, i, j + 1) - get(g, pt, i, j - 1)) +
+                 w2 * fabs(get(g, pt, i + 1, j) - 2 * c + get(g, pt, i - 1, j)) +
+                 w3 * fabs(get(g, pt, i, j + 1) - 2 * c + get(g, pt, i, j - 1)) + w4 * fabs(c);
+    flags[e] = err > crit ? 1 : 0;
+}
+
+int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r);
+
+int fill_common(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int r, ProgArgs &A) {
+    if (!table || !prog || !table->d) return fail(ctx, VX_EINVAL, "program: null table or program");
+    if (r < 2 || r > MAXR) return fail(ctx, VX_ENOTSUP, "program: refinement ratio %d not in [2, %d]", r, MAXR);
+    int rc = check_program(ctx, table, const_cast<vx_program *>(prog), r);
+    if (rc) return rc;
+    A.pt = table->d;
+    A.task = prog->d;
+    A.r = r;
+    A.inv_r2 = 1.0 / std::pow(r, 2);
+    A.r2 = std::pow((double)r, 2.0);
+    A.inv_rr = (1.0 / (r * r));
+    for (int i = 0; i < 3 * MAXR; i++) A.ref[i] = 0.0;
+    for (int i = 0; i < r; i++) {  // Rectangle.cpp:91-101
+        double tl = -0.5 + i / (double)r;
+        double tr = -0.5 + (i + 1.0) / (double)r;
+        A.ref[3 * i] = (tl + tr) * 0.5;
+        A.ref[3 * i + 1] = (tl * tl + tl * tr + tr * tr) / 3.0 - (1.0 / 12);
+        A.ref[3 * i + 2] = (tl * tl * tl + tl * tl * tr + tl * tr * tr + tr * tr * tr) * 0.25;
+    }
+    return VX_OK;
+}
+
+// Host bounds check of every cell a program's tasks touch (ghost ring
+// included), done once per (program, table) pair before the first launch.
+bool inr(const vx_patch &P, int i, int j) { return i >= -2 && i < P.n_x + 2 && j >= -2 && j < P.n_p + 2; }
+
+int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r) {
+    if (prog->checked_serial == tb->serial && prog->checked_r == r) return VX_OK;
+    const int n = tb->n;
+    for (size_t t = 0; t < prog->host.size(); t++) {
+        const vx_cell_task &T = prog->host[t];
+        bool ok = T.dst >= 0 && T.dst < n;
+        const bool uses_src = T.op != VX_OP_ZERO;
+        if (ok && uses_src) ok = T.src >= 0 && T.src < n;
+        if (ok) {
+            const vx_patch &D = tb->host[T.dst];
+            const vx_patch &S = tb->host[uses_src ? T.src : T.dst];
+            const int di = T.di, dj = T.dj, si = T.si, sj = T.sj;
+            switch (T.op) {
+                case VX_OP_ZERO: ok = inr(D, di, dj); break;
+                case VX_OP_COPY: ok = inr(D, di, dj) && inr(S, si, sj); break;
+                case VX_OP_COARSE: {
+                    int ic = si / r - S.x_pos, jc = sj / r - S.p_pos;
+                    ok = inr(D, di, dj) && inr(S, ic - 2, jc - 2) && inr(S, ic + 2, jc + 2) && T.aux >= 0 &&
+                         T.aux < r * r;
+                    break;
+                }
+                case VX_OP_RESTRICT: {
+                    int i0 = si * r - S.x_pos, j0 = sj * r - S.p_pos;
+                    ok = inr(D, di, dj) && inr(S, i0, j0) && inr(S, i0 + r - 1, j0 + r - 1);
+                    break;
+                }
+                case VX_OP_C_SAME: case VX_OP_C_UPDATE:
+                    ok = T.aux >= 0 && T.aux < 4 && inr(D, di, dj) && inr(S, si, sj) && inr(D, di - 1, dj - 1) &&
+                         inr(S, si - 1, sj - 1);
+                    break;
+                case VX_OP_C_DIFF:
+                    ok = T.aux >= 0 && T.aux < 4 && inr(D, di, dj) && inr(D, di - 1, dj - 1) && inr(S, si - 1, sj - 1) &&
+                         inr(S, si + r - 1, sj + r - 1);
+                    break;
+                case VX_OP_FLUX_X: case VX_OP_FLUX_P:
+                    ok = inr(D, di, dj) && inr(S, si - 1, sj - 1) && inr(S, si + r - 1, sj + r - 1);
+                    break;
+                default: ok = false;
+            }
+        }
+        if (!ok)
+            return fail(ctx, VX_EINVAL, "program task %zu (op %d, dst %d, src %d) out of range", t, T.op, T.dst,
+                        T.src);
+    }
+    prog->checked_serial = tb->serial;
+    prog->checked_r = r;
+    return VX_OK;
+}
+
+int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const double *E) {
+    const size_t nb = prog->batch.empty() ? 0 : prog->batch.size() - 1;
+    for (size_t b = 0; b < nb; b++) {
+        A.begin = prog->batch[b];
+        A.end = prog->batch[b + 1];
+        const int n = A.end - A.begin;
+        if (n <= 0) continue;
+        k_program<<<(n + 255) / 256, 256, 0, ctx->stream>>>(A, E);
+        int rc = check_launch(ctx, "k_program");
+        if (rc) return rc;
+    }
+    return VX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_patch_table_create(vx_ctx *ctx, const vx_patch *patches, int32_t n, vx_patch_table **out) {
+    if (!out || n < 0 || (n > 0 && !patches)) return fail(ctx, VX_EINVAL, "patch table: bad arguments");
+    vx_patch_table *t = new vx_patch_table;
+    t->ctx = ctx;
+    t->n = n;
+    t->serial = ++g_table_serial;
+    t->host.assign(patches, patches + n);
+    if (n > 0) {
+        hipError_t e = hipMalloc(&t->d, sizeof(vx_patch) * n);
+        if (e != hipSuccess) {
+            delete t;
+            return fail(ctx, VX_ENOMEM, "patch table: %s", hipGetErrorString(e));
+        }
+        int rc = vx_patch_table_update(ctx, t, patches, n);
+        if (rc) {
+            vx_patch_table_destroy(t);
+            return rc;
+        }
+    }
+    *out = t;
+    return VX_OK;
+}
+
+int vx_patch_table_update(vx_ctx *ctx, vx_patch_table *t, const vx_patch *patches, int32_t n) {
+    if (!t || n != t->n) return fail(ctx, VX_EINVAL, "patch table: size mismatch");
+    if (n == 0) return VX_OK;
+    for (int k = 0; k < n; k++) {
+        const vx_patch &a = t->host[k], &b = patches[k];
+        if (a.n_x != b.n_x || a.n_p != b.n_p || a.pitch != b.pitch || a.x_pos != b.x_pos || a.p_pos != b.p_pos)
+            return fail(ctx, VX_EINVAL, "patch table: patch %d changed shape", k);
+        int rc = validate_patch(ctx, &b);
+        if (rc) return rc;
+    }
+    t->host.assign(patches, patches + n);
+    hipError_t e = hipMemcpyAsync(t->d, patches, sizeof(vx_patch) * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ctx, VX_EHIP, "patch table upload: %s", hipGetErrorString(e));
+    return VX_OK;
+}
+
+void vx_patch_table_destroy(vx_patch_table *t) {
+    if (!t) return;
+    if (t->d) (void)hipFree(t->d);
+    delete t;
+}
+
+int vx_program_create(vx_ctx *ctx, const vx_cell_task *tasks, int32_t n_tasks, const int32_t *batch_start,
+                      int32_t n_batches, vx_program **out) {
+    if (!out || n_tasks < 0 || n_batches < 0 || (n_tasks > 0 && (!tasks || !batch_start)))
+        return fail(ctx, VX_EINVAL, "program: bad arguments");
+    if (n_tasks > 0 && (batch_start[0] != 0 || batch_start[n_batches] != n_tasks))
+        return fail(ctx, VX_EINVAL, "program: batches must cover [0, %d)", n_tasks);
+    for (int b = 0; b < n_batches; b++)
+        if (n_tasks > 0 && batch_start[b] > batch_start[b + 1])
+            return fail(ctx, VX_EINVAL, "program: batch starts not ordered");
+    vx_program *p = new vx_program;
+    p->ctx = ctx;
+    if (n_tasks > 0) {
+        p->batch.assign(batch_start, batch_start + n_batches + 1);
+        p->host.assign(tasks, tasks + n_tasks);
+        hipError_t e = hipMalloc(&p->d, sizeof(vx_cell_task) * n_tasks);
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d, tasks, sizeof(vx_cell_task) * n_tasks, hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+            if (p->d) (void)hipFree(p->d);
+            delete p;
+            return fail(ctx, VX_ENOMEM, "program upload: %s", hipGetErrorString(e));
+        }
+    }
+    *out = p;
+    return VX_OK;
+}
+
+void vx_program_destroy(vx_program *prog) {
+    if (!prog) return;
+    if (prog->d) (void)hipFree(prog->d);
+    delete prog;
+}
+
+int vx_program_run(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int mode, int slot, int r) {
+    if (prog && prog->batch.empty()) return VX_OK;
+    ProgArgs A{};
+    int rc = fill_common(ctx, table, prog, r, A);
+    if (rc) return rc;
+    if (mode != VX_MODE_GHOST && mode != VX_MODE_TRANSFER) return fail(ctx, VX_EINVAL, "program: bad mode %d", mode);
+    if (slot < 0 || slot > 2) return fail(ctx, VX_EINVAL, "program: bad slot %d", slot);
+    A.mode = mode;
+    A.slot = slot;
+    return launch_batches(ctx, prog, A, nullptr);
+}
+
+int vx_program_run_fluxes(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int step, double dt,
+                          int r, const vx_fields *fl) {
+    if (prog && prog->batch.empty()) return VX_OK;
+    ProgArgs A{};
+    int rc = fill_common(ctx, table, prog, r, A);
+    if (rc) return rc;
+    if (step < 0 || step > 5 || !fl) return fail(ctx, VX_EINVAL, "flux program: bad step %d", step);
+    A.step = step;
+    A.ns = step + 1;
+    for (int k = 0; k < 6; k++) A.a[k] = 0.0;
+    for (int k = 0; k < A.ns; k++) A.a[k] = vx::rk_a[step][k] * dt;
+    A.aSum = A.a[0];
+    for (int k = 1; k < A.ns; k++) A.aSum = A.aSum + A.a[k];
+    A.a2 = fl->a2;
+    A.n_f = fl->n_f;
+    return launch_batches(ctx, prog, A, fl->E);
+}
+
+int vx_flag_cells(vx_ctx *ctx, const vx_patch *p, const double *w, double criteria, int32_t lo_x, int32_t hi_x,
+                  int32_t lo_p, int32_t hi_p, uint8_t *flags) {
+    int rc = validate_patch(ctx, p);
+    if (rc) return rc;
+    if (!w || !flags) return fail(ctx, VX_EINVAL, "flags: null weights or output");
+    if (lo_x < 0 || lo_p < 0 || hi_x > p->n_x || hi_p > p->n_p)
+        return fail(ctx, VX_EINVAL, "flags: range [%d,%d)x[%d,%d) outside the patch", lo_x, hi_x, lo_p, hi_p);
+    if (hi_x <= lo_x || hi_p <= lo_p) return VX_OK;
+    long n = (long)(hi_x - lo_x) * (hi_p - lo_p);
+    k_flags<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(*p, w[0], w[1], w[2], w[3], w[4], criteria,
+                                                                 lo_x, hi_x, lo_p, hi_p, flags);
+    return check_launch(ctx, "k_flags");
+}
+
+}  // extern "C"

@@ -296,7 +296,7 @@ dim3 grid_cells(int nx, int np, int extra_i, int extra_j) {
 
 extern "C" {
 
-int vx_fct_sub0(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl) {
+int vx_fct_sub0_fluxes(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl) {
     int rc = validate_patch(ctx, p);
     if (rc) return rc;
     if (step < 0 || step > 5 || !fl) return fail(ctx, VX_EINVAL, "sub0: bad step %d", step);
@@ -311,13 +311,24 @@ int vx_fct_sub0(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fi
         LaunchTimer lt(ctx, "k_sub0");
         k_sub0<<<grid, 256, 0, ctx->stream>>>(A);
     }
-    rc = check_launch(ctx, "k_sub0");
+    return check_launch(ctx, "k_sub0");
+}
+
+int vx_fct_sub0_predictor(vx_ctx *ctx, const vx_patch *p, int step, double dt) {
+    int rc = validate_patch(ctx, p);
     if (rc) return rc;
+    if (step < 0 || step > 5) return fail(ctx, VX_EINVAL, "sub0 predictor: bad step %d", step);
     double a[6], aSum;
     int ns;
     rk_coefficients(step, dt, a, ns, aSum);
     k_lo_predictor<<<grid_cells(p->n_x, p->n_p, 0, 0), 256, 0, ctx->stream>>>(*p, aSum);
     return check_launch(ctx, "k_lo_predictor");
+}
+
+int vx_fct_sub0(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl) {
+    int rc = vx_fct_sub0_fluxes(ctx, p, step, dt, fl);
+    if (rc) return rc;
+    return vx_fct_sub0_predictor(ctx, p, step, dt);
 }
 
 int vx_fct_sub1(vx_ctx *ctx, const vx_patch *p) {

@@ -31,6 +31,7 @@ typedef struct vx_scenario_params {
     double theta;             /* m_e c^2 */
     double amplitude, k, drift;
     double a0, n_over_nc, ramp, fwhm_cycles, lambda0;
+    double gap;               /* laser: vacuum gap (wavelengths) splitting the plateau, 0 = none */
 } vx_scenario_params;
 
 #ifdef __cplusplus

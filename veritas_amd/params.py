@@ -39,6 +39,7 @@ class ScenarioParams(ctypes.Structure):
         ("ramp", ctypes.c_double),
         ("fwhm_cycles", ctypes.c_double),
         ("lambda0", ctypes.c_double),
+        ("gap", ctypes.c_double),
     ]
 
     def replace(self, **kw):
@@ -58,7 +59,7 @@ def _base(**kw):
                        refinement_criteria=1e-8, size_weight=0.0, pre_length=0.0,
                        post_length=0.0, n_ref=1.0e24, length=math.pi / 2, pmin=-1.0,
                        pmax=1.0, theta=0.01, amplitude=0.01, k=4.0, drift=0.2, a0=0.5,
-                       n_over_nc=0.01, ramp=50.0, fwhm_cycles=10.0, lambda0=1.0e-6)
+                       n_over_nc=0.01, ramp=50.0, fwhm_cycles=10.0, lambda0=1.0e-6, gap=0.0)
     for k, v in kw.items():
         setattr(p, k, v)
     return p
@@ -77,5 +78,22 @@ def two_stream(**kw):
     seeded phases; x length 32 pi with Nx=4096, p in [-0.6, 0.6] with Np=4096."""
     d = dict(kind=TWO_STREAM, nx=4096, np=4096, length=32 * math.pi, pmin=-0.6, pmax=0.6,
              theta=1e-3, amplitude=1e-3, drift=0.2, modes=8)
+    d.update(kw)
+    return _base(**d)
+
+
+def laser(**kw):
+    """Linearly polarised Gaussian pulse (a0 = 0.5, FWHM 10 cycles) entering
+    from the left into underdense plasma: plateau density 0.01 n_c after a
+    10-wavelength vacuum gap and a 50-wavelength linear ramp, in a
+    400-wavelength domain; p in [-2, 2] m_e c.  Base grid Nx = 8192, Np = 512,
+    two refinement levels at ratio 2.  `wavelengths` sets the domain length in
+    laser wavelengths (lengths are otherwise in c/omega_p)."""
+    n_over_nc = kw.get("n_over_nc", 0.01)
+    waves = kw.pop("wavelengths", 400.0)
+    lam = 2 * math.pi * math.sqrt(n_over_nc)  # one wavelength in c/omega_p
+    d = dict(kind=LASER, nx=8192, np=512, levels=3, r=2, length=waves * lam, pmin=-2.0,
+             pmax=2.0, theta=1e-3, a0=0.5, n_over_nc=n_over_nc, ramp=50.0, fwhm_cycles=10.0,
+             lambda0=1.0e-6, refinement_criteria=1e-3, min_efficiency=0.6, size_weight=0.0)
     d.update(kw)
     return _base(**d)
