@@ -1,0 +1,113 @@
+"""GPU parity of the adaptive mesh and of the laser drive: hierarchy layout,
+refinement flags, per-patch f after steps with coarse-fine ghosts, coarse-fine
+fluxes, limiter-C sync and restriction, and the regrid transfer, all against
+the CPU oracle bit for bit; transverse fields and A^2 are non-zero here."""
+import sys
+
+import numpy as np
+import pytest
+
+import veritas_amd as va
+from _amr_cases import particle_number, two_slab_laser
+from _oracle import OracleRun
+from test_gpu_parity import assert_same, assert_ulp
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def compare_mesh(g, o, tag, what=("f1", "f0")):
+    lay_g, lay_o = g.layout(), o.layout()
+    assert np.array_equal(lay_g, lay_o), f"{tag}: layouts differ\n{lay_g}\n{lay_o}"
+    for lvl in sorted(set(lay_o[:, 0])):
+        n = int((lay_o[:, 0] == lvl).sum())
+        for k in range(n):
+            for w in what:
+                assert_same(f"{tag} level {lvl} patch {k} {w}", g.patch(lvl, k, w, ghosts=2),
+                            o.patch(lvl, k, w, ghosts=2))
+
+
+def compare_fields(g, o, tag):
+    assert_ulp(f"{tag} phi", g.get("phi"), o.get("phi"))
+    assert_ulp(f"{tag} E", g.get("E"), o.get("E"))
+    for w in ("By", "Bz", "Ey", "Ez", "Ay", "Az", "a2"):
+        assert_same(f"{tag} {w}", g.get(w), o.get(w))
+
+
+def test_laser_single_level_fused_matches():
+    """One walled patch: the fused stage kernel with a non-zero A^2 and the
+    laser-driven Maxwell stage."""
+    p = two_slab_laser(levels=1)
+    say("case: laser single level fused, 12 steps")
+    g, o = va.Run(p, fused=True), OracleRun(p)
+    dt = o.calc_dt(0.4)
+    assert g.calc_dt(0.4) == dt
+    g.advance(dt, 12)
+    o.advance(dt, 12)
+    assert_same("f1", g.get("f1"), o.get("f1"))
+    compare_fields(g, o, "laser")
+    assert np.abs(o.get("Bz")).max() > 0 and np.abs(o.get("a2")).max() > 0
+    g.close()
+    o.close()
+
+
+@pytest.fixture(scope="module")
+def amr_pair():
+    p = two_slab_laser()
+    say("case: two-slab laser, 3 levels")
+    g, o = va.Run(p, fused=False), OracleRun(p)
+    yield p, g, o
+    g.close()
+    o.close()
+
+
+def test_amr_initial_hierarchy_matches(amr_pair):
+    p, g, o = amr_pair
+    say("step: initial hierarchy")
+    compare_mesh(g, o, "initial")
+    for lvl in range(2):
+        assert np.array_equal(g.flags(lvl), o.flags(lvl)), f"flags of hierarchy level {lvl}"
+
+
+def test_amr_steps_and_regrid_match(amr_pair):
+    p, g, o = amr_pair
+    dt = o.calc_dt(0.4)
+    assert g.calc_dt(0.4) == dt
+    for n, tag in ((1, "after 1 step"), (3, "after 4 steps")):
+        say(f"step: advance {n}")
+        g.advance(dt, n)
+        o.advance(dt, n)
+        compare_mesh(g, o, tag)
+        compare_fields(g, o, tag)
+    for lvl in range(2):
+        assert np.array_equal(g.flags(lvl), o.flags(lvl)), f"flags of hierarchy level {lvl}"
+    say("step: regrid")
+    g.regrid()
+    o.regrid()
+    compare_mesh(g, o, "after regrid")
+    say("step: advance 3 after regrid")
+    g.advance(dt, 3)
+    o.advance(dt, 3)
+    compare_mesh(g, o, "after regrid + 3 steps")
+    compare_fields(g, o, "after regrid + 3 steps")
+    s = o.scalars()
+    n_g = particle_number(g_as_oracle_layout(g, o), s["dx"], s["dp"])
+    n_o = particle_number(o, s["dx"], s["dp"])
+    assert n_g == n_o
+
+
+class g_as_oracle_layout:
+    """GPU f1 with the oracle's nested masks (the layouts are equal)."""
+
+    def __init__(self, g, o):
+        self.g, self.o = g, o
+
+    def layout(self):
+        return self.g.layout()
+
+    def patch(self, lvl, k, w):
+        return self.o.patch(lvl, k, w) if w == "nested" else self.g.patch(lvl, k, w)
