@@ -24,6 +24,7 @@
 //    every earlier stage from stored stage values.
 // Every quantity is computed only inside the reference's loop range and is 0
 // outside it, as the reference's never-written array entries are.
+
 #include "vx_dev.h"
 #include "vx_internal.h"
 
@@ -164,7 +165,13 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     };
     load_h(r0 - 1);
 
-    for (int r = r0; r <= i1 + HALO - 1; r++) {
+    // The RK combination sum_k a_k FxH_k is evaluated left to right; the terms
+    // of the earlier stages are folded into Sx/Sp as soon as their prefetched
+    // values are consumed, which frees their registers a whole iteration early.
+    double Sx = 0.0, Sp = 0.0;
+
+    // One march row; each quantity is masked to its reference loop range.
+    auto row = [&](int r) {
         // ---- ingest row r
         fs_0 = pfs_a;
         f0_0 = pf0_a;
@@ -172,11 +179,14 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         pf0_a = pf0_b;
         pfs_b = ldp(fin, pitch, nx, np, r + 2, j);
         pf0_b = ldp(f0p, pitch, nx, np, r + 2, j);
-        double hxc[NH], hpc[NH];
+        if (STEP > 0) {
+            Sx = A.a[0] * hx[0];
+            Sp = A.a[0] * hp[0];
 #pragma unroll
-        for (int k = 0; k < NH; k++) {
-            hxc[k] = hx[k];
-            hpc[k] = hp[k];
+            for (int k = 1; k < NH; k++) {
+                Sx = Sx + A.a[k] * hx[k];
+                Sp = Sp + A.a[k] * hp[k];
+            }
         }
         load_h(r);
         double as_p1 = face_as(A, A.a2, r + 1);
@@ -186,19 +196,20 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
 
         // ---- x-face r-1: ex, fx (Rectangle.cpp:1278-1293), FxH_s (1313-1334)
         const int ix = r - 1;
+        const bool vFx = ix >= 0 && ix < nx + 1 && jr_ex;
         double Gd = shdn1(G_m1);  // Gamma at p-face j+1
         double ex = (ix >= -1 && ix < nx + 2 && jr_ex) ? dpcc * (Gd - G_m1) : 0.0;
-        double fx = (ix >= 0 && ix < nx + 1 && jr_ex) ? weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0) : 0.0;
+        double fx = vFx ? weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0) : 0.0;
         double ex_u = shdn1(ex), ex_d = shup1(ex);
         double fx_u = shdn1(fx), fx_d = shup1(fx);
-        const bool vFx = ix >= 0 && ix < nx + 1 && jr_ex;
         double FxH = vFx ? P.dx_inv * (fx * ex + w3 * (fx_u - fx_d) * (ex_u - ex_d)) : 0.0;
 
         // ---- column r: ep, fp (Rectangle.cpp:1294-1312)
-        double qE = (r >= -1 && r < nx + 1) ? col_qE(A, A.E, r) : 0.0;
-        ep_0 = (r >= -1 && r < nx + 1 && jr_ep) ? qE - ccdx * (G_p1 - G_0) : 0.0;
+        const bool cr = r >= -1 && r < nx + 1;
+        double qE = cr ? col_qE(A, A.E, r) : 0.0;
+        ep_0 = (cr && jr_ep) ? qE - ccdx * (G_p1 - G_0) : 0.0;
         double s1 = shup1(fs_0), s2 = shup1(s1), s3 = shdn1(fs_0);
-        fp_0 = (r >= -1 && r < nx + 1 && jr_fp) ? weno(s2, s1, fs_0, s3, ep_0 > 0.0) : 0.0;
+        fp_0 = (cr && jr_fp) ? weno(s2, s1, fs_0, s3, ep_0 > 0.0) : 0.0;
         // ---- column r-1: FpH_s (Rectangle.cpp:1353-1375)
         const bool vFp = ix >= -1 && ix < nx + 1 && jr_fp;
         double FpH = vFp ? P.dp_inv * (fp_m1 * ep_m1 + w3 * (fp_0 - fp_m2) * (ep_0 - ep_m2)) : 0.0;
@@ -214,10 +225,10 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         double G0d = shdn1(G0_0);
         double ex0 = dpcc * (G0d - G0_0);
         FxL_0 = (r >= 0 && r < nx + 1 && jr_ex) ? P.dx_inv * ((ex0 > 0.0 ? f0_m1 : f0_0) * ex0) : 0.0;
-        double qE0 = (r >= -1 && r < nx + 1) ? col_qE(A, A.E0, r) : 0.0;
+        double qE0 = cr ? col_qE(A, A.E0, r) : 0.0;
         double ep0 = qE0 - ccdx * (G0_p1 - G0_0);
         double f0_jm1 = shup1(f0_0);
-        FpL_0 = (r >= -1 && r < nx + 1 && jr_fp) ? P.dp_inv * ((ep0 > 0.0 ? f0_jm1 : f0_0) * ep0) : 0.0;
+        FpL_0 = (cr && jr_fp) ? P.dp_inv * ((ep0 > 0.0 ? f0_jm1 : f0_0) * ep0) : 0.0;
 
         // ---- cell r-1: low-order predictor (Rectangle.cpp:1518-1534, gather form)
         {
@@ -240,15 +251,8 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         // ---- x-face r-1 and column r-1: RK flux combination (Rectangle.cpp:1396-1517)
         {
             double xLS = A.aSum * FxL_m1, pLS = A.aSum * FpL_m1;
-            double sx = A.a[0] * (STEP == 0 ? FxH : hxc[0]);
-            double sp = A.a[0] * (STEP == 0 ? FpH : hpc[0]);
-#pragma unroll
-            for (int k = 1; k <= STEP; k++) {
-                double hxk = (k == STEP) ? FxH : hxc[k < NH ? k : 0];
-                double hpk = (k == STEP) ? FpH : hpc[k < NH ? k : 0];
-                sx = sx + A.a[k] * hxk;
-                sp = sp + A.a[k] * hpk;
-            }
+            double sx = STEP == 0 ? A.a[0] * FxH : Sx + A.a[STEP] * FxH;
+            double sp = STEP == 0 ? A.a[0] * FpH : Sp + A.a[STEP] * FpH;
             FxD_m1 = sx - xLS;
             FpD_m1 = sp - pLS;
         }
@@ -312,7 +316,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         FxD_m3 = FxD_m2; FxD_m2 = FxD_m1;
         FpD_m3 = FpD_m2; FpD_m2 = FpD_m1;
         Rp_m4 = Rp_m3; Rp_m3 = Rp_m2; Rm_m4 = Rm_m3; Rm_m3 = Rm_m2;
-    }
+    };
+
+    for (int r = r0; r <= i1 + HALO - 1; r++) row(r);
 }
 
 }  // namespace
