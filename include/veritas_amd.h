@@ -121,6 +121,11 @@ int vx_ghost_zero(vx_ctx *ctx, const vx_patch *p, int slot);
 /* p-moments of one patch (Rectangle::CalculateRhoAndJ, Rectangle.cpp:157-282):
  * chargeR and currentR, summed over p in the reference's order. */
 int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl);
+/* Rectangle::CalculateEnergy (Rectangle.cpp:284-305): energyR[j * rtb + k]
+ * (n_p * rtb doubles, device) = dx * sum over non-nested cells (i, j) of the
+ * p-sub-cell value k, summed in increasing i (the reference's loop races on
+ * energyR; this is its sequential order). */
+int vx_energy_patch(vx_ctx *ctx, const vx_patch *p, double *energyR);
 /* Level/Mesh/EMFieldSolver assembly of per-patch moments onto the finest grid
  * (Level.cpp:42-62, Mesh.cpp:52-56, EMSolver.cpp:104-122), patches given in the
  * reference's order (species, then levels finest->coarsest, then rectangles).

@@ -78,6 +78,9 @@ def host_lib():
         L.vxh_flags.restype = i
         L.vxh_flags.argtypes = [vp, i, i, vp, i]
         L.vxh_regrid.argtypes = [vp]
+        L.vxh_file_output.argtypes = [vp, d]
+        L.vxh_output_rectangles.argtypes = [vp, d]
+        L.vxh_output_flush.argtypes = [vp]
         _host = L
     return _host
 
@@ -212,3 +215,16 @@ class Run:
     def regrid(self):
         """SolverManager::reGrid (SolverManager.cpp:48-58)."""
         self._ck(host_lib().vxh_regrid(self.h))
+
+    # ---- output files (SolverManager::fileOutput / OutputRectangles)
+    def file_output(self, t):
+        """Append the field dumps of time t to output/ in the working directory."""
+        self._ck(host_lib().vxh_file_output(self.h, t))
+
+    def output_rectangles(self, t):
+        """Write output/rectangleData/rectangleData_p*_l*_t<t>.txt."""
+        self._ck(host_lib().vxh_output_rectangles(self.h, t))
+
+    def flush_output(self):
+        """Wait until every requested file is written."""
+        self._ck(host_lib().vxh_output_flush(self.h))

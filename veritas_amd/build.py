@@ -20,7 +20,7 @@ ARCH = os.environ.get("VX_ARCH", "gfx950")
 
 HIP_SOURCES = ["vx_runtime.hip", "vx_fct.hip", "vx_stage.hip", "vx_moments.hip", "vx_field.hip",
                "vx_amr.hip"]
-HOST_SOURCES = ["host/veritas.cpp", "host/amr.cpp", "host/host_capi.cpp"]
+HOST_SOURCES = ["host/veritas.cpp", "host/amr.cpp", "host/output.cpp", "host/host_capi.cpp"]
 
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -55,7 +55,7 @@ def build(verbose=False):
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", kern])
     host = os.path.join(LIB, "libveritas.so")
     _run(["g++", *HOST_FLAGS, "-shared", *[os.path.join(CSRC, s) for s in HOST_SOURCES],
-          f"-L{LIB}", "-lvx_kernels", "-Wl,-rpath,$ORIGIN", "-o", host])
+          f"-L{LIB}", "-lvx_kernels", "-lpthread", "-Wl,-rpath,$ORIGIN", "-o", host])
     return kern, host
 
 

@@ -179,6 +179,12 @@ int vxh_flags(void *h, int sp, int lvl, int *out, int n) {
 
 int vxh_regrid(void *h) { return guarded(h, [&](Run *r) { r->sm->reGrid(); }); }
 
+// SolverManager::fileOutput / OutputRectangles: text files under output/ of
+// the working directory, written by a background thread; vxh_output_flush waits.
+int vxh_file_output(void *h, double t) { return guarded(h, [&](Run *r) { r->sm->fileOutput(t); }); }
+int vxh_output_rectangles(void *h, double t) { return guarded(h, [&](Run *r) { r->sm->OutputRectangles(t); }); }
+int vxh_output_flush(void *h) { return guarded(h, [&](Run *r) { r->sm->FlushOutput(); }); }
+
 // Named arrays, same names and layouts as the oracle's C API.
 int vxh_get(void *h, const char *what, int ghosts, double *out, int n) {
     Run *r = (Run *)h;
@@ -201,6 +207,10 @@ int vxh_get(void *h, const char *what, int ghosts, double *out, int n) {
         if (w == "Ez") return copy_out(dl(r, fs.Ez, ne8), out, n);
         if (w == "Ay") return copy_out(dl(r, fs.Ay, ne8), out, n);
         if (w == "Az") return copy_out(dl(r, fs.Az, ne8), out, n);
+        if (w == "dNdP") {
+            r->sm->EMSolver->AssembleEnergy();
+            return copy_out(r->sm->EMSolver->energies.at(0), out, n);
+        }
         if (w == "scalars") {
             const vx_patch &d = p.desc();
             std::vector<double> ex0 = dl(r, fs.Ex0, 1);

@@ -412,6 +412,7 @@ EMFieldSolver::EMFieldSolver(Settings &s, const std::vector<std::shared_ptr<Mesh
     fs.Az = alloc((size_t)8 * fs.n_e);
     fs.a2 = alloc((size_t)N + 1);
     fs.work = alloc(vx_field_work_elems(N));
+    charges_species = alloc((size_t)N * s.q.size());
     double *E0 = alloc(N), *a20 = alloc((size_t)N + 1);
     fl.E0 = E0;
     fl.a20 = a20;
@@ -461,7 +462,7 @@ void EMFieldSolver::AssembleRhoAndJ() {
     for (auto &m : meshes)
         for (auto &l : m->levels)
             for (auto &r : l->rectangles) r->CalculateRhoAndJ();
-    dev->check(vx_moments_assemble(dev->ctx, plan, fs.charge, fs.J, nullptr), "assemble");
+    dev->check(vx_moments_assemble(dev->ctx, plan, fs.charge, fs.J, charges_species), "assemble");
 }
 
 void EMFieldSolver::UpdatePotential() { dev->check(vx_poisson(dev->ctx, &fs), "poisson"); }

@@ -9,6 +9,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../../include/veritas_amd.h"
@@ -128,6 +129,8 @@ public:
     std::vector<double> Download(const double *plane, int ghosts) const;
     std::vector<double> DownloadSlot(int slot, int ghosts) const { return Download(P.f[slot], ghosts); }
     const double *Plane(const std::string &name) const;
+    std::vector<double> CalculateEnergy();  // Rectangle.cpp:284-305 on the device
+    double *energyR_d = nullptr;
 
 private:
     Settings &settings;
@@ -190,6 +193,8 @@ public:
     void updateHierarchy(bool init = false);       // Mesh.cpp:132-210
     void getError(int lvl, bool init, std::vector<coords> &flagged);  // Mesh.cpp:212-295
     bool SinglePatch() const;
+    // Mesh.cpp:877-902: f slot 0 of every patch; returns the formatted files
+    std::vector<std::pair<std::string, std::string>> rectangleData(double tidx) const;
     void Sub0(double timeStep, int step);  // sub-step 0 of every patch with the coarse-fine fluxes
     bool fused = true;  // single walled patch: one fused call per stage
 
@@ -224,6 +229,11 @@ public:
     const vx_field_state &state() const { return fs; }
     void RebuildMomentPlan();
     void SnapshotStage0Fields();  // E and A^2 of RK stage 0 for the fused stage
+    // output (host/output.cpp)
+    void AssembleEnergy();                          // EMSolver.cpp:124-131
+    std::vector<std::vector<double>> energies;      // dNdP per species, finest p grid
+    std::vector<double> Download(const double *d, size_t n) const;
+    double *charges_species = nullptr;              // n_species * n_f, last assembly
 
 private:
     Settings &settings;
@@ -244,6 +254,12 @@ public:
     void AdvanceFields(double timeStep);
     double CalculateDt(double cfl) { return cfl * EMSolver->EstimateCFLBound(); }
     void reGrid();  // SolverManager.cpp:48-58
+    // SolverManager.cpp:103-160 and 64-70: the arrays are copied from the
+    // device, then formatted and appended to output/ on a writer thread
+    void fileOutput(double t);
+    void OutputRectangles(double t);
+    void FlushOutput();  // wait for the writer thread
+    ~SolverManager();
     // pieces of Advance for stage-by-stage checks
     void StageMoments() { EMSolver->AssembleRhoAndJ(); }
     void StagePoisson() { EMSolver->UpdatePotential(); }
@@ -255,6 +271,10 @@ public:
     std::vector<std::shared_ptr<Mesh>> meshes;
     std::shared_ptr<EMFieldSolver> EMSolver;
     Settings &settings;
+
+private:
+    std::shared_ptr<struct OutputFiles> files;
+    std::thread writer;
 };
 
 }  // namespace vxh

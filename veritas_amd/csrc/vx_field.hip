@@ -55,43 +55,53 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// exclusive prefix sum of v[0..n) into out[0..n) (out may alias v); each
-// thread owns a contiguous chunk; fixed (deterministic) order
+// exclusive prefix sum of v[0..n) into out[0..n) (out may alias v), in
+// tiles of PT * 4 entries: each thread takes 4 consecutive entries of a tile
+// (a wavefront reads 2 KB contiguously), the tile's thread totals are scanned
+// across the workgroup and a running carry links the tiles.  Fixed order.
 __device__ void block_exclusive_scan(const double *v, double *out, int n, double *lds) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int chunk = (n + PT - 1) / PT;
-    const int a = t * chunk, b = min(n, a + chunk);
-    double s = 0.0;
-    for (int i = a; i < b; i++) s += v[i];
-    double inc = wave_incl_scan(s);
-    if (lane == 63) lds[w] = inc;
-    __syncthreads();
-    if (t == 0) {
-        double run = 0.0;
-        for (int k = 0; k < PW; k++) {
-            double tot = lds[k];
-            lds[PW + k] = run;
-            run += tot;
+    constexpr int PER = 4, TILE = PT * PER;
+    double carry = 0.0;
+    for (int base = 0; base < n; base += TILE) {
+        const int a = base + t * PER;
+        double e[PER];
+        double s = 0.0;
+#pragma unroll
+        for (int k = 0; k < PER; k++) {
+            e[k] = (a + k < n) ? v[a + k] : 0.0;
+            s += e[k];
         }
+        double inc = wave_incl_scan(s);
+        if (lane == 63) lds[w] = inc;
+        __syncthreads();
+        if (t == 0) {
+            double run = 0.0;
+            for (int k = 0; k < PW; k++) {
+                double tot = lds[k];
+                lds[PW + k] = run;
+                run += tot;
+            }
+            lds[2 * PW] = run;
+        }
+        __syncthreads();
+        double run = carry + (lds[PW + w] + (inc - s));
+#pragma unroll
+        for (int k = 0; k < PER; k++)
+            if (a + k < n) {
+                out[a + k] = run;
+                run += e[k];
+            }
+        carry += lds[2 * PW];
+        __syncthreads();
     }
-    __syncthreads();
-    double run = lds[PW + w] + (inc - s);
-    for (int i = a; i < b; i++) {
-        double vi = v[i];
-        out[i] = run;
-        run += vi;
-    }
-    __syncthreads();
 }
 
-// block sum of v[lo..hi), deterministic order
+// block sum of v[lo..hi), deterministic order (coalesced strided partials)
 __device__ double block_sum(const double *v, int lo, int hi, double *lds) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int n = hi - lo;
-    const int chunk = (n + PT - 1) / PT;
-    const int a = lo + t * chunk, b = min(hi, a + chunk);
     double s = 0.0;
-    for (int i = a; i < b; i++) s += v[i];
+    for (int i = lo + t; i < hi; i += PT) s += v[i];
     s = wave_sum(s);
     if (lane == 0) lds[w] = s;
     __syncthreads();

@@ -19,6 +19,9 @@ struct vx_ctx {
     std::string timed_kernel;
     std::vector<hipEvent_t> tev;  // pairs: start, stop
     size_t tcount = 0;
+    // p-moment logarithm table of the patch being reduced (vx_moments_patch)
+    double *mom_table = nullptr;
+    size_t mom_table_n = 0;
 };
 
 struct vx_moment_plan {

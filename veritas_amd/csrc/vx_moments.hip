@@ -35,6 +35,11 @@ struct MomArgs {
     double inv_rtb;       // 1.0 / relativeToBottom
     double dpq, jscale;   // dp*q and -q*q/mass
     double rel[3 * MAX_RTB];  // interpolatCoefsREL (Rectangle.cpp:103-110)
+    // When every sub-column of the patch has the same A^2 (the logarithm terms
+    // then depend on p only), tab[t] = c2 log(h(t)/h(t-1)) for t in [0, n_p+2)
+    // with h(j) = Gamma(p_j) + c1 p_j, and tab[n_p+2] = 1; otherwise
+    // tab[n_p+2] = 0 and the kernel evaluates them per column group.
+    const double *tab;
 };
 
 // GetInterpolantsREL(f1..f5)[k] + correction (Rectangle.cpp:139-155, 195-204)
@@ -66,6 +71,28 @@ __device__ __forceinline__ double cell_a2(const vx_fields &fl, long i) {
     return (ay * ay) + (az * az);
 }
 
+// Logarithm table of one patch (see MomArgs::tab); one workgroup.
+template <int RTB>
+__global__ __launch_bounds__(1024) void k_moment_table(MomArgs A, double *tab) {
+    const vx_patch &P = A.p;
+    const int t = threadIdx.x;
+    const double a0 = P.q * P.q * cell_a2(A.fl, (long)P.x_pos * RTB);
+    int same = 1;
+    for (int s = t; s < P.n_x * RTB; s += blockDim.x)
+        same &= (P.q * P.q * cell_a2(A.fl, (long)P.x_pos * RTB + s)) == a0;
+    same = __syncthreads_and(same);
+    if (!same) {
+        if (t == 0) tab[P.n_p + 2] = 0.0;
+        return;
+    }
+    auto h = [&](int j) {
+        double pm = momentum(P.pmin, P.dp, j, P.p_pos);
+        return gamma_(pm, a0, P.mc2inv) + A.c1 * pm;
+    };
+    for (int e = t; e < P.n_p + 2; e += blockDim.x) tab[e] = A.c2 * vx_log(h(e) / h(e - 1));
+    if (t == 0) tab[P.n_p + 2] = 1.0;
+}
+
 // LDS exchange inside one wavefront: order the lanes' LDS writes before the
 // other lanes' reads (no workgroup barrier is needed; waves are independent).
 __device__ __forceinline__ void wave_sync() {
@@ -94,6 +121,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     __shared__ double Lb[4][NCH][68];  // c2 log(h(j+1)/h(j)), p index jb-1+e
     __shared__ double TR[4][NCH][65], TJ[4][NCH][65];
     if (i0 >= P.n_x) return;  // uniform per wavefront
+    const bool use_tab = A.tab[P.n_p + 2] != 0.0;  // uniform A^2: tabulated logarithms
     const int pitch = P.pitch;
     const double *g = P.f[1];
     double asq[NCH];
@@ -134,7 +162,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             }
         }
         // logarithm terms, computed once per distinct A^2 among the chains
-        for (int ch = 0; ch < NCH; ch++) {
+        for (int ch = 0; ch < NCH && !use_tab; ch++) {
             if (ch > 0 && asq[ch] == asq[ch - 1]) {
                 for (int e = lane; e < 66; e += 64) Lb[wave][ch][e] = Lb[wave][ch - 1][e];
                 continue;
@@ -158,7 +186,16 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             double tr = -0.0, tj = -0.0;
             if (!skip) {
                 double tm = S[wave][ch][lane], t0 = S[wave][ch][lane + 1], tp = S[wave][ch][lane + 2];
-                double gm = Lb[wave][ch][lane + 1], gm1 = Lb[wave][ch][lane], gp1 = Lb[wave][ch][lane + 2];
+                double gm, gm1, gp1;
+                if (use_tab) {
+                    gm = A.tab[j + 1];
+                    gm1 = A.tab[j];
+                    gp1 = A.tab[j + 2];
+                } else {
+                    gm = Lb[wave][ch][lane + 1];
+                    gm1 = Lb[wave][ch][lane];
+                    gp1 = Lb[wave][ch][lane + 2];
+                }
                 tr = t0;
                 tj = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
             }
@@ -188,6 +225,33 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             else P.currentR[(long)i * RTB + k] = acc * A.jscale;
         }
     }
+}
+
+// Rectangle::CalculateEnergy (Rectangle.cpp:284-305) without its data race:
+// chain (j, k) sums the p-sub-cell value k of every non-nested cell (i, j) in
+// increasing i, then scales by dx (the reference's OpenMP loop adds to the same
+// energyR entries from several threads; the oracle and this kernel use the
+// sequential order).
+template <int RTB>
+__global__ __launch_bounds__(256) void k_energy(MomArgs A, double *energyR) {
+    const vx_patch &P = A.p;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= P.n_p * RTB) return;
+    const int j = e / RTB, k = e % RTB;
+    const int pitch = P.pitch;
+    const double *g = P.f[1];
+    double acc = 0.0;
+    for (int i = 0; i < P.n_x; i++) {
+        if (P.nested && P.nested[pidx(pitch, i, j)]) continue;
+        double f1 = g[pidx(pitch, i, j - 2)], f2 = g[pidx(pitch, i, j - 1)], f3 = g[pidx(pitch, i, j)];
+        double f4 = g[pidx(pitch, i, j + 1)], f5 = g[pidx(pitch, i, j + 2)];
+        f5 -= f3; f4 -= f3; f2 -= f3; f1 -= f3;
+        double a1 = kInterp[0] * f1 + kInterp[1] * f2 + kInterp[2] * f4 + kInterp[3] * f5;
+        double a2 = kInterp[4] * f1 + kInterp[5] * f2 + kInterp[6] * f4 + kInterp[7] * f5;
+        double a3 = kInterp[8] * f1 + kInterp[9] * f2 + kInterp[10] * f4 + kInterp[11] * f5;
+        acc += A.rel[3 * k] * a1 + A.rel[3 * k + 1] * a2 + A.rel[3 * k + 2] * a3 + f3;
+    }
+    energyR[(long)j * RTB + k] = acc * P.dx;
 }
 
 struct MomentSrc {
@@ -226,19 +290,9 @@ __global__ void k_assemble(const MomentSrc *src, const int *level_start, int n_s
     J[x] = jsum;
 }
 
-}  // namespace
-
-extern "C" {
-
-int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
-    int rc = validate_patch(ctx, p);
-    if (rc) return rc;
-    if (p->rtb > MAX_RTB) return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d > %d", p->rtb, MAX_RTB);
-    if (!fl) return fail(ctx, VX_EINVAL, "moments: null fields");
-    if ((rc = validate_span(ctx, p, fl->n_f))) return rc;
-    MomArgs A;
+void fill_args(MomArgs &A, const vx_patch *p, const vx_fields *fl) {
     A.p = *p;
-    A.fl = *fl;
+    if (fl) A.fl = *fl;
     double m_inv = 1 / p->m;
     A.c1 = m_inv * vx_c_inv;
     A.c2 = 1 / A.c1;
@@ -248,13 +302,62 @@ int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
     A.dpq = p->dp * p->q;
     A.jscale = -p->q * p->q / p->m;
     for (int i = 0; i < 3 * MAX_RTB; i++) A.rel[i] = 0.0;
-    for (int i = 0; i < p->rtb; i++) {
+    for (int i = 0; i < p->rtb; i++) {  // Rectangle.cpp:103-110
         double tl = -0.5 + i / rtb;
         double tr = -0.5 + (i + 1.0) / rtb;
         A.rel[3 * i] = (tl + tr) * 0.5;
         A.rel[3 * i + 1] = (tl * tl + tl * tr + tr * tr) / 3.0 - (1.0 / 12);
         A.rel[3 * i + 2] = (tl * tl * tl + tl * tl * tr + tl * tr * tr + tr * tr * tr) * 0.25;
     }
+    A.tab = nullptr;
+}
+
+}  // namespace
+
+extern "C" {
+
+int vx_energy_patch(vx_ctx *ctx, const vx_patch *p, double *energyR) {
+    int rc = validate_patch(ctx, p);
+    if (rc) return rc;
+    if (!energyR) return fail(ctx, VX_EINVAL, "energy: null output");
+    MomArgs A{};
+    fill_args(A, p, nullptr);
+    const unsigned nb = (unsigned)((p->n_p * p->rtb + 255) / 256);
+    switch (p->rtb) {
+        case 1: k_energy<1><<<nb, 256, 0, ctx->stream>>>(A, energyR); break;
+        case 2: k_energy<2><<<nb, 256, 0, ctx->stream>>>(A, energyR); break;
+        case 4: k_energy<4><<<nb, 256, 0, ctx->stream>>>(A, energyR); break;
+        default: return fail(ctx, VX_ENOTSUP, "energy: relativeToBottom %d not in {1,2,4}", p->rtb);
+    }
+    return check_launch(ctx, "k_energy");
+}
+
+int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
+    int rc = validate_patch(ctx, p);
+    if (rc) return rc;
+    if (p->rtb > MAX_RTB) return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d > %d", p->rtb, MAX_RTB);
+    if (!fl) return fail(ctx, VX_EINVAL, "moments: null fields");
+    if ((rc = validate_span(ctx, p, fl->n_f))) return rc;
+    MomArgs A;
+    fill_args(A, p, fl);
+    const size_t need = (size_t)p->n_p + 3;
+
+    if (ctx->mom_table_n < need) {
+        if (ctx->mom_table) (void)hipFree(ctx->mom_table);
+        ctx->mom_table = nullptr;
+        ctx->mom_table_n = 0;
+        if (hipMalloc(&ctx->mom_table, sizeof(double) * need) != hipSuccess)
+            return fail(ctx, VX_ENOMEM, "moments: logarithm table");
+        ctx->mom_table_n = need;
+    }
+    A.tab = ctx->mom_table;
+    switch (p->rtb) {
+        case 1: k_moment_table<1><<<1, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
+        case 2: k_moment_table<2><<<1, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
+        case 4: k_moment_table<4><<<1, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
+        default: return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d not in {1,2,4}", p->rtb);
+    }
+    if ((rc = check_launch(ctx, "k_moment_table"))) return rc;
     LaunchTimer lt(ctx, "k_moments");
     switch (p->rtb) {
         case 1: k_moments<1, 4><<<(unsigned)((p->n_x + 15) / 16), 256, 0, ctx->stream>>>(A); break;

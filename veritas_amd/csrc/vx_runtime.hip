@@ -32,6 +32,7 @@ void vx_ctx_destroy(vx_ctx *ctx) {
     for (auto &e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->tev) (void)hipEventDestroy(e);
+    if (ctx->mom_table) (void)hipFree(ctx->mom_table);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
