@@ -81,6 +81,8 @@ def host_lib():
         L.vxh_file_output.argtypes = [vp, d]
         L.vxh_output_rectangles.argtypes = [vp, d]
         L.vxh_output_flush.argtypes = [vp]
+        L.vxh_save.argtypes = [vp, ctypes.c_char_p]
+        L.vxh_load.argtypes = [vp, ctypes.c_char_p]
         _host = L
     return _host
 
@@ -228,3 +230,12 @@ class Run:
     def flush_output(self):
         """Wait until every requested file is written."""
         self._ck(host_lib().vxh_output_flush(self.h))
+
+    # ---- checkpoint/restart
+    def save(self, path):
+        """Write a checkpoint (hierarchy, f, fields, clock) to `path`."""
+        self._ck(host_lib().vxh_save(self.h, os.fspath(path).encode()))
+
+    def load(self, path):
+        """Replace this run's state with a checkpoint of a run of the same scenario."""
+        self._ck(host_lib().vxh_load(self.h, os.fspath(path).encode()))

@@ -185,6 +185,9 @@ int vxh_file_output(void *h, double t) { return guarded(h, [&](Run *r) { r->sm->
 int vxh_output_rectangles(void *h, double t) { return guarded(h, [&](Run *r) { r->sm->OutputRectangles(t); }); }
 int vxh_output_flush(void *h) { return guarded(h, [&](Run *r) { r->sm->FlushOutput(); }); }
 
+int vxh_save(void *h, const char *path) { return guarded(h, [&](Run *r) { r->sm->SaveCheckpoint(path); }); }
+int vxh_load(void *h, const char *path) { return guarded(h, [&](Run *r) { r->sm->LoadCheckpoint(path); }); }
+
 // Named arrays, same names and layouts as the oracle's C API.
 int vxh_get(void *h, const char *what, int ghosts, double *out, int n) {
     Run *r = (Run *)h;

@@ -129,6 +129,7 @@ public:
     std::vector<double> Download(const double *plane, int ghosts) const;
     std::vector<double> DownloadSlot(int slot, int ghosts) const { return Download(P.f[slot], ghosts); }
     const double *Plane(const std::string &name) const;
+    void UploadSlot(int slot, const std::vector<double> &full);  // whole plane incl. ghosts
     std::vector<double> CalculateEnergy();  // Rectangle.cpp:284-305 on the device
     double *energyR_d = nullptr;
 
@@ -193,6 +194,7 @@ public:
     void updateHierarchy(bool init = false);       // Mesh.cpp:132-210
     void getError(int lvl, bool init, std::vector<coords> &flagged);  // Mesh.cpp:212-295
     bool SinglePatch() const;
+    void Restore(const std::vector<level> &h);  // rebuild patches for a saved hierarchy
     // Mesh.cpp:877-902: f slot 0 of every patch; returns the formatted files
     std::vector<std::pair<std::string, std::string>> rectangleData(double tidx) const;
     void Sub0(double timeStep, int step);  // sub-step 0 of every patch with the coarse-fine fluxes
@@ -259,6 +261,10 @@ public:
     void fileOutput(double t);
     void OutputRectangles(double t);
     void FlushOutput();  // wait for the writer thread
+    // checkpoint/restart (host/checkpoint.cpp; the reference has none): the
+    // hierarchy, f slots 0 and 1 of every patch, the field state and the clock
+    void SaveCheckpoint(const std::string &path);
+    void LoadCheckpoint(const std::string &path);
     ~SolverManager();
     // pieces of Advance for stage-by-stage checks
     void StageMoments() { EMSolver->AssembleRhoAndJ(); }
