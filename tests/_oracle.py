@@ -78,6 +78,10 @@ class OracleRun:
     def advance(self, dt, nsteps=1):
         lib().vxo_advance(self.h, dt, nsteps)
 
+    def advance_fields(self, dt, nsteps=1):
+        """Field-only steps (SolverManager::AdvanceFields): f and J frozen."""
+        lib().vxo_advance_fields(self.h, dt, nsteps)
+
     def get(self, what, ghosts=0):
         n = lib().vxo_get(self.h, what.encode(), ghosts, None, 0)
         if n < 0:
