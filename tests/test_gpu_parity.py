@@ -161,16 +161,19 @@ def test_two_stream_full_size_two_steps():
     assert abs(g.get("f1").sum() / n0 - 1) <= 1e-13
 
 
-def test_large_finest_grid_poisson_path():
-    """N_f = 8192 takes the tiled single-workgroup Poisson kernel (the 4096-cell
-    cases above take the register variant): phi, E and f after 4 steps."""
-    p = landau(nx=8192, np=16)
+@pytest.mark.parametrize("nx", [8192, 20000, 32768])
+def test_large_finest_grid_poisson_path(nx):
+    """N_f above 4096 takes the multi-workgroup Poisson launches (the 4096-cell
+    cases above take the register variant); 20000 leaves a partial last chunk
+    and 32768 is the finest grid of the three-level laser workload: phi, E and
+    f after 4 steps."""
+    p = landau(nx=nx, np=16)
     g, o = va.Run(p, fused=True), OracleRun(p)
     dt = o.calc_dt(0.4)
     assert g.calc_dt(0.4) == dt
     g.advance(dt, 4)
     o.advance(dt, 4)
-    compare_fields(g, o, "N_f 8192")
+    compare_fields(g, o, f"N_f {nx}")
     assert_same("f1", g.get("f1"), o.get("f1"))
     g.close()
     o.close()

@@ -2,8 +2,9 @@
 //
 // Poisson (EMSolver.cpp:28-88,156-192).  The reference factors the dense N x N
 // matrix with LAPACK and solves it at every RK stage, O(N^2) per solve.  Here
-// the same matrix is solved to near correct rounding in O(N) by one
-// 1024-thread workgroup:
+// the same matrix is solved to near correct rounding in O(N) (one 1024-thread
+// workgroup with the vector in registers up to N = 4096, a sequence of
+// multi-workgroup launches above):
 //   rows 1..N-1 of the matrix are the periodic 4th-order stencil
 //   (w2, w1, w3, w1, w2) with column 0 removed, i.e. the periodic operator with
 //   phi_0 pinned to 0; row 0 then gives phi_0.  The periodic operator factors
@@ -36,9 +37,6 @@ struct PoissonArgs {
     double invN;
 };
 
-// Work layout (doubles): b | x | r | c | y | tmp | partials
-__host__ __device__ inline size_t wk_off(int n, int which) { return (size_t)which * n; }
-
 constexpr int PW = PT / 64;  // wavefronts in the Poisson workgroup
 
 __device__ __forceinline__ double wave_incl_scan(double v) {
@@ -55,138 +53,6 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// exclusive prefix sum of v[0..n) into out[0..n) (out may alias v), in
-// tiles of PT * 4 entries: each thread takes 4 consecutive entries of a tile
-// (a wavefront reads 2 KB contiguously), the tile's thread totals are scanned
-// across the workgroup and a running carry links the tiles.  Fixed order.
-__device__ void block_exclusive_scan(const double *v, double *out, int n, double *lds) {
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    constexpr int PER = 4, TILE = PT * PER;
-    double carry = 0.0;
-    for (int base = 0; base < n; base += TILE) {
-        const int a = base + t * PER;
-        double e[PER];
-        double s = 0.0;
-#pragma unroll
-        for (int k = 0; k < PER; k++) {
-            e[k] = (a + k < n) ? v[a + k] : 0.0;
-            s += e[k];
-        }
-        double inc = wave_incl_scan(s);
-        if (lane == 63) lds[w] = inc;
-        __syncthreads();
-        if (t == 0) {
-            double run = 0.0;
-            for (int k = 0; k < PW; k++) {
-                double tot = lds[k];
-                lds[PW + k] = run;
-                run += tot;
-            }
-            lds[2 * PW] = run;
-        }
-        __syncthreads();
-        double run = carry + (lds[PW + w] + (inc - s));
-#pragma unroll
-        for (int k = 0; k < PER; k++)
-            if (a + k < n) {
-                out[a + k] = run;
-                run += e[k];
-            }
-        carry += lds[2 * PW];
-        __syncthreads();
-    }
-}
-
-// block sum of v[lo..hi), deterministic order (coalesced strided partials)
-__device__ double block_sum(const double *v, int lo, int hi, double *lds) {
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    double s = 0.0;
-    for (int i = lo + t; i < hi; i += PT) s += v[i];
-    s = wave_sum(s);
-    if (lane == 0) lds[w] = s;
-    __syncthreads();
-    if (t == 0) {
-        double tot = 0.0;
-        for (int k = 0; k < PW; k++) tot += lds[k];
-        lds[2 * PW] = tot;
-    }
-    __syncthreads();
-    double r = lds[2 * PW];
-    __syncthreads();
-    return r;
-}
-
-// Approximate solve of A c = r (A as EMSolver.cpp:41-67 assembles it).
-__device__ void approx_solve(const PoissonArgs &P, const double *r, double *c, double *y,
-                             double *tmp, double *lds) {
-    const int N = P.fs.n_f;
-    const int t = threadIdx.x;
-    // zero-mean right-hand side of the periodic system: r'_0 = -sum_{i>=1} r_i
-    double tail = block_sum(r, 1, N, lds);
-    for (int i = t; i < N; i += PT) tmp[i] = (i == 0) ? -tail : r[i];
-    __syncthreads();
-    // L2 y = r' with L2 = [-1, 2, -1]: the differences d_i = y_i - y_{i-1}
-    // obey d_i = d_0 - S_i with S_i = sum_{k<i} r'_k, and periodicity fixes
-    // d_0 = mean(S).  With y_0 = 0, y_i = sum_{k=1..i} d_k.
-    block_exclusive_scan(tmp, tmp, N, lds);  // tmp = S
-    double d0 = block_sum(tmp, 0, N, lds) * P.invN;
-    for (int i = t; i < N; i += PT) y[i] = (i == 0) ? 0.0 : d0 - tmp[i];  // D_i = d_i (D_0 = 0)
-    __syncthreads();
-    block_exclusive_scan(y, tmp, N, lds);  // tmp_i = sum_{k<i} D_k
-    for (int i = t; i < N; i += PT) tmp[i] = tmp[i] + y[i];  // y_i = sum_{k<=i} D_k
-    __syncthreads();
-    // c' = M^-1 y (periodic 33-tap convolution)
-    for (int i = t; i < N; i += PT) {
-        double s = P.g[0] * tmp[i];
-        for (int k = 1; k < NG; k++) {
-            int ip = i + k, im = i - k;
-            ip -= ip >= N ? N : 0;
-            im += im < 0 ? N : 0;
-            s += P.g[k] * (tmp[ip] + tmp[im]);
-        }
-        c[i] = s;
-    }
-    __syncthreads();
-    double c0 = c[0];
-    __syncthreads();
-    for (int i = t; i < N; i += PT)
-        if (i > 0) c[i] = c[i] - c0;
-    __syncthreads();
-    if (t == 0)
-        c[0] = r[0] - (((P.w1 * c[1] + P.w2 * c[2]) + P.w2 * c[N - 2]) + P.w1 * c[N - 1]);
-    __syncthreads();
-}
-
-// res = b - A x in double-double, rounded once (term order as the oracle's)
-__device__ void residual(const PoissonArgs &P, const double *b, const double *x, double *res) {
-    const int N = P.fs.n_f;
-    for (int i = threadIdx.x; i < N; i += PT) {
-        double sh = b[i], sl = 0.0;
-        auto sub = [&](double a, double xv) {
-            double p = a * xv;
-            double e = fma(a, xv, -p);
-            double tt = sh - p;
-            double bv = tt - sh;
-            double err = (sh - (tt - bv)) + (-p - bv);
-            sl = sl + (err - e);
-            sh = tt;
-        };
-        if (i == 0) {
-            sub(1.0, x[0]); sub(P.w1, x[1]); sub(P.w2, x[2]); sub(P.w2, x[N - 2]); sub(P.w1, x[N - 1]);
-        } else {
-            const double cf[5] = {P.w2, P.w1, P.w3, P.w1, P.w2};
-            for (int d = -2; d <= 2; d++) {
-                int j = i + d;
-                j = j < 0 ? j + N : (j >= N ? j - N : j);
-                if (j == 0) continue;
-                sub(cf[d + 2], x[j]);
-            }
-        }
-        res[i] = sh + sl;
-    }
-    __syncthreads();
-}
-
 // EMFieldSolver::GetEfield (EMSolver.cpp:137-154)
 __device__ __forceinline__ double efield(const double *phi, int N, double fc, double ex0, int i) {
     int ip1 = i + 1, im1 = i - 1, ip2 = i + 2, im2 = i - 2;
@@ -201,42 +67,263 @@ __device__ __forceinline__ double efield(const double *phi, int N, double fc, do
     return -fc * (8 * (phi[ip1] - phi[im1]) - phi[ip2] + phi[im2]) + ex0;
 }
 
-__global__ __launch_bounds__(PT) void k_poisson(PoissonArgs P) {
+// ------------------------------------------------ multi-workgroup solve, n_f > 4096
+// Workgroup b owns the CHUNK consecutive entries [b*CHUNK, b*CHUNK + CHUNK).
+// One approximate solve of A c = v (the same steps as the single-workgroup
+// variant below) is three launches:
+//   k_pm_scan_s  S_i = sum_{k<i} v'_k with v'_0 = -sum_{k>=1} v_k, and per-chunk sums of S;
+//   k_pm_scan_y  d0 = mean(S), D_i = d0 - S_i (D_0 = 0), y_i = sum_{k<=i} D_k;
+//   k_pm_conv    c = M^-1 y - (M^-1 y)_0 with c_0 from row 0, x (+)= c, then either the
+//                next residual r = b - A x (double-double) with its per-chunk sums, or
+//                phi, Ex0 and E.
+// A sum across workgroups is formed from per-chunk partials that every workgroup
+// reads and adds in the same order, so all of them hold the same bits.  k_pm_conv
+// recomputes x on a 2-entry halo around its chunk (for the residual and E
+// stencils) and at the six entries around 0 (row 0 and Ex0) with the owner's
+// operations, so every copy of an entry has the owner's bits.  x and r alternate
+// between two buffers: a launch reads one and writes the other.
+constexpr int MT = 256, MW = MT / 64, MPER = 4, CHUNK = MT * MPER;
+constexpr int HALO = 2;
+constexpr int CW = NG - 1 + HALO;  // y window margin: convolution reach + halo
+constexpr int NSPEC = 6;           // entries N-3, N-2, N-1, 0, 1, 2
+
+__host__ __device__ inline int pm_chunks(int n) { return (n + CHUNK - 1) / CHUNK; }
+
+// Work layout (doubles): b | xa | xb | ra | rb | S | y | Pa[nb] | Pb[nb] | Ex0 on entry
+struct PmWork {
+    double *b, *xa, *xb, *ra, *rb, *S, *y, *Pa, *Pb, *ex0;
+};
+__host__ __device__ inline PmWork pm_work(double *w, int n) {
+    const size_t N = n;
+    const int nb = pm_chunks(n);
+    PmWork W;
+    W.b = w;
+    W.xa = w + N;
+    W.xb = w + 2 * N;
+    W.ra = w + 3 * N;
+    W.rb = w + 4 * N;
+    W.S = w + 5 * N;
+    W.y = w + 6 * N;
+    W.Pa = w + 7 * N;
+    W.Pb = W.Pa + nb;
+    W.ex0 = W.Pb + nb;
+    return W;
+}
+
+struct PoissonMB {
+    vx_field_state fs;
+    double w1, w2, w3;
+    double g[NG];
+    double invN;
+    int nb;
+    PmWork W;
+    const double *in;    // right-hand side of this solve (b, then residuals)
+    const double *x_in;  // x before this solve's correction
+    double *x_out;       // x after it
+    double *r_out;       // next residual
+};
+
+__device__ __forceinline__ int wrap(int u, int N) { return u < 0 ? u + N : (u >= N ? u - N : u); }
+
+// sum of v[lo..hi) in a fixed order; every lane of every wave gets the same bits
+__device__ __forceinline__ double wave_range_sum(const double *v, int lo, int hi) {
+    double s = 0.0;
+    for (int k = lo + (threadIdx.x & 63); k < hi; k += 64) s += v[k];
+    return wave_sum(s);
+}
+
+__device__ __forceinline__ double mb_sum(double v, double *lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    v = wave_sum(v);
+    if (lane == 0) lds[w] = v;
+    __syncthreads();
+    double r = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+    __syncthreads();
+    return r;
+}
+
+// exclusive prefix of the per-thread totals over the workgroup
+__device__ __forceinline__ double mb_offset(double tot, double *lds) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double inc = wave_incl_scan(tot);
+    double exc = __shfl_up(inc, 1, 64);
+    if (lane == 0) exc = 0.0;
+    if (lane == 63) lds[w] = inc;
+    __syncthreads();
+    double off = 0.0;
+    for (int k = 0; k < w; k++) off += lds[k];
+    __syncthreads();
+    return off + exc;
+}
+
+// (M^-1 y) at window position c: the periodic 33-tap convolution
+__device__ __forceinline__ double conv_at(const double *w, int c, const double *g) {
+    double s = g[0] * w[c];
+    for (int k = 1; k < NG; k++) s += g[k] * (w[c + k] + w[c - k]);
+    return s;
+}
+
+// b = dx^2 eps0_inv (charge + neutral) (EMSolver.cpp:157-167), its chunk sums
+// over entries >= 1, and the entry value of Ex0
+__global__ __launch_bounds__(MT) void k_pm_rhs(PoissonMB P) {
+    __shared__ double lds[MW];
     const vx_field_state &F = P.fs;
-    const int N = F.n_f;
-    const int t = threadIdx.x;
-    __shared__ double lds[PT];
-    double *b = F.work + wk_off(N, 0);
-    double *x = F.work + wk_off(N, 1);
-    double *r = F.work + wk_off(N, 2);
-    double *c = F.work + wk_off(N, 3);
-    double *y = F.work + wk_off(N, 4);
-    double *tmp = F.work + wk_off(N, 5);
-    // EMSolver.cpp:157-167: PHI = eps0_inv (charge + neutral), then * dx^2
+    const int N = F.n_f, base = blockIdx.x * CHUNK + threadIdx.x * MPER;
     const double te = vx_eps0_inv;
-    for (int i = t; i < N; i += PT) {
-        double v = te * (F.charge[i] + F.neutral[i]);
-        v *= F.dx2;
-        b[i] = v;
+    double s = 0.0;
+#pragma unroll
+    for (int k = 0; k < MPER; k++) {
+        const int i = base + k;
+        if (i < N) {
+            double v = te * (F.charge[i] + F.neutral[i]);
+            v *= F.dx2;
+            P.W.b[i] = v;
+            if (i >= 1) s += v;
+        }
+    }
+    const double tot = mb_sum(s, lds);
+    if (threadIdx.x == 0) {
+        P.W.Pa[blockIdx.x] = tot;
+        if (blockIdx.x == 0) *P.W.ex0 = *F.Ex0;
+    }
+}
+
+__global__ __launch_bounds__(MT) void k_pm_scan_s(PoissonMB P) {
+    __shared__ double lds[MW];
+    const int N = P.fs.n_f, blk = blockIdx.x, base = blk * CHUNK + threadIdx.x * MPER;
+    const double tail = wave_range_sum(P.W.Pa, 0, P.nb);
+    // S at the chunk's first entry: v'_0 = -tail plus the entries >= 1 before the chunk
+    const double start = blk == 0 ? 0.0 : -tail + wave_range_sum(P.W.Pa, 0, blk);
+    double e[MPER], tot = 0.0;
+#pragma unroll
+    for (int k = 0; k < MPER; k++) {
+        const int i = base + k;
+        e[k] = (i == 0) ? -tail : (i < N ? P.in[i] : 0.0);
+        tot += e[k];
+    }
+    double run = start + mb_offset(tot, lds);
+    double ssum = 0.0;
+#pragma unroll
+    for (int k = 0; k < MPER; k++) {
+        const int i = base + k;
+        if (i < N) {
+            P.W.S[i] = run;
+            ssum += run;
+        }
+        run += e[k];
+    }
+    const double t2 = mb_sum(ssum, lds);
+    if (threadIdx.x == 0) P.W.Pb[blk] = t2;
+}
+
+__global__ __launch_bounds__(MT) void k_pm_scan_y(PoissonMB P) {
+    __shared__ double lds[MW];
+    const int N = P.fs.n_f, blk = blockIdx.x, base = blk * CHUNK + threadIdx.x * MPER;
+    const double d0 = wave_range_sum(P.W.Pb, 0, P.nb) * P.invN;
+    // sum of D over the earlier chunks: (entries - [chunk 0]) * d0 - (chunk sum of S)
+    double sd = 0.0;
+    for (int k = threadIdx.x & 63; k < blk; k += 64) {
+        const int n = min(CHUNK, N - k * CHUNK) - (k == 0 ? 1 : 0);
+        sd += (double)n * d0 - P.W.Pb[k];
+    }
+    const double offD = wave_sum(sd);
+    double D[MPER], tot = 0.0;
+#pragma unroll
+    for (int k = 0; k < MPER; k++) {
+        const int i = base + k;
+        D[k] = (i == 0 || i >= N) ? 0.0 : d0 - P.W.S[i];
+        tot += D[k];
+    }
+    double run = offD + mb_offset(tot, lds);
+#pragma unroll
+    for (int k = 0; k < MPER; k++) {
+        const int i = base + k;
+        run += D[k];
+        if (i < N) P.W.y[i] = run;
+    }
+}
+
+// MODE 0: first solve (x = c); 1: refinement (x += c); both then form the next
+// residual.  MODE 2: last refinement, writes phi (x_out), Ex0 and E.
+template <int MODE>
+__global__ __launch_bounds__(MT) void k_pm_conv(PoissonMB P) {
+    __shared__ double yw[CHUNK + 2 * CW];
+    __shared__ double ys[NSPEC + 2 * (NG - 1)];
+    __shared__ double xw[CHUNK + 2 * HALO];
+    __shared__ double cs[NSPEC];
+    __shared__ double lds[MW];
+    const vx_field_state &F = P.fs;
+    const int N = F.n_f, blk = blockIdx.x, lo = blk * CHUNK;
+    const int n_own = min(CHUNK, N - lo);
+    const double *y = P.W.y;
+    for (int j = threadIdx.x; j < n_own + 2 * CW; j += MT) yw[j] = y[wrap(lo - CW + j, N)];
+    for (int j = threadIdx.x; j < NSPEC + 2 * (NG - 1); j += MT) ys[j] = y[wrap(-3 - (NG - 1) + j, N)];
+    __syncthreads();
+    if (threadIdx.x < NSPEC) cs[threadIdx.x] = conv_at(ys, threadIdx.x + NG - 1, P.g);
+    __syncthreads();
+    const double cp0 = cs[3];  // (M^-1 y)_0
+    // row 0 (the approximate solve's last step): c_0 from c_1, c_2, c_{N-2}, c_{N-1}
+    const double c0 = P.in[0] - (((P.w1 * (cs[4] - cp0) + P.w2 * (cs[5] - cp0)) + P.w2 * (cs[1] - cp0)) +
+                                 P.w1 * (cs[2] - cp0));
+    // x on the chunk and its halo
+    for (int j = threadIdx.x; j < n_own + 2 * HALO; j += MT) {
+        const int gi = wrap(lo - HALO + j, N);
+        const double c = (gi == 0) ? c0 : conv_at(yw, j + NG - 1, P.g) - cp0;
+        const double xn = MODE == 0 ? c : P.x_in[gi] + c;
+        xw[j] = xn;
+        if (j >= HALO && j < n_own + HALO) P.x_out[gi] = xn;
     }
     __syncthreads();
-    approx_solve(P, b, x, y, tmp, lds);
-    for (int it = 0; it < 3; it++) {
-        residual(P, b, x, r);
-        approx_solve(P, r, c, y, tmp, lds);
-        for (int i = t; i < N; i += PT) x[i] = x[i] + c[i];
-        __syncthreads();
+    if (MODE < 2) {
+        // r = b - A x in double-double, rounded once (term order as the oracle's)
+        double s = 0.0;
+        for (int j = threadIdx.x; j < n_own; j += MT) {
+            const int i = lo + j, c = j + HALO;
+            double sh = P.W.b[i], sl = 0.0;
+            auto sub = [&](double a, double xv) {
+                double p = a * xv;
+                double e = fma(a, xv, -p);
+                double tt = sh - p;
+                double bv = tt - sh;
+                double err = (sh - (tt - bv)) + (-p - bv);
+                sl = sl + (err - e);
+                sh = tt;
+            };
+            if (i == 0) {
+                sub(1.0, xw[c]); sub(P.w1, xw[c + 1]); sub(P.w2, xw[c + 2]); sub(P.w2, xw[c - 2]);
+                sub(P.w1, xw[c - 1]);
+            } else {
+                const double cf[5] = {P.w2, P.w1, P.w3, P.w1, P.w2};
+                for (int d = -2; d <= 2; d++) {
+                    if (wrap(i + d, N) == 0) continue;
+                    sub(cf[d + 2], xw[c + d]);
+                }
+            }
+            const double r = sh + sl;
+            P.r_out[i] = r;
+            if (i >= 1) s += r;
+        }
+        const double tot = mb_sum(s, lds);
+        if (threadIdx.x == 0) P.W.Pa[blk] = tot;
+    } else {
+        // EMSolver.cpp:191: Ex0 += -(E(-1) + E(0)) * 0.5 with the entry Ex0, from phi
+        // at N-3..N-1, 0..2 (recomputed here with the owners' operations)
+        double ps[NSPEC];
+#pragma unroll
+        for (int k = 0; k < NSPEC; k++) {
+            const int gi = wrap(k - 3, N);
+            ps[k] = P.x_in[gi] + (k == 3 ? c0 : cs[k] - cp0);
+        }
+        const double ex0 = *P.W.ex0, fc = F.field_coef;
+        const double e_m1 = -fc * (8 * (ps[3] - ps[1]) - ps[4] + ps[0]) + ex0;
+        const double e_0 = -fc * (8 * (ps[4] - ps[2]) - ps[5] + ps[1]) + ex0;
+        const double ex0n = ex0 + -(e_m1 + e_0) * 0.5;
+        if (blk == 0 && threadIdx.x == 0) *F.Ex0 = ex0n;
+        for (int j = threadIdx.x; j < n_own; j += MT) {
+            const int c = j + HALO;
+            F.E[lo + j] = -fc * (8 * (xw[c + 1] - xw[c - 1]) - xw[c + 2] + xw[c - 2]) + ex0n;
+        }
     }
-    for (int i = t; i < N; i += PT) F.phi[i] = x[i];
-    __syncthreads();
-    // EMSolver.cpp:191: Ex0 += -(E(-1) + E(0)) * 0.5 with the previous Ex0
-    double ex0 = *F.Ex0;
-    double e_m1 = efield(F.phi, N, F.field_coef, ex0, -1);
-    double e_0 = efield(F.phi, N, F.field_coef, ex0, 0);
-    double ex0n = ex0 + -(e_m1 + e_0) * 0.5;
-    __syncthreads();
-    if (t == 0) *F.Ex0 = ex0n;
-    for (int i = t; i < N; i += PT) F.E[i] = efield(F.phi, N, F.field_coef, ex0n, i);
 }
 
 // Register/LDS variant for n_f <= 4 * PT: thread t owns the CH consecutive
@@ -555,7 +642,7 @@ int validate_fs(vx_ctx *ctx, const vx_field_state *fs) {
 
 extern "C" {
 
-size_t vx_field_work_elems(int32_t n_f) { return (size_t)6 * n_f; }
+size_t vx_field_work_elems(int32_t n_f) { return (size_t)7 * n_f + 2 * (size_t)pm_chunks(n_f) + 1; }
 
 int vx_poisson(vx_ctx *ctx, const vx_field_state *fs) {
     int rc = validate_fs(ctx, fs);
@@ -579,8 +666,35 @@ int vx_poisson(vx_ctx *ctx, const vx_field_state *fs) {
         if (ch <= 1) k_poisson_small<1><<<1, PT, 0, ctx->stream>>>(P);
         else if (ch <= 2) k_poisson_small<2><<<1, PT, 0, ctx->stream>>>(P);
         else k_poisson_small<4><<<1, PT, 0, ctx->stream>>>(P);
-    } else {
-        k_poisson<<<1, PT, 0, ctx->stream>>>(P);
+        return check_launch(ctx, "k_poisson");
+    }
+    PoissonMB M;
+    M.fs = *fs;
+    M.w1 = P.w1;
+    M.w2 = P.w2;
+    M.w3 = P.w3;
+    for (int k = 0; k < NG; k++) M.g[k] = P.g[k];
+    M.invN = P.invN;
+    M.nb = pm_chunks(fs->n_f);
+    M.W = pm_work(fs->work, fs->n_f);
+    const PmWork &W = M.W;
+    const unsigned nb = (unsigned)M.nb;
+    k_pm_rhs<<<nb, MT, 0, ctx->stream>>>(M);
+    // four approximate solves: of b (x = c), then of three residuals (x += c)
+    const double *in[4] = {W.b, W.ra, W.rb, W.ra};
+    double *r_out[4] = {W.ra, W.rb, W.ra, nullptr};
+    const double *x_in[4] = {nullptr, W.xa, W.xb, W.xa};
+    double *x_out[4] = {W.xa, W.xb, W.xa, fs->phi};
+    for (int s = 0; s < 4; s++) {
+        M.in = in[s];
+        M.r_out = r_out[s];
+        M.x_in = x_in[s];
+        M.x_out = x_out[s];
+        k_pm_scan_s<<<nb, MT, 0, ctx->stream>>>(M);
+        k_pm_scan_y<<<nb, MT, 0, ctx->stream>>>(M);
+        if (s == 0) k_pm_conv<0><<<nb, MT, 0, ctx->stream>>>(M);
+        else if (s < 3) k_pm_conv<1><<<nb, MT, 0, ctx->stream>>>(M);
+        else k_pm_conv<2><<<nb, MT, 0, ctx->stream>>>(M);
     }
     return check_launch(ctx, "k_poisson");
 }

@@ -42,26 +42,6 @@ struct MomArgs {
     const double *tab;
 };
 
-// GetInterpolantsREL(f1..f5)[k] + correction (Rectangle.cpp:139-155, 195-204)
-__device__ __forceinline__ double rel_value(const MomArgs &A, const double *g, int pitch, int i,
-                                            int j, int k, double corr) {
-    double f1 = g[pidx(pitch, i - 2, j)], f2 = g[pidx(pitch, i - 1, j)];
-    double f3 = g[pidx(pitch, i, j)], f4 = g[pidx(pitch, i + 1, j)], f5 = g[pidx(pitch, i + 2, j)];
-    f5 -= f3; f4 -= f3; f2 -= f3; f1 -= f3;
-    double a1 = kInterp[0] * f1 + kInterp[1] * f2 + kInterp[2] * f4 + kInterp[3] * f5;
-    double a2 = kInterp[4] * f1 + kInterp[5] * f2 + kInterp[6] * f4 + kInterp[7] * f5;
-    double a3 = kInterp[8] * f1 + kInterp[9] * f2 + kInterp[10] * f4 + kInterp[11] * f5;
-    return A.rel[3 * k] * a1 + A.rel[3 * k + 1] * a2 + A.rel[3 * k + 2] * a3 + f3 + corr;
-}
-
-// correction making the sub-cell values average to the cell value
-__device__ double rel_correction(const MomArgs &A, const double *g, int pitch, int i, int j) {
-    int n = A.p.rtb;
-    double te = 0.0;
-    for (int k = 0; k < n; k++) te += rel_value(A, g, pitch, i, j, k, 0.0);
-    return g[pidx(pitch, i, j)] - A.inv_rtb * te;
-}
-
 // EMFieldSolver::GetCellAverageASquared (EMSolver.hpp:57-64)
 __device__ __forceinline__ double cell_a2(const vx_fields &fl, long i) {
     i += fl.n_prepad;
@@ -131,36 +111,58 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         asq[ch] = P.q * P.q * cell_a2(A.fl, (long)(i + P.x_pos) * RTB + k);
     }
     double acc = 0.0;  // lanes < NCH: charge chain; lanes 32..32+NCH-1: current chain
-    // sub-cell value of chain ch at p-cell j (Rectangle.cpp:183-204, 258-274)
-    auto subcell = [&](int ch, int j) -> double {
-        int c = ch / RTB, k = ch % RTB;
-        int i = i0 + c;
-        if (i >= P.n_x || j >= P.n_p + 2) return 0.0;
+    // sub-cell values of the RTB chains of column c at p-cell j (Rectangle.cpp:
+    // 183-204, 258-274): GetInterpolantsREL(f1..f5)[k] (Rectangle.cpp:139-155)
+    // plus the correction that makes the sub-cell values average to the cell
+    // value (Rectangle.cpp:195-204), formed once per cell for all k
+    auto column = [&](int c, int j, double *out) {
+        const int i = i0 + c;
+        if (i >= P.n_x || j >= P.n_p + 2) {
+            for (int k = 0; k < RTB; k++) out[k] = 0.0;
+            return;
+        }
         if (RTB == 1) {
             // interpolant coefficients and correction are exactly 0 for
             // relativeToBottom == 1: the sub-cell value is the cell value
-            return g[pidx(pitch, i, j)];
+            out[0] = g[pidx(pitch, i, j)];
+            return;
         }
-        return rel_value(A, g, pitch, i, j, k, rel_correction(A, g, pitch, i, j));
+        double f1 = g[pidx(pitch, i - 2, j)], f2 = g[pidx(pitch, i - 1, j)];
+        const double f3 = g[pidx(pitch, i, j)];
+        double f4 = g[pidx(pitch, i + 1, j)], f5 = g[pidx(pitch, i + 2, j)];
+        f5 -= f3; f4 -= f3; f2 -= f3; f1 -= f3;
+        const double a1 = kInterp[0] * f1 + kInterp[1] * f2 + kInterp[2] * f4 + kInterp[3] * f5;
+        const double a2 = kInterp[4] * f1 + kInterp[5] * f2 + kInterp[6] * f4 + kInterp[7] * f5;
+        const double a3 = kInterp[8] * f1 + kInterp[9] * f2 + kInterp[10] * f4 + kInterp[11] * f5;
+        double v[RTB], te = 0.0;
+#pragma unroll
+        for (int k = 0; k < RTB; k++) {
+            v[k] = A.rel[3 * k] * a1 + A.rel[3 * k + 1] * a2 + A.rel[3 * k + 2] * a3 + f3;
+            te += v[k] + 0.0;  // the interpolant with a zero correction
+        }
+        const double corr = f3 - A.inv_rtb * te;
+#pragma unroll
+        for (int k = 0; k < RTB; k++) out[k] = v[k] + corr;
     };
     // chunk prefetch: values for p in [jb-1, jb+65) held in registers one
     // chunk ahead so the global-memory latency overlaps the ordered sums
     double pre[NCH][2];
-    for (int ch = 0; ch < NCH; ch++) {
-        pre[ch][0] = subcell(ch, -1 + lane);
-        pre[ch][1] = lane < 2 ? subcell(ch, 63 + lane) : 0.0;
-    }
+    auto prefetch = [&](int j0) {
+        for (int c = 0; c < C; c++) {
+            double v[RTB];
+            column(c, j0 + lane, v);
+            for (int k = 0; k < RTB; k++) pre[c * RTB + k][0] = v[k];
+            if (lane < 2) column(c, j0 + 64 + lane, v);
+            for (int k = 0; k < RTB; k++) pre[c * RTB + k][1] = lane < 2 ? v[k] : 0.0;
+        }
+    };
+    prefetch(-1);
     for (int jb = 0; jb < P.n_p; jb += 64) {
         for (int ch = 0; ch < NCH; ch++) {
             S[wave][ch][lane] = pre[ch][0];
             if (lane < 2) S[wave][ch][64 + lane] = pre[ch][1];
         }
-        if (jb + 64 < P.n_p) {
-            for (int ch = 0; ch < NCH; ch++) {
-                pre[ch][0] = subcell(ch, jb + 63 + lane);
-                pre[ch][1] = lane < 2 ? subcell(ch, jb + 127 + lane) : 0.0;
-            }
-        }
+        if (jb + 64 < P.n_p) prefetch(jb + 63);
         // logarithm terms, computed once per distinct A^2 among the chains
         for (int ch = 0; ch < NCH && !use_tab; ch++) {
             if (ch > 0 && asq[ch] == asq[ch - 1]) {
