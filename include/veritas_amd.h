@@ -170,6 +170,12 @@ int vx_poisson(vx_ctx *ctx, const vx_field_state *fs);
  * x = 0 for this stage's time (veritas.cpp:80-104). */
 int vx_maxwell_stage(vx_ctx *ctx, const vx_field_state *fs, int step, double dt, double by0,
                      double bz0);
+/* The same stage with the hook values read from device memory when the
+ * kernels run (drive[0] = by0, drive[1] = bz0), so that a recorded step graph
+ * takes new hook values per replay (EMSolver.cpp:492-499 evaluates the hooks
+ * at every stage). */
+int vx_maxwell_stage_dev(vx_ctx *ctx, const vx_field_state *fs, int step, double dt,
+                         const double *drive);
 /* EMFieldSolver::EnforceChargeNeutralization tail (EMSolver.cpp:624-628):
  * J = 0, neutral = -charge. */
 int vx_neutralize(vx_ctx *ctx, const vx_field_state *fs);
@@ -262,6 +268,24 @@ int vx_event_elapsed_ms(vx_ctx *ctx, int slot_a, int slot_b, float *ms);
  * vx_timing_end, which synchronises and returns the summed device time. */
 int vx_timing_begin(vx_ctx *ctx, const char *kernel);
 int vx_timing_end(vx_ctx *ctx, double *total_ms, int *launches);
+
+/* ---- step graphs (SolverManager::Advance, SolverManager.cpp:28-39, replayed
+ * as one hipGraph launch per step) ----
+ * vx_graph_begin starts recording the context stream: entry points called
+ * until vx_graph_end are recorded, not run.  Entry points that synchronise,
+ * allocate or copy to or from pageable host memory fail while recording (and
+ * the recording is then discarded by vx_graph_end, which returns the error).
+ * A graph holds the kernel arguments it was recorded with; the caller replays
+ * it only while those buffers and values still apply. */
+typedef struct vx_graph vx_graph;
+int vx_graph_begin(vx_ctx *ctx);
+int vx_graph_end(vx_ctx *ctx, vx_graph **out);
+int vx_graph_launch(vx_ctx *ctx, const vx_graph *g);
+void vx_graph_destroy(vx_graph *g);
+/* Stream-ordered upload of at most 256 bytes that does not wait for the
+ * device: the bytes are staged in a pinned ring slot (a slot is reused only
+ * after its previous copy has completed). */
+int vx_upload_small(vx_ctx *ctx, void *dst, const void *src, size_t bytes);
 
 #ifdef __cplusplus
 }

@@ -58,6 +58,8 @@ def host_lib():
         L.vxh_destroy.argtypes = [vp]
         L.vxh_calc_dt.argtypes = [vp, d, ctypes.POINTER(d)]
         L.vxh_set_fused.argtypes = [vp, i]
+        L.vxh_set_graphs.argtypes = [vp, i]
+        L.vxh_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vxh_advance.argtypes = [vp, d, i]
         L.vxh_advance_fields.argtypes = [vp, d, i]
         L.vxh_sync.argtypes = [vp]
@@ -90,13 +92,14 @@ def host_lib():
 class Run:
     """One SolverManager (SolverManager.hpp) for a scenario on one GPU."""
 
-    def __init__(self, params, device=0, fused=True):
+    def __init__(self, params, device=0, fused=True, graphs=True):
         L = host_lib()
         self.params = params
         self.h = L.vxh_create(ctypes.byref(params), device)
         if not self.h:
             raise VeritasError(L.vxh_create_error().decode())
         self.set_fused(fused)
+        self.set_graphs(graphs)
 
     def _ck(self, rc):
         if rc != 0:
@@ -112,6 +115,15 @@ class Run:
 
     def set_fused(self, on):
         self._ck(host_lib().vxh_set_fused(self.h, int(bool(on))))
+
+    def set_graphs(self, on):
+        """Replay steps as recorded hipGraphs (SolverManager::SetGraphs)."""
+        self._ck(host_lib().vxh_set_graphs(self.h, int(bool(on))))
+
+    def graph_stats(self):
+        out = (ctypes.c_int * 2)()
+        self._ck(host_lib().vxh_graph_stats(self.h, out))
+        return {"replays": out[0], "records": out[1]}
 
     def calc_dt(self, cfl=None):
         out = ctypes.c_double()

@@ -931,6 +931,7 @@ void Mesh::InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &old) {
 
 // SolverManager.cpp:48-58 (without the screen output)
 void SolverManager::reGrid() {
+    DropGraphs();  // recorded steps hold the old patch tables and programs
     for (auto &m : meshes) {
         m->updateHierarchy();
         m->SetFieldSolver(EMSolver.get());

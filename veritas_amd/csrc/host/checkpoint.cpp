@@ -97,6 +97,7 @@ void SolverManager::SaveCheckpoint(const std::string &path) {
 }
 
 void SolverManager::LoadCheckpoint(const std::string &path) {
+    DropGraphs();  // the hierarchy and plane pointers are replaced
     Reader rd;
     rd.in.open(path, std::ios::binary);
     if (!rd.in) throw Error("checkpoint: cannot open " + path);

@@ -68,6 +68,18 @@ int vxh_set_fused(void *h, int on) {
     return guarded(h, [&](Run *r) { r->sm->SetFused(on != 0); });
 }
 
+int vxh_set_graphs(void *h, int on) {
+    return guarded(h, [&](Run *r) { r->sm->SetGraphs(on != 0); });
+}
+
+// out[0] = steps replayed from recorded graphs, out[1] = graphs recorded
+int vxh_graph_stats(void *h, int *out) {
+    return guarded(h, [&](Run *r) {
+        out[0] = r->sm->graph_replays;
+        out[1] = r->sm->graph_records;
+    });
+}
+
 int vxh_advance(void *h, double dt, int nsteps) {
     return guarded(h, [&](Run *r) {
         for (int i = 0; i < nsteps; i++) r->sm->Advance(dt);

@@ -102,7 +102,10 @@ void SolverManager::FlushOutput() {
     if (writer.joinable()) writer.join();
 }
 
-SolverManager::~SolverManager() { FlushOutput(); }
+SolverManager::~SolverManager() {
+    FlushOutput();
+    DropGraphs();
+}
 
 // SolverManager.cpp:103-160
 void SolverManager::fileOutput(double t) {

@@ -413,6 +413,7 @@ EMFieldSolver::EMFieldSolver(Settings &s, const std::vector<std::shared_ptr<Mesh
     fs.a2 = alloc((size_t)N + 1);
     fs.work = alloc(vx_field_work_elems(N));
     charges_species = alloc((size_t)N * s.q.size());
+    drive_d = alloc(12);
     double *E0 = alloc(N), *a20 = alloc((size_t)N + 1);
     fl.E0 = E0;
     fl.a20 = a20;
@@ -474,6 +475,10 @@ void EMFieldSolver::SnapshotStage0Fields() {
 
 // EMSolver.cpp:194-202; the hooks are evaluated at x = 0 and the stage time
 void EMFieldSolver::RGKStep(int step, double timestep) {
+    if (device_drive) {
+        dev->check(vx_maxwell_stage_dev(dev->ctx, &fs, step, timestep, drive_d + 2 * step), "maxwell");
+        return;
+    }
     double by0 = settings.GetBY(0, settings.time);
     double bz0 = settings.GetBZ(0, settings.time);
     dev->check(vx_maxwell_stage(dev->ctx, &fs, step, timestep, by0, bz0), "maxwell");
@@ -510,8 +515,9 @@ SolverManager::SolverManager(Settings &s, int device)
     EMSolver->EnforceChargeNeutralization();
 }
 
-// SolverManager.cpp:28-39
-void SolverManager::Advance(double timeStep) {
+// SolverManager.cpp:28-39 (SolverManager::Advance, host/graph.cpp, replays
+// these launches as a graph)
+void SolverManager::StepKernels(double timeStep) {
     for (int i = 0; i < 6; i++) {
         EMSolver->AssembleRhoAndJ();
         EMSolver->UpdatePotential();
@@ -552,6 +558,7 @@ void SolverManager::StageFields(double dt, int step) {
 }
 
 void SolverManager::SetFused(bool on) {
+    DropGraphs();
     for (auto &m : meshes) m->fused = on;
 }
 

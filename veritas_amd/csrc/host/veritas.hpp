@@ -123,6 +123,17 @@ public:
     void FCTStageFused(double timestep, int step);  // one fused kernel per stage (walls only)
     void UnaliasSlots();  // give f[0] and f[1] distinct planes (before sub-step kernels)
     int slot_version = 0;  // bumped whenever the f plane pointers change
+    // the f[0..2] plane pointers and their version (saved and restored around
+    // the recording of a step graph, which moves them without running)
+    struct Slots {
+        double *f[3];
+        int version;
+    };
+    Slots GetSlots() const { return {{P.f[0], P.f[1], P.f[2]}, slot_version}; }
+    void SetSlots(const Slots &s) {
+        for (int k = 0; k < 3; k++) P.f[k] = s.f[k];
+        slot_version = s.version;
+    }
     void CalculateRhoAndJ();
     bool AllWalls() const { return up && down && left && right; }
     const vx_patch &desc() const { return P; }
@@ -214,6 +225,7 @@ private:
     void RefreshTable();
     void EnsureTable();  // refresh the device table if plane pointers moved
     int table_version = -1;
+    friend class SolverManager;  // saves and restores table_version around step graphs
 };
 
 // EMSolver.hpp:9-49
@@ -236,6 +248,10 @@ public:
     std::vector<std::vector<double>> energies;      // dNdP per species, finest p grid
     std::vector<double> Download(const double *d, size_t n) const;
     double *charges_species = nullptr;              // n_species * n_f, last assembly
+    // step graphs: RGKStep reads the hook values of stage s from drive_d[2s..2s+1]
+    // (device memory written before each replay) instead of taking them as arguments
+    bool device_drive = false;
+    double *drive_d = nullptr;
 
 private:
     Settings &settings;
@@ -273,6 +289,12 @@ public:
     void StageVlasovSub(double dt, int step, int sub);
     void StageFields(double dt, int step);
     void SetFused(bool on);
+    // Advance as a graph launch (host/graph.cpp): the first step from a given
+    // plane-pointer state and dt runs eagerly, the second records the step's
+    // launches once, later ones replay the recording.  On by default.
+    void SetGraphs(bool on);
+    int graph_replays = 0;   // steps run from a recorded graph
+    int graph_records = 0;   // graphs recorded
     std::shared_ptr<Device> dev;
     std::vector<std::shared_ptr<Mesh>> meshes;
     std::shared_ptr<EMFieldSolver> EMSolver;
@@ -281,6 +303,27 @@ public:
 private:
     std::shared_ptr<struct OutputFiles> files;
     std::thread writer;
+    void StepKernels(double timeStep);  // SolverManager.cpp:28-39, launched eagerly
+    // host state that a step moves: plane pointers, table versions, the clock
+    struct HostState {
+        std::vector<Rectangle::Slots> slots;
+        std::vector<int> table_versions;
+        double time = 0.0;
+    };
+    struct StepGraph {
+        std::vector<uintptr_t> key;
+        HostState post;  // after the step (the clock is advanced by the replay itself)
+        vx_graph *g = nullptr;
+        ~StepGraph() { vx_graph_destroy(g); }
+    };
+    std::vector<std::unique_ptr<StepGraph>> step_graphs;
+    std::vector<std::vector<uintptr_t>> graph_seen, graph_refused;
+    bool graphs_on = true;
+    std::vector<uintptr_t> GraphKey(double dt) const;
+    HostState SaveHostState() const;
+    void RestoreHostState(const HostState &h);
+    StepGraph *RecordStep(double dt, const std::vector<uintptr_t> &key);
+    void DropGraphs();
 };
 
 }  // namespace vxh

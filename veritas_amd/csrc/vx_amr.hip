@@ -382,6 +382,7 @@ extern "C" {
 
 int vx_patch_table_create(vx_ctx *ctx, const vx_patch *patches, int32_t n, vx_patch_table **out) {
     if (!out || n < 0 || (n > 0 && !patches)) return fail(ctx, VX_EINVAL, "patch table: bad arguments");
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "patch table while a step graph is recorded");
     vx_patch_table *t = new vx_patch_table;
     t->ctx = ctx;
     t->n = n;
@@ -405,6 +406,7 @@ int vx_patch_table_create(vx_ctx *ctx, const vx_patch *patches, int32_t n, vx_pa
 
 int vx_patch_table_update(vx_ctx *ctx, vx_patch_table *t, const vx_patch *patches, int32_t n) {
     if (!t || n != t->n) return fail(ctx, VX_EINVAL, "patch table: size mismatch");
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "patch table upload while a step graph is recorded");
     if (n == 0) return VX_OK;
     for (int k = 0; k < n; k++) {
         const vx_patch &a = t->host[k], &b = patches[k];
@@ -430,6 +432,7 @@ int vx_program_create(vx_ctx *ctx, const vx_cell_task *tasks, int32_t n_tasks, c
                       int32_t n_batches, vx_program **out) {
     if (!out || n_tasks < 0 || n_batches < 0 || (n_tasks > 0 && (!tasks || !batch_start)))
         return fail(ctx, VX_EINVAL, "program: bad arguments");
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "program upload while a step graph is recorded");
     if (n_tasks > 0 && (batch_start[0] != 0 || batch_start[n_batches] != n_tasks))
         return fail(ctx, VX_EINVAL, "program: batches must cover [0, %d)", n_tasks);
     for (int b = 0; b < n_batches; b++)

@@ -533,6 +533,7 @@ struct MaxArgs {
     vx_field_state fs;
     int step;
     double by0, bz0;
+    const double *drive;  // when set, by0 = drive[0] and bz0 = drive[1] in device memory
     double dx_inv;
     double a[6], b[6];
 };
@@ -554,8 +555,9 @@ __global__ void k_maxwell_rhs(MaxArgs A) {
         int ip1 = fidx(ne, i + 1, 1);
         rBy = dx_inv * (Ez[ip1] - Ez[i1]);
         rBz = -dx_inv * (Ey[ip1] - Ey[i1]);
-        rEy = -dx_inv * (Bz[i1] - A.bz0) * vx_eps0_inv * vx_mu_inv;
-        rEz = dx_inv * (By[i1] - A.by0) * vx_eps0_inv * vx_mu_inv;
+        const double by0 = A.drive ? A.drive[0] : A.by0, bz0 = A.drive ? A.drive[1] : A.bz0;
+        rEy = -dx_inv * (Bz[i1] - bz0) * vx_eps0_inv * vx_mu_inv;
+        rEz = dx_inv * (By[i1] - by0) * vx_eps0_inv * vx_mu_inv;
         rAy = -Ey[i1];
         rAz = -Ez[i1];
     } else if (i < npre || (i >= npre + N && i < ne - 2)) {
@@ -699,8 +701,10 @@ int vx_poisson(vx_ctx *ctx, const vx_field_state *fs) {
     return check_launch(ctx, "k_poisson");
 }
 
-int vx_maxwell_stage(vx_ctx *ctx, const vx_field_state *fs, int step, double dt, double by0,
-                     double bz0) {
+}  // extern "C"
+
+static int maxwell_stage(vx_ctx *ctx, const vx_field_state *fs, int step, double dt, double by0,
+                         double bz0, const double *drive) {
     int rc = validate_fs(ctx, fs);
     if (rc) return rc;
     if (step < 0 || step > 5) return fail(ctx, VX_EINVAL, "maxwell: bad step %d", step);
@@ -709,6 +713,7 @@ int vx_maxwell_stage(vx_ctx *ctx, const vx_field_state *fs, int step, double dt,
     A.step = step;
     A.by0 = by0;
     A.bz0 = bz0;
+    A.drive = drive;
     A.dx_inv = 1 / fs->dx_f;
     for (int k = 0; k < 6; k++) A.a[k] = A.b[k] = 0.0;
     if (step <= 4) {
@@ -726,6 +731,19 @@ int vx_maxwell_stage(vx_ctx *ctx, const vx_field_state *fs, int step, double dt,
     if ((rc = check_launch(ctx, "k_maxwell_update"))) return rc;
     k_faces<<<(unsigned)((fs->n_f + 255) / 256), 256, 0, ctx->stream>>>(*fs);
     return check_launch(ctx, "k_faces");
+}
+
+extern "C" {
+
+int vx_maxwell_stage(vx_ctx *ctx, const vx_field_state *fs, int step, double dt, double by0,
+                     double bz0) {
+    return maxwell_stage(ctx, fs, step, dt, by0, bz0, nullptr);
+}
+
+int vx_maxwell_stage_dev(vx_ctx *ctx, const vx_field_state *fs, int step, double dt,
+                         const double *drive) {
+    if (!drive) return fail(ctx, VX_EINVAL, "maxwell: null drive");
+    return maxwell_stage(ctx, fs, step, dt, 0.0, 0.0, drive);
 }
 
 int vx_neutralize(vx_ctx *ctx, const vx_field_state *fs) {

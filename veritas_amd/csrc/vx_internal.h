@@ -22,6 +22,18 @@ struct vx_ctx {
     // p-moment logarithm table of the patch being reduced (vx_moments_patch)
     double *mom_table = nullptr;
     size_t mom_table_n = 0;
+    // step-graph recording (vx_graph_begin / vx_graph_end)
+    bool capturing = false;
+    // pinned staging ring of vx_upload_small
+    static constexpr int RING = 64, RING_BYTES = 256;
+    char *ring = nullptr;
+    hipEvent_t ring_ev[RING] = {};
+    bool ring_pending[RING] = {};
+    int ring_next = 0;
+};
+
+struct vx_graph {
+    hipGraphExec_t exec = nullptr;
 };
 
 struct vx_moment_plan {
@@ -56,7 +68,7 @@ struct LaunchTimer {
     vx_ctx *ctx;
     hipEvent_t stop = nullptr;
     LaunchTimer(vx_ctx *c, const char *name) : ctx(c) {
-        if (ctx->timed_kernel.empty() || ctx->timed_kernel != name) return;
+        if (ctx->capturing || ctx->timed_kernel.empty() || ctx->timed_kernel != name) return;
         size_t k = 2 * ctx->tcount;
         if (k + 2 > ctx->tev.size()) {
             for (int e = 0; e < 64; e++) {

@@ -345,6 +345,7 @@ int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
     const size_t need = (size_t)p->n_p + 3;
 
     if (ctx->mom_table_n < need) {
+        if (ctx->capturing) return fail(ctx, VX_EINVAL, "moments: table growth while a step graph is recorded");
         if (ctx->mom_table) (void)hipFree(ctx->mom_table);
         ctx->mom_table = nullptr;
         ctx->mom_table_n = 0;
@@ -376,6 +377,7 @@ int vx_moment_plan_create(vx_ctx *ctx, const vx_patch *patches, const vx_moment_
     *out = nullptr;
     if (!lay || lay->n_species < 1 || lay->n_levels < 1 || n_f < 1)
         return fail(ctx, VX_EINVAL, "bad moment layout");
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "moment plan while a step graph is recorded");
     int ngroups = lay->n_species * lay->n_levels;
     int npatch = lay->level_start[ngroups];
     std::vector<MomentSrc> src(npatch > 0 ? npatch : 1);

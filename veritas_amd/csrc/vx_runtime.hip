@@ -1,4 +1,6 @@
 // Context, memory and timing entry points of the C ABI.
+#include <cstring>
+
 #include "vx_internal.h"
 
 using namespace vxi;
@@ -33,19 +35,31 @@ void vx_ctx_destroy(vx_ctx *ctx) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->tev) (void)hipEventDestroy(e);
     if (ctx->mom_table) (void)hipFree(ctx->mom_table);
+    for (auto &e : ctx->ring_ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->ring) (void)hipHostFree(ctx->ring);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
 const char *vx_last_error(vx_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
-int vx_sync(vx_ctx *ctx) { return check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync"); }
+// entry points that wait for the device or allocate cannot be recorded in a step graph
+static int not_recording(vx_ctx *ctx, const char *what) {
+    return ctx->capturing ? fail(ctx, VX_EINVAL, "%s while a step graph is being recorded", what) : VX_OK;
+}
+
+int vx_sync(vx_ctx *ctx) {
+    if (int rc = not_recording(ctx, "sync")) return rc;
+    return check_hip(ctx, hipStreamSynchronize(ctx->stream), "sync");
+}
 
 void *vx_stream(vx_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 int vx_malloc(vx_ctx *ctx, size_t bytes, void **dptr) {
     if (!dptr) return fail(ctx, VX_EINVAL, "null out pointer");
     *dptr = nullptr;
+    if (int rc = not_recording(ctx, "vx_malloc")) return rc;
     if (bytes == 0) return VX_OK;
     hipError_t e = hipMalloc(dptr, bytes);
     if (e != hipSuccess) return fail(ctx, VX_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
@@ -56,16 +70,19 @@ int vx_malloc(vx_ctx *ctx, size_t bytes, void **dptr) {
 
 int vx_free(vx_ctx *ctx, void *dptr) {
     if (!dptr) return VX_OK;
+    if (int rc = not_recording(ctx, "vx_free")) return rc;
     (void)hipStreamSynchronize(ctx->stream);
     return check_hip(ctx, hipFree(dptr), "hipFree");
 }
 
 int vx_memcpy_h2d(vx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (int rc = not_recording(ctx, "vx_memcpy_h2d")) return rc;
     int rc = check_hip(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream), "h2d");
     return rc ? rc : vx_sync(ctx);
 }
 
 int vx_memcpy_d2h(vx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (int rc = not_recording(ctx, "vx_memcpy_d2h")) return rc;
     int rc = check_hip(ctx, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream), "d2h");
     return rc ? rc : vx_sync(ctx);
 }
@@ -127,6 +144,78 @@ int32_t vx_plane_pitch(int32_t n_p) { return ((n_p + 4 + 7) / 8) * 8; }
 
 size_t vx_plane_elems(int32_t n_x, int32_t n_p) {
     return (size_t)(n_x + 4) * (size_t)vx_plane_pitch(n_p);
+}
+
+
+int vx_graph_begin(vx_ctx *ctx) {
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "a step graph is already being recorded");
+    int rc = check_hip(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal),
+                       "begin capture");
+    if (rc == VX_OK) ctx->capturing = true;
+    return rc;
+}
+
+int vx_graph_end(vx_ctx *ctx, vx_graph **out) {
+    if (out) *out = nullptr;
+    if (!ctx->capturing) return fail(ctx, VX_EINVAL, "no step graph is being recorded");
+    ctx->capturing = false;
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (e != hipSuccess || !graph) {
+        if (graph) (void)hipGraphDestroy(graph);
+        (void)hipGetLastError();
+        return fail(ctx, VX_EHIP, "end capture: %s", hipGetErrorString(e));
+    }
+    if (!out) {  // discard
+        (void)hipGraphDestroy(graph);
+        return VX_OK;
+    }
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) return fail(ctx, VX_EHIP, "graph instantiate: %s", hipGetErrorString(e));
+    vx_graph *g = new vx_graph;
+    g->exec = exec;
+    *out = g;
+    return VX_OK;
+}
+
+int vx_graph_launch(vx_ctx *ctx, const vx_graph *g) {
+    if (!g || !g->exec) return fail(ctx, VX_EINVAL, "null graph");
+    if (int rc = not_recording(ctx, "vx_graph_launch")) return rc;
+    return check_hip(ctx, hipGraphLaunch(g->exec, ctx->stream), "graph launch");
+}
+
+void vx_graph_destroy(vx_graph *g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    delete g;
+}
+
+int vx_upload_small(vx_ctx *ctx, void *dst, const void *src, size_t bytes) {
+    if (bytes > (size_t)vx_ctx::RING_BYTES) return fail(ctx, VX_EINVAL, "upload of %zu bytes", bytes);
+    if (int rc = not_recording(ctx, "vx_upload_small")) return rc;
+    if (!ctx->ring) {
+        int rc = check_hip(ctx, hipHostMalloc((void **)&ctx->ring, (size_t)vx_ctx::RING * vx_ctx::RING_BYTES),
+                           "pinned ring");
+        if (rc) return rc;
+        for (auto &ev : ctx->ring_ev)
+            if ((rc = check_hip(ctx, hipEventCreateWithFlags(&ev, hipEventDisableTiming), "ring event")))
+                return rc;
+    }
+    const int k = ctx->ring_next;
+    ctx->ring_next = (k + 1) % vx_ctx::RING;
+    if (ctx->ring_pending[k]) {
+        int rc = check_hip(ctx, hipEventSynchronize(ctx->ring_ev[k]), "ring slot wait");
+        if (rc) return rc;
+    }
+    char *slot = ctx->ring + (size_t)k * vx_ctx::RING_BYTES;
+    std::memcpy(slot, src, bytes);
+    int rc = check_hip(ctx, hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, ctx->stream), "upload");
+    if (rc) return rc;
+    rc = check_hip(ctx, hipEventRecord(ctx->ring_ev[k], ctx->stream), "ring event record");
+    ctx->ring_pending[k] = rc == VX_OK;
+    return rc;
 }
 
 }  // extern "C"
