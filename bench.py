@@ -208,7 +208,8 @@ def main():
                                f"phase-space cells, FP64, dt=0.4xCFL",
                    "nx": params.nx, "np": params.np, "mode": a.mode,
                    "hbm_frac_216B": value * BYTES_PER_UPDATE / HBM_PEAK,
-                   "wall_s": wall, "graph_ms_per_step": graph_ms},
+                   "wall_s": wall, "graph_ms_per_step": graph_ms,
+                   "tile_rows": run.tile_rows() if a.mode == "fused" else None},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(kernel, params.nx, params.np),

@@ -110,6 +110,10 @@ int vx_fct_stage_walls(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
  * to check that the library provides it. */
 int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
                        const double *fin, double *fout);
+/* Rows per strip that vx_fct_stage_fused uses for patches of n_x x n_p cells
+ * (timed on the first call for that shape; 0 before it).  No counterpart in
+ * the reference: it reports the tile shape the library chose. */
+int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p);
 int vx_fct_stage_fused_available(void);
 
 /* Ghost fill of slot `slot` of a patch whose neighbours are all the domain-edge

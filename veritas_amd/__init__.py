@@ -60,6 +60,7 @@ def host_lib():
         L.vxh_set_fused.argtypes = [vp, i]
         L.vxh_set_graphs.argtypes = [vp, i]
         L.vxh_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.vxh_tile_rows.argtypes = [vp, i, i]
         L.vxh_advance.argtypes = [vp, d, i]
         L.vxh_advance_fields.argtypes = [vp, d, i]
         L.vxh_sync.argtypes = [vp]
@@ -119,6 +120,13 @@ class Run:
     def set_graphs(self, on):
         """Replay steps as recorded hipGraphs (SolverManager::SetGraphs)."""
         self._ck(host_lib().vxh_set_graphs(self.h, int(bool(on))))
+
+    def tile_rows(self, n_x=None, n_p=None):
+        """Rows per strip the fused stage uses for an n_x x n_p patch (0 until
+        its first launch times the candidates)."""
+        n_x = self.params.nx if n_x is None else n_x
+        n_p = self.params.np if n_p is None else n_p
+        return host_lib().vxh_tile_rows(self.h, n_x, n_p)
 
     def graph_stats(self):
         out = (ctypes.c_int * 2)()

@@ -80,6 +80,11 @@ int vxh_graph_stats(void *h, int *out) {
     });
 }
 
+// strip height chosen for the fused stage of an n_x x n_p patch (0: not yet chosen)
+int vxh_tile_rows(void *h, int n_x, int n_p) {
+    return vx_stage_tile_rows(((Run *)h)->sm->dev->ctx, n_x, n_p);
+}
+
 int vxh_advance(void *h, double dt, int nsteps) {
     return guarded(h, [&](Run *r) {
         for (int i = 0; i < nsteps; i++) r->sm->Advance(dt);

@@ -321,6 +321,67 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     for (int r = r0; r <= i1 + HALO - 1; r++) row(r);
 }
 
+void launch_stage(vx_ctx *ctx, StageArgs &A) {
+    A.nstrip = (A.p.n_x + A.TI - 1) / A.TI;
+    dim3 grid((unsigned)(A.nbx * ((A.nstrip + 3) / 4)));
+    switch (A.step) {
+        case 0: k_stage<0><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 1: k_stage<1><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 2: k_stage<2><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 3: k_stage<3><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 4: k_stage<4><<<grid, 256, 0, ctx->stream>>>(A); break;
+        default: k_stage<5><<<grid, 256, 0, ctx->stream>>>(A); break;
+    }
+}
+
+// Rows per strip (the tile height) of a patch shape.  A strip costs its rows
+// plus 6 pipeline warm-up rows, and a wavefront walks them one after another,
+// so small patches want short strips (more wavefronts) and large ones long
+// strips (less warm-up).  The first call for a shape times the candidates on
+// the call's own launch, which is idempotent (it reads f_s, f^n and the fluxes
+// of earlier stages and writes only f_(s+1) and this stage's fluxes, the same
+// values for every strip height), and keeps the fastest.  While a step graph
+// is being recorded the choice falls back to the shortest strip that still
+// gives 2048 wavefronts.
+int tile_rows(vx_ctx *ctx, const StageArgs &A0) {
+    const int nx = A0.p.n_x, np = A0.p.n_p;
+    for (auto &c : ctx->ti_cache)
+        if (c[0] == nx && c[1] == np) return c[2];
+    static const int cand[] = {64, 32, 16, 8, 4, 2};
+    int guess = 2;
+    for (int ti : cand)
+        if ((long)A0.nbx * ((nx + ti - 1) / ti) >= 2048) {
+            guess = ti;
+            break;
+        }
+    if (ctx->capturing) return guess;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int best = guess;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        float best_ms = 1e30f;
+        for (int ti : cand) {
+            if (ti > 2 && ti >= 2 * nx) continue;  // a single strip either way
+            StageArgs A = A0;
+            A.TI = ti;
+            launch_stage(ctx, A);  // warm
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) break;
+            launch_stage(ctx, A);
+            launch_stage(ctx, A);
+            if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = ti;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    ctx->ti_cache.push_back({nx, np, best});
+    return best;
+}
+
 }  // namespace
 
 extern "C" {
@@ -355,20 +416,17 @@ int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     for (int k = 0; k < A.ns; k++) A.a[k] = vx::rk_a[step][k] * dt;
     A.aSum = A.a[0];
     for (int k = 1; k < A.ns; k++) A.aSum = A.aSum + A.a[k];
-    A.TI = p->n_x >= 512 ? 64 : (p->n_x >= 128 ? 32 : 16);
-    A.nstrip = (p->n_x + A.TI - 1) / A.TI;
     A.nbx = (p->n_p + OUTW - 1) / OUTW;
-    dim3 grid((unsigned)(A.nbx * ((A.nstrip + 3) / 4)));
+    A.TI = tile_rows(ctx, A);
     LaunchTimer lt(ctx, "k_stage");
-    switch (step) {
-        case 0: k_stage<0><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 1: k_stage<1><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 2: k_stage<2><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 3: k_stage<3><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 4: k_stage<4><<<grid, 256, 0, ctx->stream>>>(A); break;
-        default: k_stage<5><<<grid, 256, 0, ctx->stream>>>(A); break;
-    }
+    launch_stage(ctx, A);
     return check_launch(ctx, "k_stage");
+}
+
+int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p) {
+    for (auto &c : ctx->ti_cache)
+        if (c[0] == n_x && c[1] == n_p) return c[2];
+    return 0;
 }
 
 }  // extern "C"
