@@ -72,6 +72,35 @@ def cpu_baseline(params, steps):
                       f"(FP64 CPU restatement, OpenMP, {el:.1f} s; initialisation excluded)"}
 
 
+def mixed_error(va, params, steps):
+    """Relative L2 error of the mixed-precision form against the FP64 path
+    (bit-identical to the CPU oracle) at 25/50/100 % of `steps`, both taking
+    dt = 0.4 x CFL of the FP64 run every step."""
+    import numpy as np
+    g, r = va.Run(params, mixed=True), va.Run(params)
+    n0 = r.get("f1").sum()
+
+    def rel(x, y):
+        n = np.linalg.norm(y)
+        return float(np.linalg.norm(x - y) / n) if n > 0 else float(np.linalg.norm(x))
+
+    out, step = {}, 0
+    for m in (steps // 4, steps // 2, steps):
+        while step < m:
+            dt = r.calc_dt(0.4)
+            g.advance(dt)
+            r.advance(dt)
+            step += 1
+        fg = g.get("f1")
+        out[str(m)] = {"f": rel(fg, r.get("f1")), "Ex": rel(g.get("E"), r.get("E")),
+                       "Ey": rel(g.get("Ey"), r.get("Ey")), "Bz": rel(g.get("Bz"), r.get("Bz")),
+                       "particle_number": abs(fg.sum() / n0 - 1)}
+    g.close()
+    r.close()
+    return {"steps": steps, "reference": "FP64 GPU path (bit-identical to the CPU oracle)",
+            "dt": "0.4 x CFL of the FP64 run, every step", "at_step": out}
+
+
 def graph_step_ms(run, dt, steps):
     """ms per step with the steps replayed as recorded graphs (the default
     mode of SolverManager::Advance); the kernel-timed run above launches
@@ -155,6 +184,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", choices=["fused", "plain"], default="fused")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: FP32 WENO weights and Zalesak ratios, compensated FP64 moments "
+                         "(reports its error against the FP64 path at 25/50/100%% of --error-steps)")
+    ap.add_argument("--error-steps", type=int, default=500)
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--np", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -171,7 +204,8 @@ def main():
     if a.workload == "laser_amr":
         return bench_amr(a, va)
     params = va.two_stream(nx=a.nx, np=a.np)
-    run = va.Run(params, fused=(a.mode == "fused"), graphs=False)  # eager: per-launch events
+    mixed = a.precision == "mixed"
+    run = va.Run(params, fused=(a.mode == "fused"), graphs=False, mixed=mixed)  # eager: per-launch events
     dt = run.calc_dt(0.4)
     kernel = "k_stage" if (a.mode == "fused" and va.has_fused_stage()) else "k_sub0"
 
@@ -202,7 +236,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32" if mixed else "f64",
         "data": "synthetic (seeded two-stream initial condition)",
         "config": {"workload": f"relativistic two-stream, single level, {params.nx}x{params.np} "
                                f"phase-space cells, FP64, dt=0.4xCFL",
@@ -218,6 +252,9 @@ def main():
                      "launches": nlaunch, "kernel_share": kms / ms if ms else None},
     }
     run.close()
+    if mixed:
+        out["config"]["precision"] = "FP32 WENO weights and Zalesak ratios; FP64 fluxes, updates, compensated moments, fields"
+        out["error_vs_fp64"] = mixed_error(va, params, a.error_steps)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(params, a.cpu_steps)
     else:

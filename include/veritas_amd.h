@@ -110,6 +110,15 @@ int vx_fct_stage_walls(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
  * to check that the library provides it. */
 int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
                        const double *fin, double *fout);
+/* Mixed-precision form of vx_fct_stage_fused (no counterpart in the
+ * reference): the WENO weights of both face reconstructions and the Zalesak
+ * ratios are evaluated in FP32 on f scaled by the power of two nearest below
+ * f_scale (pass the patch's max |f|); candidate face values,
+ * fluxes, the RK combination and every update of f stay FP64.  Not
+ * bit-identical to the FP64 path: its error against the FP64 oracle is
+ * measured by the tests. */
+int vx_fct_stage_fused_mp(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
+                          const double *fin, double *fout, double f_scale);
 /* Rows per strip that vx_fct_stage_fused uses for patches of n_x x n_p cells
  * (timed on the first call for that shape; 0 before it).  No counterpart in
  * the reference: it reports the tile shape the library chose. */
@@ -125,6 +134,12 @@ int vx_ghost_zero(vx_ctx *ctx, const vx_patch *p, int slot);
 /* p-moments of one patch (Rectangle::CalculateRhoAndJ, Rectangle.cpp:157-282):
  * chargeR and currentR, summed over p in the reference's order. */
 int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl);
+/* The same moments with each p sum compensated: every lane accumulates its
+ * p-cells with error-free additions (TwoSum), then one butterfly of TwoSums
+ * combines the lanes' sums and errors (the
+ * mixed-precision form keeps the moment reductions FP64 and compensated; the
+ * reference's sequential order is not kept; no counterpart in the reference). */
+int vx_moments_patch_compensated(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl);
 /* Rectangle::CalculateEnergy (Rectangle.cpp:284-305): energyR[j * rtb + k]
  * (n_p * rtb doubles, device) = dx * sum over non-nested cells (i, j) of the
  * p-sub-cell value k, summed in increasing i (the reference's loop races on

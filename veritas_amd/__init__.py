@@ -59,6 +59,7 @@ def host_lib():
         L.vxh_calc_dt.argtypes = [vp, d, ctypes.POINTER(d)]
         L.vxh_set_fused.argtypes = [vp, i]
         L.vxh_set_graphs.argtypes = [vp, i]
+        L.vxh_set_mixed.argtypes = [vp, i]
         L.vxh_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vxh_tile_rows.argtypes = [vp, i, i]
         L.vxh_advance.argtypes = [vp, d, i]
@@ -93,7 +94,7 @@ def host_lib():
 class Run:
     """One SolverManager (SolverManager.hpp) for a scenario on one GPU."""
 
-    def __init__(self, params, device=0, fused=True, graphs=True):
+    def __init__(self, params, device=0, fused=True, graphs=True, mixed=False):
         L = host_lib()
         self.params = params
         self.h = L.vxh_create(ctypes.byref(params), device)
@@ -101,6 +102,8 @@ class Run:
             raise VeritasError(L.vxh_create_error().decode())
         self.set_fused(fused)
         self.set_graphs(graphs)
+        if mixed:
+            self.set_mixed(True)
 
     def _ck(self, rc):
         if rc != 0:
@@ -116,6 +119,12 @@ class Run:
 
     def set_fused(self, on):
         self._ck(host_lib().vxh_set_fused(self.h, int(bool(on))))
+
+    def set_mixed(self, on):
+        """Mixed-precision form (SolverManager::SetMixed): FP32 WENO weights and
+        Zalesak ratios in the fused stage of a single walled patch, compensated
+        FP64 moment sums; not bit-identical to the FP64 path."""
+        self._ck(host_lib().vxh_set_mixed(self.h, int(bool(on))))
 
     def set_graphs(self, on):
         """Replay steps as recorded hipGraphs (SolverManager::SetGraphs)."""

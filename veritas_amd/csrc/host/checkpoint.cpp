@@ -49,6 +49,7 @@ struct Reader {
 void Rectangle::UploadSlot(int slot, const std::vector<double> &full) {
     UnaliasSlots();
     if (full.size() != vx_plane_elems(n_x, n_p)) throw Error("checkpoint: plane size mismatch");
+    for (double v : full) fScale = std::max(fScale, std::fabs(v));
     dev->check(vx_memcpy_h2d(dev->ctx, P.f[slot], full.data(), sizeof(double) * full.size()), "upload slot");
 }
 

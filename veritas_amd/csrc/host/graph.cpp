@@ -84,9 +84,11 @@ SolverManager::StepGraph *SolverManager::RecordStep(double dt, const std::vector
     settings.time = pre.time;
     if (!ok || rc != VX_OK || !g) {
         vx_graph_destroy(g);
+        if (graph_refused.size() >= 16) graph_refused.erase(graph_refused.begin());
         graph_refused.push_back(key);
         return nullptr;
     }
+    if (step_graphs.size() >= 8) step_graphs.erase(step_graphs.begin());
     auto sg = std::make_unique<StepGraph>();
     sg->key = key;
     sg->post = post;
@@ -110,7 +112,13 @@ void SolverManager::Advance(double timeStep) {
         for (auto &k : graph_seen) seen = seen || k == key;
         for (auto &k : graph_refused) refused = refused || k == key;
         if (!seen || refused) {
-            if (!seen) graph_seen.push_back(key);
+            if (!seen) {
+                // a caller that changes dt every step (the reference's driver
+                // takes min(CalculateDt, dt_max), veritas.cpp:131-137) never
+                // returns to a state; keep only the latest keys
+                if (graph_seen.size() >= 16) graph_seen.erase(graph_seen.begin());
+                graph_seen.push_back(key);
+            }
             StepKernels(timeStep);
             return;
         }

@@ -68,6 +68,10 @@ int vxh_set_fused(void *h, int on) {
     return guarded(h, [&](Run *r) { r->sm->SetFused(on != 0); });
 }
 
+int vxh_set_mixed(void *h, int on) {
+    return guarded(h, [&](Run *r) { r->sm->SetMixed(on != 0); });
+}
+
 int vxh_set_graphs(void *h, int on) {
     return guarded(h, [&](Run *r) { r->sm->SetGraphs(on != 0); });
 }

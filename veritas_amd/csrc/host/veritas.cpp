@@ -222,6 +222,7 @@ void Rectangle::InitializeDistribution() {
         }
     }
     size_t bytes = sizeof(double) * h.size();
+    for (double v : h) fScale = std::max(fScale, std::fabs(v));
     dev->check(vx_memcpy_h2d(dev->ctx, P.f[0], h.data(), bytes), "upload f0");
     dev->check(vx_memcpy_h2d(dev->ctx, P.f[1], h.data(), bytes), "upload f1");
 }
@@ -246,15 +247,20 @@ void Rectangle::FCTStageWalls(double timestep, int step) {
 // f[1] = f_s is read, f_{s+1} goes to the plane that holds neither f^n nor
 // f_s; at stage 5 f^n and f_s become the same plane (f[0] <- f[1],
 // Rectangle.cpp:1614-1622, without a copy).
-void Rectangle::FCTStageFused(double timestep, int step) {
+void Rectangle::FCTStageFused(double timestep, int step, bool mixed) {
     double *out = nullptr;
     for (double *b : buf)
         if (b != P.f[0] && b != P.f[1]) {
             out = b;
             break;
         }
-    dev->check(vx_fct_stage_fused(dev->ctx, &P, step, timestep, &EMSolver->fields(), P.f[1], out),
-               "fct stage fused");
+    if (mixed)
+        dev->check(vx_fct_stage_fused_mp(dev->ctx, &P, step, timestep, &EMSolver->fields(), P.f[1], out,
+                                         fScale > 0.0 ? fScale : settings.GetfMax(particleType)),
+                   "fct stage fused (mixed)");
+    else
+        dev->check(vx_fct_stage_fused(dev->ctx, &P, step, timestep, &EMSolver->fields(), P.f[1], out),
+                   "fct stage fused");
     P.f[1] = out;
     if (step == 5) P.f[0] = out;
     for (double *b : buf)
@@ -276,7 +282,10 @@ void Rectangle::UnaliasSlots() {
 }
 
 void Rectangle::CalculateRhoAndJ() {
-    dev->check(vx_moments_patch(dev->ctx, &P, &EMSolver->fields()), "moments");
+    if (EMSolver->compensated_moments)
+        dev->check(vx_moments_patch_compensated(dev->ctx, &P, &EMSolver->fields()), "moments");
+    else
+        dev->check(vx_moments_patch(dev->ctx, &P, &EMSolver->fields()), "moments");
 }
 
 std::vector<double> Rectangle::Download(const double *plane, int g) const {
@@ -352,7 +361,7 @@ void Mesh::EnsureTable() {
 void Mesh::Advance(double timeStep, int step) {
     Rectangle *single = SinglePatch() ? levels.back()->rectangles[0].get() : nullptr;
     if (fused && single && single->AllWalls()) {
-        single->FCTStageFused(timeStep, step);
+        single->FCTStageFused(timeStep, step, mixed);
         return;
     }
     Sub0(timeStep, step);
@@ -560,6 +569,12 @@ void SolverManager::StageFields(double dt, int step) {
 void SolverManager::SetFused(bool on) {
     DropGraphs();
     for (auto &m : meshes) m->fused = on;
+}
+
+void SolverManager::SetMixed(bool on) {
+    DropGraphs();
+    for (auto &m : meshes) m->mixed = on;
+    EMSolver->compensated_moments = on;
 }
 
 }  // namespace vxh

@@ -120,7 +120,7 @@ public:
     void InitializeDistribution();
     void FCTTimeStep(double timestep, int step, int subStep);
     void FCTStageWalls(double timestep, int step);  // sub-step kernels, whole stage (walls only)
-    void FCTStageFused(double timestep, int step);  // one fused kernel per stage (walls only)
+    void FCTStageFused(double timestep, int step, bool mixed = false);  // one fused kernel per stage (walls only)
     void UnaliasSlots();  // give f[0] and f[1] distinct planes (before sub-step kernels)
     int slot_version = 0;  // bumped whenever the f plane pointers change
     // the f[0..2] plane pointers and their version (saved and restored around
@@ -143,6 +143,7 @@ public:
     void UploadSlot(int slot, const std::vector<double> &full);  // whole plane incl. ghosts
     std::vector<double> CalculateEnergy();  // Rectangle.cpp:284-305 on the device
     double *energyR_d = nullptr;
+    double fScale = 0.0;  // max |f| of the host uploads (scale of the mixed-precision FP32 copies)
 
 private:
     Settings &settings;
@@ -210,6 +211,7 @@ public:
     std::vector<std::pair<std::string, std::string>> rectangleData(double tidx) const;
     void Sub0(double timeStep, int step);  // sub-step 0 of every patch with the coarse-fine fluxes
     bool fused = true;  // single walled patch: one fused call per stage
+    bool mixed = false; // the fused stage in mixed precision (FP32 weights and ratios)
 
 private:
     Settings &settings;
@@ -252,6 +254,7 @@ public:
     // (device memory written before each replay) instead of taking them as arguments
     bool device_drive = false;
     double *drive_d = nullptr;
+    bool compensated_moments = false;  // mixed-precision form: Neumaier p sums
 
 private:
     Settings &settings;
@@ -289,6 +292,9 @@ public:
     void StageVlasovSub(double dt, int step, int sub);
     void StageFields(double dt, int step);
     void SetFused(bool on);
+    // mixed-precision form (single walled patch): FP32 WENO weights and Zalesak
+    // ratios in the fused stage, compensated FP64 moment sums; FP64 otherwise
+    void SetMixed(bool on);
     // Advance as a graph launch (host/graph.cpp): the first step from a given
     // plane-pointer state and dt runs eagerly, the second records the step's
     // launches once, later ones replay the recording.  On by default.

@@ -23,8 +23,8 @@ struct vx_ctx {
     // p-moment logarithm table of the patch being reduced (vx_moments_patch)
     double *mom_table = nullptr;
     size_t mom_table_n = 0;
-    // strip heights of the fused stage chosen per patch shape: {n_x, n_p, rows}
-    std::vector<std::array<int, 3>> ti_cache;
+    // strip heights of the fused stage chosen per patch shape: {n_x, n_p, rows, mixed}
+    std::vector<std::array<int, 4>> ti_cache;
     // step-graph recording (vx_graph_begin / vx_graph_end)
     bool capturing = false;
     // pinned staging ring of vx_upload_small

@@ -81,6 +81,30 @@ __device__ __forceinline__ void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Compensated wavefront sum of the lanes' (value, error) pairs (mixed-precision
+// form, where the reference's p order is not kept): a butterfly of error-free
+// additions (TwoSum); every lane ends with the same sum s and error e.
+__device__ __forceinline__ void wave_two_sum(double v, double ve, double &s, double &e) {
+    s = v;
+    e = ve;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const double so = __shfl_xor(s, off, 64), eo = __shfl_xor(e, off, 64);
+        const double t = s + so, z = t - s;
+        const double err = (s - (t - z)) + (so - z);
+        e = (e + eo) + err;
+        s = t;
+    }
+}
+
+// (acc, cmp) += v with an error-free addition
+__device__ __forceinline__ void add_two(double &acc, double &cmp, double v) {
+    const double t = acc + v, z = t - acc;
+    const double err = (acc - (t - z)) + (v - z);
+    cmp = cmp + err;
+    acc = t;
+}
+
 // A wavefront owns C x columns (NCH = C * RTB finest sub-columns, the
 // reference's accumulation chains) and walks p in chunks of 64.  Per chunk the
 // 64 lanes (one per p-cell) evaluate the summands of every chain in parallel
@@ -88,7 +112,7 @@ __device__ __forceinline__ void wave_sync() {
 // chain ch in p order and lane 32 + ch the current summands.  Skipped cells
 // (nested, or past n_p) contribute -0.0, which is an exact identity for
 // addition, so every chain performs the reference's additions in its order.
-template <int RTB, int C>
+template <int RTB, int C, bool COMP>
 __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     constexpr int NCH = C * RTB;
     static_assert(NCH <= 32, "chains must fit in half a wavefront");
@@ -111,6 +135,10 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         asq[ch] = P.q * P.q * cell_a2(A.fl, (long)(i + P.x_pos) * RTB + k);
     }
     double acc = 0.0;  // lanes < NCH: charge chain; lanes 32..32+NCH-1: current chain
+    // COMP: per chain, this lane's running sum over its p-cells and its compensation
+    double accR[NCH], cmpR[NCH], accJ[NCH], cmpJ[NCH];
+#pragma unroll
+    for (int ch = 0; ch < NCH; ch++) accR[ch] = cmpR[ch] = accJ[ch] = cmpJ[ch] = 0.0;
     // sub-cell values of the RTB chains of column c at p-cell j (Rectangle.cpp:
     // 183-204, 258-274): GetInterpolantsREL(f1..f5)[k] (Rectangle.cpp:139-155)
     // plus the correction that makes the sub-cell values average to the cell
@@ -181,6 +209,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         // summands of p-cell j = jb + lane for every chain (Rectangle.cpp:214-253)
         const int j = jb + lane;
         const bool live = j < P.n_p;
+#pragma unroll
         for (int ch = 0; ch < NCH; ch++) {
             int c = ch / RTB;
             int i = i0 + c;
@@ -201,12 +230,17 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
                 tr = t0;
                 tj = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
             }
-            TR[wave][ch][lane] = tr;
-            TJ[wave][ch][lane] = tj;
+            if (COMP) {
+                add_two(accR[ch], cmpR[ch], tr);
+                add_two(accJ[ch], cmpJ[ch], tj);
+            } else {
+                TR[wave][ch][lane] = tr;
+                TJ[wave][ch][lane] = tj;
+            }
         }
         wave_sync();
         // ordered accumulation
-        if ((lane & 31) < NCH) {
+        if (!COMP && (lane & 31) < NCH) {
             const double *row = (lane < 32) ? &TR[wave][lane][0] : &TJ[wave][lane - 32][0];
             double a = acc;
             for (int e = 0; e < 64; e += 8) {
@@ -218,13 +252,32 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         }
         wave_sync();
     }
+    if (COMP) {
+        // one compensated reduction across the lanes per chain
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+            double s, e;
+            wave_two_sum(accR[ch], cmpR[ch], s, e);
+            accR[ch] = s;
+            cmpR[ch] = e;
+            wave_two_sum(accJ[ch], cmpJ[ch], s, e);
+            accJ[ch] = s;
+            cmpJ[ch] = e;
+        }
+    }
     if ((lane & 31) < NCH) {
         int ch = lane & 31;
         int c = ch / RTB, k = ch % RTB;
         int i = i0 + c;
         if (i < P.n_x) {
-            if (lane < 32) P.chargeR[(long)i * RTB + k] = acc * A.dpq;
-            else P.currentR[(long)i * RTB + k] = acc * A.jscale;
+            double sum = acc;
+            if (COMP) {
+#pragma unroll
+                for (int q = 0; q < NCH; q++)
+                    if (q == ch) sum = lane < 32 ? accR[q] + cmpR[q] : accJ[q] + cmpJ[q];
+            }
+            if (lane < 32) P.chargeR[(long)i * RTB + k] = sum * A.dpq;
+            else P.currentR[(long)i * RTB + k] = sum * A.jscale;
         }
     }
 }
@@ -334,7 +387,9 @@ int vx_energy_patch(vx_ctx *ctx, const vx_patch *p, double *energyR) {
     return check_launch(ctx, "k_energy");
 }
 
-int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
+}  // extern "C"
+
+static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bool comp) {
     int rc = validate_patch(ctx, p);
     if (rc) return rc;
     if (p->rtb > MAX_RTB) return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d > %d", p->rtb, MAX_RTB);
@@ -362,13 +417,28 @@ int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
     }
     if ((rc = check_launch(ctx, "k_moment_table"))) return rc;
     LaunchTimer lt(ctx, "k_moments");
-    switch (p->rtb) {
-        case 1: k_moments<1, 4><<<(unsigned)((p->n_x + 15) / 16), 256, 0, ctx->stream>>>(A); break;
-        case 2: k_moments<2, 2><<<(unsigned)((p->n_x + 7) / 8), 256, 0, ctx->stream>>>(A); break;
-        case 4: k_moments<4, 1><<<(unsigned)((p->n_x + 3) / 4), 256, 0, ctx->stream>>>(A); break;
+    const unsigned g1 = (unsigned)((p->n_x + 15) / 16), g2 = (unsigned)((p->n_x + 7) / 8),
+                   g4 = (unsigned)((p->n_x + 3) / 4);
+    switch (p->rtb * 2 + (comp ? 1 : 0)) {
+        case 2: k_moments<1, 4, false><<<g1, 256, 0, ctx->stream>>>(A); break;
+        case 3: k_moments<1, 4, true><<<g1, 256, 0, ctx->stream>>>(A); break;
+        case 4: k_moments<2, 2, false><<<g2, 256, 0, ctx->stream>>>(A); break;
+        case 5: k_moments<2, 2, true><<<g2, 256, 0, ctx->stream>>>(A); break;
+        case 8: k_moments<4, 1, false><<<g4, 256, 0, ctx->stream>>>(A); break;
+        case 9: k_moments<4, 1, true><<<g4, 256, 0, ctx->stream>>>(A); break;
         default: return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d not in {1,2,4}", p->rtb);
     }
     return check_launch(ctx, "k_moments");
+}
+
+extern "C" {
+
+int vx_moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
+    return moments_patch(ctx, p, fl, false);
+}
+
+int vx_moments_patch_compensated(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl) {
+    return moments_patch(ctx, p, fl, true);
 }
 
 int vx_moment_plan_create(vx_ctx *ctx, const vx_patch *patches, const vx_moment_layout *lay,

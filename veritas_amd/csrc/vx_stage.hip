@@ -24,6 +24,15 @@
 //    every earlier stage from stored stage values.
 // Every quantity is computed only inside the reference's loop range and is 0
 // outside it, as the reference's never-written array entries are.
+//
+// Mixed-precision variant (MP): the divisions of the stage (the WENO weights
+// of both face reconstructions, and the Zalesak ratios R+-) run in FP32 on f
+// scaled by an exact power of two (f * 2^-k with 2^k near the patch's max |f|);
+// the candidate face values, the fluxes, the RK combination and every
+// accumulation into f stay FP64.  A face value is fR + w (fL - fR) with the
+// FP32 weight w, so its error is of order 1e-7 |fL - fR|, not 1e-7 |f|.
+
+#include <cmath>
 
 #include "vx_dev.h"
 #include "vx_internal.h"
@@ -48,7 +57,49 @@ struct StageArgs {
     int TI;       // output rows per strip
     int nstrip;   // strips in x
     int nbx;      // tiles along p
+    double sinv;  // MP: 2^-k, the scale of the FP32 copies of f
+    float eps32;  // MP: the WENO epsilon (1e-10) in the scaled units
 };
+
+// WENO face value with the weights in FP32 (g: f * sinv in FP32) and the
+// candidates and the blend in FP64; same candidate choice as vxd::weno.
+__device__ __forceinline__ double weno_mp(double f1, double f2, double f3, double f4, float g1, float g2,
+                                          float g3, float g4, float eps, bool right) {
+    const double fL = (1.0 / 6) * (-f1 + 5 * f2 + 2 * f3);
+    const double fR = (1.0 / 6) * (2 * f2 + 5 * f3 - f4);
+    const float AL = g1 - 2 * g2 + g3, BL = g3 - g1;
+    const float AR = g2 - 2 * g3 + g4, BR = g4 - g2;
+    const float bL = eps + (4.0f / 3 * (AL * AL) + 0.5f * AL * BL + 0.25f * (BL * BL));
+    const float bR = eps + (4.0f / 3 * (AR * AR) - 0.5f * AR * BR + 0.25f * (BR * BR));
+    // oL / (oL + oR) with o = 0.5 / b^2 is bR^2 / (bL^2 + bR^2); both are
+    // divided by the larger b first, so the squares stay in [0, 1]
+    const float m = fmaxf(bL, bR);
+    const bool ok = m > 1e-30f;
+    const float rm = __builtin_amdgcn_rcpf(ok ? m : 1.0f);
+    const float x = bL * rm, y = bR * rm, x2 = x * x, y2 = y * y;
+    const float rden = __builtin_amdgcn_rcpf(x2 + y2);
+    const float wL = ok ? y2 * rden : 0.5f, wR = ok ? x2 * rden : 0.5f;
+    const float wL0 = wL * (0.75f + wL * (wL - 0.5f));
+    const float wR0 = wR * (0.75f + wR * (wR - 0.5f));
+    const bool takeL = right ? (wL0 > wR0) : (wL0 < wR0);
+    const float w = (takeL ? wL0 : wR0) * __builtin_amdgcn_rcpf(wL0 + wR0);
+    return fR + (double)w * (fL - fR);
+}
+
+// Zalesak ratio min(1, Q / P) for P > 0 in FP32 on the scaled values
+__device__ __forceinline__ double zalesak_mp(double Q, double P, double sinv) {
+    const float qf = (float)(Q * sinv), pf = (float)(P * sinv);
+    if (pf > 1e-30f) return (double)fminf(1.0f, qf * __builtin_amdgcn_rcpf(pf));
+    return qf > 0.0f ? 1.0 : 0.0;
+}
+
+// FP32 lane shifts for the scaled copies of f
+__device__ __forceinline__ float shdn1f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ float shup1f(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
+}
 
 // Whole-wavefront lane shifts by one lane (DPP wave_shl:1 / wave_shr:1, plain
 // VALU moves instead of LDS permutes): shdn gives lane l the value of lane
@@ -82,7 +133,7 @@ __device__ __forceinline__ double col_qE(const StageArgs &A, const double *E, in
     return A.p.q * patch_E(E, A.n_f, A.p.rtb, A.p.x_pos, i);
 }
 
-template <int STEP>
+template <int STEP, bool MP>
 __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     const vx_patch &P = A.p;
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
@@ -140,6 +191,9 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     double FxD_m3 = 0, FxD_m2 = 0, FxD_m1 = 0;
     double FpD_m3 = 0, FpD_m2 = 0, FpD_m1 = 0;
     double Rp_m4 = 0, Rp_m3 = 0, Rp_m2 = 0, Rm_m4 = 0, Rm_m3 = 0, Rm_m2 = 0;
+    float gs_m3 = 0, gs_m2 = 0, gs_m1 = 0, gs_0 = 0;  // MP: FP32 copies of f_s rows
+    const double sinv = A.sinv;
+    const float eps32 = A.eps32;
 
     // Gamma(p_j, q^2 A^2 at x-face i), current fields and stage-0 fields
     double as_m1 = face_as(A, A.a2, r0 - 1), as_0 = face_as(A, A.a2, r0);
@@ -175,6 +229,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         // ---- ingest row r
         fs_0 = pfs_a;
         f0_0 = pf0_a;
+        if (MP) gs_0 = (float)(fs_0 * sinv);
         pfs_a = pfs_b;
         pf0_a = pf0_b;
         pfs_b = ldp(fin, pitch, nx, np, r + 2, j);
@@ -199,7 +254,11 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         const bool vFx = ix >= 0 && ix < nx + 1 && jr_ex;
         double Gd = shdn1(G_m1);  // Gamma at p-face j+1
         double ex = (ix >= -1 && ix < nx + 2 && jr_ex) ? dpcc * (Gd - G_m1) : 0.0;
-        double fx = vFx ? weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0) : 0.0;
+        double fx = 0.0;
+        if (vFx) {
+            if (MP) fx = weno_mp(fs_m3, fs_m2, fs_m1, fs_0, gs_m3, gs_m2, gs_m1, gs_0, eps32, ex > 0.0);
+            else fx = weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0);
+        }
         double ex_u = shdn1(ex), ex_d = shup1(ex);
         double fx_u = shdn1(fx), fx_d = shup1(fx);
         double FxH = vFx ? P.dx_inv * (fx * ex + w3 * (fx_u - fx_d) * (ex_u - ex_d)) : 0.0;
@@ -209,7 +268,13 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
         double qE = cr ? col_qE(A, A.E, r) : 0.0;
         ep_0 = (cr && jr_ep) ? qE - ccdx * (G_p1 - G_0) : 0.0;
         double s1 = shup1(fs_0), s2 = shup1(s1), s3 = shdn1(fs_0);
-        fp_0 = (cr && jr_fp) ? weno(s2, s1, fs_0, s3, ep_0 > 0.0) : 0.0;
+        fp_0 = 0.0;
+        if (MP) {
+            float t1 = shup1f(gs_0), t2 = shup1f(t1), t3 = shdn1f(gs_0);
+            if (cr && jr_fp) fp_0 = weno_mp(s2, s1, fs_0, s3, t2, t1, gs_0, t3, eps32, ep_0 > 0.0);
+        } else if (cr && jr_fp) {
+            fp_0 = weno(s2, s1, fs_0, s3, ep_0 > 0.0);
+        }
         // ---- column r-1: FpH_s (Rectangle.cpp:1353-1375)
         const bool vFp = ix >= -1 && ix < nx + 1 && jr_fp;
         double FpH = vFp ? P.dp_inv * (fp_m1 * ep_m1 + w3 * (fp_0 - fp_m2) * (ep_0 - ep_m2)) : 0.0;
@@ -273,8 +338,13 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
                               valmin(valmin(f0_m3, f2_m3), valmin(valmin(f0u, f2u), valmin(f0d, f2d)))));
                 double Qm = -wMin + f2_m2;
                 double Qp = wMax - f2_m2;
-                rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
-                rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
+                if (MP) {
+                    rp = Pp > 0.0 ? zalesak_mp(Qp, Pp, sinv) : 0.0;
+                    rm = Pm > 0.0 ? zalesak_mp(Qm, Pm, sinv) : 0.0;
+                } else {
+                    rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
+                    rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
+                }
             }
             Rp_m2 = rp;
             Rm_m2 = rm;
@@ -308,6 +378,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
 
         // ---- shift the pipeline by one row
         fs_m3 = fs_m2; fs_m2 = fs_m1; fs_m1 = fs_0;
+        gs_m3 = gs_m2; gs_m2 = gs_m1; gs_m1 = gs_0;
         f0_m3 = f0_m2; f0_m2 = f0_m1; f0_m1 = f0_0;
         as_0 = as_p1; G_m1 = G_0; G_0 = G_p1; G0_0 = G0_p1;
         ep_m2 = ep_m1; ep_m1 = ep_0; fp_m2 = fp_m1; fp_m1 = fp_0;
@@ -321,17 +392,23 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     for (int r = r0; r <= i1 + HALO - 1; r++) row(r);
 }
 
-void launch_stage(vx_ctx *ctx, StageArgs &A) {
+template <bool MP>
+void launch_stage_t(vx_ctx *ctx, const StageArgs &A, dim3 grid) {
+    switch (A.step) {
+        case 0: k_stage<0, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 1: k_stage<1, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 2: k_stage<2, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 3: k_stage<3, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 4: k_stage<4, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
+        default: k_stage<5, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
+    }
+}
+
+void launch_stage(vx_ctx *ctx, StageArgs &A, bool mp) {
     A.nstrip = (A.p.n_x + A.TI - 1) / A.TI;
     dim3 grid((unsigned)(A.nbx * ((A.nstrip + 3) / 4)));
-    switch (A.step) {
-        case 0: k_stage<0><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 1: k_stage<1><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 2: k_stage<2><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 3: k_stage<3><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 4: k_stage<4><<<grid, 256, 0, ctx->stream>>>(A); break;
-        default: k_stage<5><<<grid, 256, 0, ctx->stream>>>(A); break;
-    }
+    if (mp) launch_stage_t<true>(ctx, A, grid);
+    else launch_stage_t<false>(ctx, A, grid);
 }
 
 // Rows per strip (the tile height) of a patch shape.  A strip costs its rows
@@ -343,10 +420,10 @@ void launch_stage(vx_ctx *ctx, StageArgs &A) {
 // values for every strip height), and keeps the fastest.  While a step graph
 // is being recorded the choice falls back to the shortest strip that still
 // gives 2048 wavefronts.
-int tile_rows(vx_ctx *ctx, const StageArgs &A0) {
+int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
     const int nx = A0.p.n_x, np = A0.p.n_p;
     for (auto &c : ctx->ti_cache)
-        if (c[0] == nx && c[1] == np) return c[2];
+        if (c[0] == nx && c[1] == np && c[3] == (int)mp) return c[2];
     static const int cand[] = {64, 32, 16, 8, 4, 2};
     int guess = 2;
     for (int ti : cand)
@@ -363,10 +440,10 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0) {
             if (ti > 2 && ti >= 2 * nx) continue;  // a single strip either way
             StageArgs A = A0;
             A.TI = ti;
-            launch_stage(ctx, A);  // warm
+            launch_stage(ctx, A, mp);  // warm
             if (hipEventRecord(e0, ctx->stream) != hipSuccess) break;
-            launch_stage(ctx, A);
-            launch_stage(ctx, A);
+            launch_stage(ctx, A, mp);
+            launch_stage(ctx, A, mp);
             if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
@@ -378,7 +455,7 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0) {
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    ctx->ti_cache.push_back({nx, np, best});
+    ctx->ti_cache.push_back({nx, np, best, (int)mp});
     return best;
 }
 
@@ -388,8 +465,10 @@ extern "C" {
 
 int vx_fct_stage_fused_available(void) { return 1; }
 
-int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
-                       const double *fin, double *fout) {
+}  // extern "C"
+
+static int stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
+                       const double *fin, double *fout, bool mp, double f_scale) {
     int rc = validate_patch(ctx, p);
     if (rc) return rc;
     if (step < 0 || step > 5 || !fl || !fl->E0 || !fl->a20)
@@ -417,15 +496,35 @@ int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     A.aSum = A.a[0];
     for (int k = 1; k < A.ns; k++) A.aSum = A.aSum + A.a[k];
     A.nbx = (p->n_p + OUTW - 1) / OUTW;
-    A.TI = tile_rows(ctx, A);
+    A.sinv = 1.0;
+    A.eps32 = 0.0f;
+    if (mp) {
+        if (!(f_scale > 0.0) || !std::isfinite(f_scale))
+            return fail(ctx, VX_EINVAL, "stage_fused_mp: f scale must be positive and finite");
+        A.sinv = std::ldexp(1.0, -std::ilogb(f_scale));
+        A.eps32 = (float)(1.0e-10 * A.sinv * A.sinv);
+    }
+    A.TI = tile_rows(ctx, A, mp);
     LaunchTimer lt(ctx, "k_stage");
-    launch_stage(ctx, A);
+    launch_stage(ctx, A, mp);
     return check_launch(ctx, "k_stage");
+}
+
+extern "C" {
+
+int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
+                       const double *fin, double *fout) {
+    return stage_fused(ctx, p, step, dt, fl, fin, fout, false, 0.0);
+}
+
+int vx_fct_stage_fused_mp(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
+                          const double *fin, double *fout, double f_scale) {
+    return stage_fused(ctx, p, step, dt, fl, fin, fout, true, f_scale);
 }
 
 int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p) {
     for (auto &c : ctx->ti_cache)
-        if (c[0] == n_x && c[1] == n_p) return c[2];
+        if (c[0] == n_x && c[1] == n_p && c[3] == 0) return c[2];
     return 0;
 }
 
