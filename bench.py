@@ -178,6 +178,70 @@ def bench_amr(a, va):
     print(json.dumps(out), flush=True)
 
 
+def serve_pass(va, reqs, arrivals, a):
+    srv = va.Server(budget_bytes=int(a.budget_gb * (1 << 30)), max_active=a.slots)
+    for k, (r, t) in enumerate(zip(reqs, arrivals)):
+        srv.submit(r["params"], r["steps"], arrival=float(t), rid=k)
+    t0 = time.perf_counter()
+    srv.run()
+    wall = time.perf_counter() - t0
+    rec, st = srv.records(), srv.stats()
+    srv.close()
+    return rec, st, wall
+
+
+def bench_serve(a, va):
+    """Config 5: a seeded parameter-scan stream (Landau, two-stream and laser
+    runs, Nx 128-4096, Np 256-2048, 1-3 mesh levels; `--req-steps` steps each,
+    dt = 0.4 x CFL before every step).  Pass 1 queues every request at t = 0 and
+    measures the capacity (solves/s at saturation); pass 2 replays the stream
+    with Poisson arrivals at 80 % of that capacity and reports the
+    time-to-solution (finish - arrival) percentiles and the solve rate."""
+    import numpy as np
+    from veritas_amd.serve import poisson_arrivals, scan_requests
+    reqs = scan_requests(a.requests, seed=5, steps=a.req_steps)
+    print(f"serve: {len(reqs)} requests, pass 1 (saturation)", file=sys.stderr, flush=True)
+    rec1, st1, wall1 = serve_pass(va, reqs, np.zeros(len(reqs)), a)
+    span1 = max(r["finished"] for r in rec1)
+    capacity = len(rec1) / span1
+    rate = 0.8 * capacity
+    arrivals = poisson_arrivals(len(reqs), rate, seed=7)
+    print(f"serve: capacity {capacity:.2f} solves/s, pass 2 at {rate:.2f}/s", file=sys.stderr, flush=True)
+    rec, st, wall = serve_pass(va, reqs, arrivals, a)
+    lat = np.array([r["finished"] - r["arrival"] for r in rec])
+    span = max(r["finished"] for r in rec) - float(arrivals[0])
+    cu = sum(r["cell_updates"] for r in rec)
+    out = {
+        "metric": "scan solves/hr",
+        "value": len(rec) / span * 3600.0,
+        "unit": "solves/hr",
+        "n_gpus": 1,
+        "steps": a.req_steps,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded parameter-scan stream)",
+        "config": {"workload": f"serving: {len(reqs)} scan requests (Landau / two-stream / laser, Nx 128-4096, "
+                               f"Np 256-2048, 1-3 levels, {a.req_steps} steps each), Poisson arrivals at 80% "
+                               f"of measured capacity",
+                   "budget_gb": a.budget_gb, "slots": a.slots, "capacity_solves_per_s": capacity,
+                   "arrival_rate_per_s": rate},
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "mean": float(lat.mean()), "max": float(lat.max())},
+        "cell_updates_per_s": cu / span,
+        "saturation_pass": {"makespan_s": span1, "cell_updates_per_s": sum(r["cell_updates"] for r in rec1) / span1,
+                            "stats": st1},
+        "stats": st,
+        "batched_requests": int(sum(r["batched"] for r in rec)),
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -192,7 +256,11 @@ def main():
     ap.add_argument("--np", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--workload", choices=["two_stream", "laser_amr"], default="two_stream",
+    ap.add_argument("--requests", type=int, default=256, help="serve: requests in the scan stream")
+    ap.add_argument("--req-steps", type=int, default=40, help="serve: steps per request")
+    ap.add_argument("--budget-gb", type=float, default=32.0, help="serve: device-memory budget")
+    ap.add_argument("--slots", type=int, default=16, help="serve: runs stepped at once")
+    ap.add_argument("--workload", choices=["two_stream", "laser_amr", "serve"], default="two_stream",
                     help="two_stream: 4096^2 single level (the headline); laser_amr: the "
                          "8192x512 three-level laser configuration")
     a = ap.parse_args()
@@ -203,6 +271,8 @@ def main():
     import veritas_amd as va
     if a.workload == "laser_amr":
         return bench_amr(a, va)
+    if a.workload == "serve":
+        return bench_serve(a, va)
     params = va.two_stream(nx=a.nx, np=a.np)
     mixed = a.precision == "mixed"
     run = va.Run(params, fused=(a.mode == "fused"), graphs=False, mixed=mixed)  # eager: per-launch events

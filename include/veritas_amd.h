@@ -119,6 +119,23 @@ int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
  * measured by the tests. */
 int vx_fct_stage_fused_mp(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
                           const double *fin, double *fout, double f_scale);
+/* One patch of a batched stage launch: a walled single patch of an
+ * independent run, with its own fields, planes and dt. */
+typedef struct vx_stage_item {
+    vx_patch p;
+    vx_fields fl;
+    const double *fin; /* f_s */
+    double *fout;      /* f_(s+1) */
+    double dt;
+    double f_scale;    /* > 0: the mixed-precision stage with this f scale; 0: FP64 */
+} vx_stage_item;
+/* vx_fct_stage_fused (or _mp) for up to 64 patches at the same RK stage in
+ * one launch (two when FP64 and mixed items are mixed): each patch's results
+ * are bit-identical to its own launch.  The item arguments are staged through
+ * pinned memory on the context stream, so the patches' inputs must be ready
+ * on that stream (vx_ctx_wait).  No counterpart in the reference: the
+ * serving loop packs the patches of concurrent runs into shared launches. */
+int vx_fct_stage_fused_batch(vx_ctx *ctx, const vx_stage_item *items, int n, int step);
 /* Rows per strip that vx_fct_stage_fused uses for patches of n_x x n_p cells
  * (timed on the first call for that shape; 0 before it).  No counterpart in
  * the reference: it reports the tile shape the library chose. */
@@ -202,6 +219,13 @@ int vx_neutralize(vx_ctx *ctx, const vx_field_state *fs);
  * (host arrays q, m, dp_finest); result written to *out on the host. */
 int vx_cfl_bound(vx_ctx *ctx, const vx_field_state *fs, int n_species, const double *q,
                  const double *m, const double *dp_finest, double *out);
+/* The same bound computed on the device and copied to pinned host memory
+ * without waiting; vx_cfl_result waits for that copy and returns the bound
+ * (bit-identical to vx_cfl_bound: the maximum is exact in any order).  Lets a
+ * caller that steps several runs queue all of them before it waits for one. */
+int vx_cfl_bound_async(vx_ctx *ctx, const vx_field_state *fs, int n_species, const double *q,
+                       const double *m, const double *dp_finest);
+int vx_cfl_result(vx_ctx *ctx, double *out);
 
 /* ---- adaptive mesh: cell-task programs over a table of patches ----
  *
@@ -305,6 +329,11 @@ void vx_graph_destroy(vx_graph *g);
  * device: the bytes are staged in a pinned ring slot (a slot is reused only
  * after its previous copy has completed). */
 int vx_upload_small(vx_ctx *ctx, void *dst, const void *src, size_t bytes);
+/* Order: the waiter's later work starts after everything enqueued so far on
+ * the signaller's stream (an event on the signaller, a wait on the waiter). */
+int vx_ctx_wait(vx_ctx *waiter, vx_ctx *signaller);
+/* Device bytes the context holds through vx_malloc (freed by vx_free). */
+size_t vx_ctx_live_bytes(vx_ctx *ctx);
 
 #ifdef __cplusplus
 }

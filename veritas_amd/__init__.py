@@ -62,6 +62,19 @@ def host_lib():
         L.vxh_set_mixed.argtypes = [vp, i]
         L.vxh_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vxh_tile_rows.argtypes = [vp, i, i]
+        L.vxh_server_create.restype = vp
+        L.vxh_server_create.argtypes = [i, d, i]
+        L.vxh_server_destroy.argtypes = [vp]
+        L.vxh_server_error.restype = ctypes.c_char_p
+        L.vxh_server_error.argtypes = [vp]
+        L.vxh_server_submit.argtypes = [vp, i, vp, i, d, d, i]
+        L.vxh_server_keep_state.argtypes = [vp, i]
+        L.vxh_server_run.argtypes = [vp]
+        L.vxh_server_records.restype = i
+        L.vxh_server_records.argtypes = [vp, vp, i]
+        L.vxh_server_stats.argtypes = [vp, vp]
+        L.vxh_server_final_f.restype = i
+        L.vxh_server_final_f.argtypes = [vp, i, vp, i]
         L.vxh_advance.argtypes = [vp, d, i]
         L.vxh_advance_fields.argtypes = [vp, d, i]
         L.vxh_sync.argtypes = [vp]
@@ -268,3 +281,66 @@ class Run:
     def load(self, path):
         """Replace this run's state with a checkpoint of a run of the same scenario."""
         self._ck(host_lib().vxh_load(self.h, os.fspath(path).encode()))
+
+
+class Server:
+    """Serving loop (host/serve.cpp): requests arrive on the wall clock, are
+    admitted by device-memory footprint as slots and memory free up, and are
+    stepped together, their single walled patches in shared stage launches.
+    `submit(params, steps, arrival)` queues a request (arrival in seconds after
+    `run()` starts); `run()` returns when all have finished."""
+
+    def __init__(self, budget_bytes=64 << 30, max_active=16, device=0, keep_state=False):
+        L = host_lib()
+        self.h = L.vxh_server_create(device, float(budget_bytes), max_active)
+        if not self.h:
+            raise VeritasError(L.vxh_create_error().decode())
+        self._keep = {}
+        L.vxh_server_keep_state(self.h, int(bool(keep_state)))
+        self._n = 0
+
+    def submit(self, params, steps, arrival=0.0, cfl=0.4, mixed=False, rid=None):
+        rid = self._n if rid is None else rid
+        self._n += 1
+        self._keep[rid] = params  # the struct must outlive the call only; keep the shape for final_f
+        host_lib().vxh_server_submit(self.h, rid, ctypes.byref(params), steps, cfl, arrival, int(bool(mixed)))
+        return rid
+
+    def run(self):
+        L = host_lib()
+        if L.vxh_server_run(self.h) != 0:
+            raise VeritasError(L.vxh_server_error(self.h).decode())
+
+    def records(self):
+        L = host_lib()
+        n = L.vxh_server_records(self.h, None, 0)
+        a = np.zeros(n)
+        L.vxh_server_records(self.h, a.ctypes.data, n)
+        keys = ("id", "arrival", "admitted", "started", "finished", "footprint", "cell_updates",
+                "batched", "time")
+        return [dict(zip(keys, row)) for row in a.reshape(-1, 9)]
+
+    def stats(self):
+        out = (ctypes.c_double * 4)()
+        host_lib().vxh_server_stats(self.h, out)
+        return {"peak_bytes": out[0], "peak_active": int(out[1]), "ticks": int(out[2]),
+                "batched_launches": int(out[3])}
+
+    def final_f(self, rid):
+        p = self._keep[rid]
+        L = host_lib()
+        n = L.vxh_server_final_f(self.h, rid, None, 0)
+        if n < 0:
+            raise KeyError(rid)
+        a = np.zeros(n)
+        L.vxh_server_final_f(self.h, rid, a.ctypes.data, n)
+        return a
+
+    def close(self):
+        if getattr(self, "h", None):
+            host_lib().vxh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+

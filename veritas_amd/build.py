@@ -20,7 +20,7 @@ ARCH = os.environ.get("VX_ARCH", "gfx950")
 
 HIP_SOURCES = ["vx_runtime.hip", "vx_fct.hip", "vx_stage.hip", "vx_moments.hip", "vx_field.hip",
                "vx_amr.hip"]
-HOST_SOURCES = ["host/veritas.cpp", "host/amr.cpp", "host/output.cpp", "host/checkpoint.cpp", "host/graph.cpp",
+HOST_SOURCES = ["host/veritas.cpp", "host/amr.cpp", "host/output.cpp", "host/checkpoint.cpp", "host/graph.cpp", "host/serve.cpp",
                 "host/host_capi.cpp"]
 
 HIP_FLAGS = [f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

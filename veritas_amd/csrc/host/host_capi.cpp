@@ -4,6 +4,7 @@
 #include <cstring>
 #include <string>
 
+#include "serve.hpp"
 #include "veritas.hpp"
 
 namespace {
@@ -247,6 +248,88 @@ int vxh_get(void *h, const char *what, int ghosts, double *out, int n) {
         r->err = e.what();
         return -1;
     }
+}
+
+
+// ---- serving loop (host/serve.cpp) ----
+struct ServeHandle {
+    std::unique_ptr<vxh::Server> srv;
+    std::string err;
+};
+
+void *vxh_server_create(int device, double budget_bytes, int max_active) {
+    try {
+        ServeHandle *h = new ServeHandle;
+        h->srv.reset(new vxh::Server(device, (size_t)budget_bytes, max_active));
+        return h;
+    } catch (std::exception &e) {
+        g_create_error = e.what();
+        return nullptr;
+    }
+}
+
+void vxh_server_destroy(void *h) { delete (ServeHandle *)h; }
+const char *vxh_server_error(void *h) { return ((ServeHandle *)h)->err.c_str(); }
+
+int vxh_server_submit(void *h, int id, const vx_scenario_params *prm, int steps, double cfl, double arrival,
+                      int mixed) {
+    vxh::ServeRequest r;
+    r.id = id;
+    r.params = *prm;
+    r.steps = steps;
+    r.cfl = cfl;
+    r.arrival = arrival;
+    r.mixed = mixed != 0;
+    ((ServeHandle *)h)->srv->Submit(r);
+    return 0;
+}
+
+int vxh_server_keep_state(void *h, int on) {
+    ((ServeHandle *)h)->srv->keep_state = on != 0;
+    return 0;
+}
+
+int vxh_server_run(void *h) {
+    ServeHandle *s = (ServeHandle *)h;
+    try {
+        s->srv->Run();
+        return 0;
+    } catch (std::exception &e) {
+        s->err = e.what();
+        return -1;
+    }
+}
+
+// 9 doubles per finished request: id, arrival, admitted, started, finished,
+// footprint bytes, cell-updates, batched, simulated end time
+int vxh_server_records(void *h, double *out, int n) {
+    const auto &rec = ((ServeHandle *)h)->srv->records;
+    int m = (int)rec.size() * 9;
+    if (out && n >= m)
+        for (size_t k = 0; k < rec.size(); k++) {
+            const auto &r = rec[k];
+            double v[9] = {(double)r.id, r.arrival, r.admitted, r.started, r.finished, (double)r.footprint,
+                           r.cell_updates, r.batched ? 1.0 : 0.0, r.time};
+            std::memcpy(out + 9 * k, v, sizeof v);
+        }
+    return m;
+}
+
+// peak footprint bytes, peak active runs, ticks, batched launches
+int vxh_server_stats(void *h, double *out) {
+    const auto &s = *((ServeHandle *)h)->srv;
+    out[0] = (double)s.peak_bytes;
+    out[1] = s.peak_active;
+    out[2] = (double)s.ticks;
+    out[3] = (double)s.batched_launches;
+    return 0;
+}
+
+int vxh_server_final_f(void *h, int id, double *out, int n) {
+    const auto &m = ((ServeHandle *)h)->srv->final_f;
+    auto it = m.find(id);
+    if (it == m.end()) return -1;
+    return copy_out(it->second, out, n);
 }
 
 }  // extern "C"

@@ -268,6 +268,26 @@ void Rectangle::FCTStageFused(double timestep, int step, bool mixed) {
     slot_version++;
 }
 
+void Rectangle::FusedItem(double timestep, int step, bool mixed, vx_stage_item *item) {
+    double *out = nullptr;
+    for (double *b : buf)
+        if (b != P.f[0] && b != P.f[1]) {
+            out = b;
+            break;
+        }
+    item->p = P;
+    item->fl = EMSolver->fields();
+    item->fin = P.f[1];
+    item->fout = out;
+    item->dt = timestep;
+    item->f_scale = mixed ? (fScale > 0.0 ? fScale : settings.GetfMax(particleType)) : 0.0;
+    P.f[1] = out;
+    if (step == 5) P.f[0] = out;
+    for (double *b : buf)
+        if (b != P.f[0] && b != P.f[1]) P.f[2] = b;
+    slot_version++;
+}
+
 void Rectangle::UnaliasSlots() {
     if (P.f[0] != P.f[1]) return;
     double *spare[2];
@@ -543,6 +563,42 @@ void SolverManager::AdvanceFields(double timeStep) {
         settings.UpdateTime(i, timeStep);
         EMSolver->RGKStep(i, timeStep);
     }
+}
+
+void SolverManager::StageBegin(double dt, int step) {
+    (void)dt;
+    EMSolver->AssembleRhoAndJ();
+    EMSolver->UpdatePotential();
+    if (step == 0) EMSolver->SnapshotStage0Fields();
+}
+
+bool SolverManager::FusedItem(double dt, int step, vx_stage_item *item) {
+    if (meshes.size() != 1) return false;
+    Mesh &m = *meshes[0];
+    if (!m.fused || !m.SinglePatch()) return false;
+    Rectangle *r = m.levels.back()->rectangles[0].get();
+    if (!r->AllWalls()) return false;
+    r->FusedItem(dt, step, m.mixed, item);
+    return true;
+}
+
+void SolverManager::StageEnd(double dt, int step) {
+    settings.UpdateTime(step, dt);
+    EMSolver->RGKStep(step, dt);
+}
+
+void SolverManager::CflAsync() {
+    std::vector<double> dpf;
+    for (unsigned i = 0; i < settings.q.size(); i++) dpf.push_back(settings.GetDp(0, i));
+    dev->check(vx_cfl_bound_async(dev->ctx, &EMSolver->state(), (int)settings.q.size(), settings.q.data(),
+                                  settings.m.data(), dpf.data()),
+               "cfl async");
+}
+
+double SolverManager::CflResult(double cfl) {
+    double b = 0.0;
+    dev->check(vx_cfl_result(dev->ctx, &b), "cfl result");
+    return cfl * b;
 }
 
 void SolverManager::StageVlasov(double dt, int step) {

@@ -121,6 +121,9 @@ public:
     void FCTTimeStep(double timestep, int step, int subStep);
     void FCTStageWalls(double timestep, int step);  // sub-step kernels, whole stage (walls only)
     void FCTStageFused(double timestep, int step, bool mixed = false);  // one fused kernel per stage (walls only)
+    // the same stage as an item of a batched launch (vx_fct_stage_fused_batch);
+    // the planes rotate as FCTStageFused rotates them
+    void FusedItem(double timestep, int step, bool mixed, vx_stage_item *item);
     void UnaliasSlots();  // give f[0] and f[1] distinct planes (before sub-step kernels)
     int slot_version = 0;  // bumped whenever the f plane pointers change
     // the f[0..2] plane pointers and their version (saved and restored around
@@ -292,6 +295,14 @@ public:
     void StageVlasovSub(double dt, int step, int sub);
     void StageFields(double dt, int step);
     void SetFused(bool on);
+    // One step of Advance in pieces, for a caller that interleaves several runs
+    // (the serving loop, host/serve.cpp): per stage StageBegin, then either the
+    // batched item (FusedItem true) or StageVlasov, then StageEnd.
+    void StageBegin(double dt, int step);  // moments, Poisson, stage-0 snapshot
+    bool FusedItem(double dt, int step, vx_stage_item *item);
+    void StageEnd(double dt, int step);    // stage clock and the Maxwell stage
+    void CflAsync();                       // vx_cfl_bound_async for the next dt
+    double CflResult(double cfl);          // cfl * the bound (CalculateDt)
     // mixed-precision form (single walled patch): FP32 WENO weights and Zalesak
     // ratios in the fused stage, compensated FP64 moment sums; FP64 otherwise
     void SetMixed(bool on);

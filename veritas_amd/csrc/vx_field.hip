@@ -16,6 +16,7 @@
 //   solution (the CPU oracle reaches the same point with a band LU).
 //
 // Transverse fields (EMSolver.cpp:194-338, 479-619): one thread per cell.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -751,6 +752,90 @@ int vx_neutralize(vx_ctx *ctx, const vx_field_state *fs) {
     if (rc) return rc;
     k_neutralize<<<(unsigned)((fs->n_f + 255) / 256), 256, 0, ctx->stream>>>(*fs);
     return check_launch(ctx, "k_neutralize");
+}
+
+}  // extern "C"
+
+// EMSolver.cpp:631-664 on the device: pc = max_i (the per-cell term), each term
+// with the host code's operations; max is exact, so the order does not matter.
+constexpr int CFL_MAXSP = 8;
+struct CflArgs {
+    vx_field_state fs;
+    int n_species;
+    double aq[CFL_MAXSP], m[CFL_MAXSP], dps[CFL_MAXSP];  // |q_j|, m_j, 1 / dp_j
+    double dpsMax;
+};
+
+__global__ __launch_bounds__(1024) void k_cfl(CflArgs A, double *out) {
+    const vx_field_state &F = A.fs;
+    __shared__ double part[16];
+    double pc = 0.0;
+    for (int i = threadIdx.x; i < F.n_f; i += 1024) {
+        const double Azv = F.Az[i], Ayv = F.Ay[i];
+        const double As = Ayv * Ayv + Azv * Azv;
+        double temp = 0.0;
+        for (int j = 0; j < A.n_species; j++) {
+            const double v = A.aq[j] / A.m[j] / sqrt(1 + As / ((A.m[j] * vx_cs) * (A.m[j] * vx_cs))) * A.dps[j];
+            temp = (temp < v) ? v : temp;
+        }
+        const double t = temp * fabs(Ayv * F.Bz[i] - Azv * F.By[i]) + A.dpsMax * fabs(F.E[i]);
+        pc = (pc < t) ? t : pc;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(pc, off, 64);
+        pc = (pc < o) ? o : pc;
+    }
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = pc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int k = 0; k < 16; k++) r = (r < part[k]) ? part[k] : r;
+        *out = r;
+    }
+}
+
+extern "C" {
+
+int vx_cfl_bound_async(vx_ctx *ctx, const vx_field_state *fs, int n_species, const double *q, const double *m,
+                       const double *dp_finest) {
+    int rc = validate_fs(ctx, fs);
+    if (rc) return rc;
+    if (n_species < 1 || n_species > CFL_MAXSP) return fail(ctx, VX_EINVAL, "cfl: %d species", n_species);
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "cfl bound while a step graph is recorded");
+    if (!ctx->cfl_dev) {
+        if ((rc = check_hip(ctx, hipMalloc((void **)&ctx->cfl_dev, sizeof(double)), "cfl buffer"))) return rc;
+        if ((rc = check_hip(ctx, hipHostMalloc((void **)&ctx->cfl_host, sizeof(double)), "cfl pinned"))) return rc;
+        if ((rc = check_hip(ctx, hipEventCreateWithFlags(&ctx->cfl_ev, hipEventDisableTiming), "cfl event")))
+            return rc;
+    }
+    CflArgs A;
+    A.fs = *fs;
+    A.n_species = n_species;
+    A.dpsMax = 0.0;
+    for (int i = 0; i < n_species; i++) {  // EMSolver.cpp:637-642
+        A.aq[i] = std::fabs(q[i]);
+        A.m[i] = m[i];
+        A.dps[i] = 1 / dp_finest[i];
+        A.dpsMax = std::max(A.dpsMax, std::fabs(q[i]) * A.dps[i]);
+    }
+    k_cfl<<<1, 1024, 0, ctx->stream>>>(A, ctx->cfl_dev);
+    if ((rc = check_launch(ctx, "k_cfl"))) return rc;
+    rc = check_hip(ctx, hipMemcpyAsync(ctx->cfl_host, ctx->cfl_dev, sizeof(double), hipMemcpyDeviceToHost,
+                                       ctx->stream), "cfl copy");
+    if (rc) return rc;
+    if ((rc = check_hip(ctx, hipEventRecord(ctx->cfl_ev, ctx->stream), "cfl event record"))) return rc;
+    ctx->cfl_c_dx = vx_cs / fs->dx_f;
+    ctx->cfl_pending = true;
+    return VX_OK;
+}
+
+int vx_cfl_result(vx_ctx *ctx, double *out) {
+    if (!ctx->cfl_pending || !out) return fail(ctx, VX_EINVAL, "cfl: no bound pending");
+    int rc = check_hip(ctx, hipEventSynchronize(ctx->cfl_ev), "cfl wait");
+    if (rc) return rc;
+    ctx->cfl_pending = false;
+    *out = 1.0 / std::max((ctx->cfl_c_dx + *ctx->cfl_host), 1e-40);
+    return VX_OK;
 }
 
 // EMSolver.cpp:631-664, evaluated on the host from downloaded arrays (the

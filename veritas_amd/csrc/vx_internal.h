@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <array>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/veritas_amd.h"
@@ -25,6 +26,18 @@ struct vx_ctx {
     size_t mom_table_n = 0;
     // strip heights of the fused stage chosen per patch shape: {n_x, n_p, rows, mixed}
     std::vector<std::array<int, 4>> ti_cache;
+    // device bytes held through vx_malloc (vx_ctx_live_bytes)
+    std::unordered_map<void *, size_t> allocs;
+    size_t live_bytes = 0;
+    // cross-context ordering (vx_ctx_wait) and the asynchronous CFL bound
+    hipEvent_t xev = nullptr, cfl_ev = nullptr;
+    double *cfl_host = nullptr;  // pinned
+    double *cfl_dev = nullptr;
+    double cfl_c_dx = 0.0;       // c / dx_f of the pending bound
+    bool cfl_pending = false;
+    // staging of batched stage launches (vx_stage.hip), freed by its own function
+    void *batch_ring = nullptr;
+    void (*batch_ring_free)(void *) = nullptr;
     // step-graph recording (vx_graph_begin / vx_graph_end)
     bool capturing = false;
     // pinned staging ring of vx_upload_small

@@ -35,6 +35,11 @@ void vx_ctx_destroy(vx_ctx *ctx) {
         if (e) (void)hipEventDestroy(e);
     for (auto &e : ctx->tev) (void)hipEventDestroy(e);
     if (ctx->mom_table) (void)hipFree(ctx->mom_table);
+    if (ctx->batch_ring && ctx->batch_ring_free) ctx->batch_ring_free(ctx->batch_ring);
+    if (ctx->xev) (void)hipEventDestroy(ctx->xev);
+    if (ctx->cfl_ev) (void)hipEventDestroy(ctx->cfl_ev);
+    if (ctx->cfl_host) (void)hipHostFree(ctx->cfl_host);
+    if (ctx->cfl_dev) (void)hipFree(ctx->cfl_dev);
     for (auto &e : ctx->ring_ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->ring) (void)hipHostFree(ctx->ring);
@@ -63,6 +68,8 @@ int vx_malloc(vx_ctx *ctx, size_t bytes, void **dptr) {
     if (bytes == 0) return VX_OK;
     hipError_t e = hipMalloc(dptr, bytes);
     if (e != hipSuccess) return fail(ctx, VX_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    ctx->allocs[*dptr] = bytes;
+    ctx->live_bytes += bytes;
     e = hipMemsetAsync(*dptr, 0, bytes, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     return check_hip(ctx, e, "zero fill");
@@ -71,6 +78,11 @@ int vx_malloc(vx_ctx *ctx, size_t bytes, void **dptr) {
 int vx_free(vx_ctx *ctx, void *dptr) {
     if (!dptr) return VX_OK;
     if (int rc = not_recording(ctx, "vx_free")) return rc;
+    auto it = ctx->allocs.find(dptr);
+    if (it != ctx->allocs.end()) {
+        ctx->live_bytes -= it->second;
+        ctx->allocs.erase(it);
+    }
     (void)hipStreamSynchronize(ctx->stream);
     return check_hip(ctx, hipFree(dptr), "hipFree");
 }
@@ -217,5 +229,20 @@ int vx_upload_small(vx_ctx *ctx, void *dst, const void *src, size_t bytes) {
     ctx->ring_pending[k] = rc == VX_OK;
     return rc;
 }
+
+
+int vx_ctx_wait(vx_ctx *waiter, vx_ctx *signaller) {
+    if (!waiter || !signaller) return VX_EINVAL;
+    if (waiter->device != signaller->device) return fail(waiter, VX_EINVAL, "ctx wait across devices");
+    if (!signaller->xev) {
+        int rc = check_hip(waiter, hipEventCreateWithFlags(&signaller->xev, hipEventDisableTiming), "ctx event");
+        if (rc) return rc;
+    }
+    int rc = check_hip(waiter, hipEventRecord(signaller->xev, signaller->stream), "ctx event record");
+    if (rc) return rc;
+    return check_hip(waiter, hipStreamWaitEvent(waiter->stream, signaller->xev, 0), "ctx stream wait");
+}
+
+size_t vx_ctx_live_bytes(vx_ctx *ctx) { return ctx ? ctx->live_bytes : 0; }
 
 }  // extern "C"

@@ -33,6 +33,8 @@
 // FP32 weight w, so its error is of order 1e-7 |fL - fR|, not 1e-7 |f|.
 
 #include <cmath>
+#include <cstring>
+#include <vector>
 
 #include "vx_dev.h"
 #include "vx_internal.h"
@@ -133,19 +135,23 @@ __device__ __forceinline__ double col_qE(const StageArgs &A, const double *E, in
     return A.p.q * patch_E(E, A.n_f, A.p.rtb, A.p.x_pos, i);
 }
 
+// XCD-aware tile order: the dispatcher hands workgroup w to XCD w mod 8, so
+// renumber workgroups so that each XCD receives a contiguous run of tiles;
+// p-neighbouring tiles then march over the same rows on the same XCD and
+// share the cache lines of their overlapping halos in that XCD's L2.
+__device__ __forceinline__ int xcd_tile() {
+    const int nwg = gridDim.x, w = blockIdx.x;
+    const int xcd = w & 7, q = nwg >> 3, rem = nwg & 7;
+    return xcd * q + min(xcd, rem) + (w >> 3);
+}
+
+// One workgroup's tile (4 strips of one 64-lane p-column) of a stage.
 template <int STEP, bool MP>
-__global__ __launch_bounds__(256) void k_stage(StageArgs A) {
+__device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
     const vx_patch &P = A.p;
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
     const int lane = threadIdx.x & 63;
     // the wavefront index is uniform: row loop, row ranges and field reads stay scalar
-    // XCD-aware tile order: the dispatcher hands workgroup w to XCD w mod 8, so
-    // renumber workgroups so that each XCD receives a contiguous run of tiles;
-    // p-neighbouring tiles then march over the same rows on the same XCD and
-    // share the cache lines of their overlapping halos in that XCD's L2.
-    const int nwg = gridDim.x, w = blockIdx.x;
-    const int xcd = w & 7, q = nwg >> 3, rem = nwg & 7;
-    const int tile = xcd * q + min(xcd, rem) + (w >> 3);
     const int bx = tile % A.nbx, by = tile / A.nbx;
     const int strip = by * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (strip >= A.nstrip) return;  // uniform per wavefront
@@ -392,6 +398,26 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     for (int r = r0; r <= i1 + HALO - 1; r++) row(r);
 }
 
+template <int STEP, bool MP>
+__global__ __launch_bounds__(256) void k_stage(StageArgs A) {
+    stage_tile<STEP, MP>(A, xcd_tile());
+}
+
+// Several patches (of independent runs) at the same RK stage in one launch:
+// workgroups [wg_start[k], wg_start[k+1]) of the XCD-ordered tile sequence
+// belong to item k.
+template <int STEP, bool MP>
+__global__ __launch_bounds__(256) void k_stage_batch(const StageArgs *items, const int *wg_start, int n) {
+    const int tile = xcd_tile();
+    int lo = 0, hi = n - 1;  // the last k with wg_start[k] <= tile (uniform)
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (wg_start[mid] <= tile) lo = mid;
+        else hi = mid - 1;
+    }
+    stage_tile<STEP, MP>(items[lo], tile - wg_start[lo]);
+}
+
 template <bool MP>
 void launch_stage_t(vx_ctx *ctx, const StageArgs &A, dim3 grid) {
     switch (A.step) {
@@ -467,8 +493,9 @@ int vx_fct_stage_fused_available(void) { return 1; }
 
 }  // extern "C"
 
-static int stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
-                       const double *fin, double *fout, bool mp, double f_scale) {
+// Checks and kernel arguments of one patch's fused stage.
+static int stage_args(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
+                      const double *fin, double *fout, bool mp, double f_scale, StageArgs &A) {
     int rc = validate_patch(ctx, p);
     if (rc) return rc;
     if (step < 0 || step > 5 || !fl || !fl->E0 || !fl->a20)
@@ -480,7 +507,6 @@ static int stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
         return fail(ctx, VX_EINVAL, "stage_fused: input aliases f^n after stage 0");
     if (fout == fin || fout == p->f[0])
         return fail(ctx, VX_EINVAL, "stage_fused: output must not alias the inputs");
-    StageArgs A;
     A.p = *p;
     A.fin = fin;
     A.fout = fout;
@@ -505,9 +531,92 @@ static int stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
         A.eps32 = (float)(1.0e-10 * A.sinv * A.sinv);
     }
     A.TI = tile_rows(ctx, A, mp);
+    A.nstrip = (p->n_x + A.TI - 1) / A.TI;
+    return VX_OK;
+}
+
+static int stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
+                       const double *fin, double *fout, bool mp, double f_scale) {
+    StageArgs A;
+    int rc = stage_args(ctx, p, step, dt, fl, fin, fout, mp, f_scale, A);
+    if (rc) return rc;
     LaunchTimer lt(ctx, "k_stage");
     launch_stage(ctx, A, mp);
     return check_launch(ctx, "k_stage");
+}
+
+// Staging of batched launch arguments: device slots with pinned host mirrors;
+// a slot is rewritten only after the copy that last used it has completed.
+constexpr int BATCH_MAX = 64, BATCH_SLOTS = 8;
+constexpr size_t BATCH_SLOT_BYTES = sizeof(StageArgs) * BATCH_MAX + sizeof(int) * (BATCH_MAX + 1) + 256;
+
+struct BatchRing {
+    char *dev = nullptr, *host = nullptr;
+    hipEvent_t ev[BATCH_SLOTS] = {};
+    bool pending[BATCH_SLOTS] = {};
+    int next = 0;
+};
+
+static void batch_ring_free(void *r) {
+    BatchRing *b = (BatchRing *)r;
+    for (auto &e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->dev) (void)hipFree(b->dev);
+    if (b->host) (void)hipHostFree(b->host);
+    delete b;
+}
+
+static int batch_launch(vx_ctx *ctx, const std::vector<StageArgs> &A, int step, bool mp) {
+    if (A.empty()) return VX_OK;
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "stage batch while a step graph is recorded");
+    if (!ctx->batch_ring) {
+        BatchRing *b = new BatchRing;
+        ctx->batch_ring = b;
+        ctx->batch_ring_free = batch_ring_free;
+        int rc = check_hip(ctx, hipMalloc((void **)&b->dev, BATCH_SLOT_BYTES * BATCH_SLOTS), "batch args");
+        if (!rc) rc = check_hip(ctx, hipHostMalloc((void **)&b->host, BATCH_SLOT_BYTES * BATCH_SLOTS), "batch staging");
+        for (auto &e : b->ev)
+            if (!rc) rc = check_hip(ctx, hipEventCreateWithFlags(&e, hipEventDisableTiming), "batch event");
+        if (rc) return rc;
+    }
+    BatchRing *b = (BatchRing *)ctx->batch_ring;
+    const int k = b->next;
+    b->next = (k + 1) % BATCH_SLOTS;
+    if (b->pending[k]) {
+        int rc = check_hip(ctx, hipEventSynchronize(b->ev[k]), "batch slot wait");
+        if (rc) return rc;
+    }
+    const int n = (int)A.size();
+    char *h = b->host + BATCH_SLOT_BYTES * k, *d = b->dev + BATCH_SLOT_BYTES * k;
+    std::memcpy(h, A.data(), sizeof(StageArgs) * n);
+    int *start = (int *)(h + sizeof(StageArgs) * BATCH_MAX);
+    int total = 0;
+    for (int q = 0; q < n; q++) {
+        start[q] = total;
+        total += A[q].nbx * ((A[q].nstrip + 3) / 4);
+    }
+    start[n] = total;
+    int rc = check_hip(ctx, hipMemcpyAsync(d, h, BATCH_SLOT_BYTES, hipMemcpyHostToDevice, ctx->stream), "batch upload");
+    if (rc) return rc;
+    rc = check_hip(ctx, hipEventRecord(b->ev[k], ctx->stream), "batch event record");
+    b->pending[k] = rc == VX_OK;
+    if (rc) return rc;
+    const StageArgs *dA = (const StageArgs *)d;
+    const int *dS = (const int *)(d + sizeof(StageArgs) * BATCH_MAX);
+    dim3 grid((unsigned)total);
+#define VX_BATCH(S)                                                                                   \
+    if (mp) k_stage_batch<S, true><<<grid, 256, 0, ctx->stream>>>(dA, dS, n);                        \
+    else k_stage_batch<S, false><<<grid, 256, 0, ctx->stream>>>(dA, dS, n);
+    switch (step) {
+        case 0: VX_BATCH(0) break;
+        case 1: VX_BATCH(1) break;
+        case 2: VX_BATCH(2) break;
+        case 3: VX_BATCH(3) break;
+        case 4: VX_BATCH(4) break;
+        default: VX_BATCH(5) break;
+    }
+#undef VX_BATCH
+    return check_launch(ctx, "k_stage_batch");
 }
 
 extern "C" {
@@ -520,6 +629,23 @@ int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
 int vx_fct_stage_fused_mp(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
                           const double *fin, double *fout, double f_scale) {
     return stage_fused(ctx, p, step, dt, fl, fin, fout, true, f_scale);
+}
+
+int vx_fct_stage_fused_batch(vx_ctx *ctx, const vx_stage_item *items, int n, int step) {
+    if (n < 0 || n > BATCH_MAX || (n > 0 && !items))
+        return fail(ctx, VX_EINVAL, "stage batch: %d items (at most %d)", n, BATCH_MAX);
+    std::vector<StageArgs> fp64, mixed;
+    for (int q = 0; q < n; q++) {
+        const vx_stage_item &it = items[q];
+        StageArgs A;
+        const bool mp = it.f_scale > 0.0;
+        int rc = stage_args(ctx, &it.p, step, it.dt, &it.fl, it.fin, it.fout, mp, it.f_scale, A);
+        if (rc) return rc;
+        (mp ? mixed : fp64).push_back(A);
+    }
+    LaunchTimer lt(ctx, "k_stage");
+    int rc = batch_launch(ctx, fp64, step, false);
+    return rc ? rc : batch_launch(ctx, mixed, step, true);
 }
 
 int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p) {
