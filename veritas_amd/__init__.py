@@ -13,6 +13,7 @@ import os
 
 import numpy as np
 
+from . import serve  # noqa: F401
 from .params import ScenarioParams, landau, laser, two_stream  # noqa: F401
 
 HERE = os.path.dirname(os.path.abspath(__file__))
