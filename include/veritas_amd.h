@@ -226,6 +226,8 @@ int vx_cfl_bound(vx_ctx *ctx, const vx_field_state *fs, int n_species, const dou
 int vx_cfl_bound_async(vx_ctx *ctx, const vx_field_state *fs, int n_species, const double *q,
                        const double *m, const double *dp_finest);
 int vx_cfl_result(vx_ctx *ctx, double *out);
+/* *ready = 1 when the pending bound has arrived (vx_cfl_result will not wait). */
+int vx_cfl_ready(vx_ctx *ctx, int *ready);
 
 /* ---- adaptive mesh: cell-task programs over a table of patches ----
  *

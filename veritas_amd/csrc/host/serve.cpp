@@ -4,18 +4,20 @@
 // arrived, a slot is free (at most max_active runs at once) and its estimated
 // device footprint fits the budget beside the runs already held; a worker
 // thread then builds it (settings, initial distribution, uploads, the first
-// dt) while the main thread keeps stepping the others.  The main thread
-// advances every active run by one step per tick, stage by stage in lock-step:
+// dt) while the main thread keeps stepping the others.  Runs progress
+// independently: each pass of the main loop takes the runs whose previous step
+// has finished (their CFL bound has arrived; polled, not waited for) and
+// advances each of them by one step, stage by stage in lock-step:
 //   * each run's moments, Poisson solve and stage-0 snapshot on its own stream;
 //   * the Vlasov stage of every single walled patch in one shared launch on
 //     the server's stream (vx_fct_stage_fused_batch), ordered after the runs'
 //     streams and before them again with stream waits; adaptive-mesh runs
 //     take their own sub-step kernels on their own streams;
 //   * each run's stage clock and Maxwell stage on its own stream.
-// After a tick every run queues its CFL bound on the device
-// (vx_cfl_bound_async) before the loop waits for any of them, so the next
-// dt of every run is known without serialising the runs.  A finished run is
-// destroyed at once, which frees its memory for the next admission.
+// After its step every run queues its CFL bound on the device
+// (vx_cfl_bound_async); a small run thus takes many steps while a large one
+// takes one.  A finished run is destroyed at once, which frees its memory for
+// the next admission.
 //
 // Each run computes exactly what it computes alone: the same kernels with the
 // same arguments, and the shared launch gives each patch the bits of its own
@@ -60,7 +62,6 @@ size_t ServeEstimate(const vx_scenario_params &p) {
 Server::Server(int dev, size_t budget_, int max_active_)
     : device(dev), budget(budget_), max_active(std::max(1, std::min(max_active_, 64))) {
     batch.reset(new Device(device));
-    worker = std::thread([this] { Build(); });
 }
 
 Server::~Server() {
@@ -69,7 +70,8 @@ Server::~Server() {
         stop = true;
     }
     cv.notify_all();
-    if (worker.joinable()) worker.join();
+    for (auto &w : workers)
+        if (w.joinable()) w.join();
 }
 
 void Server::Submit(const ServeRequest &r) {
@@ -108,18 +110,18 @@ void Server::Build() {
     }
 }
 
-void Server::Tick(std::vector<std::unique_ptr<Live>> &active) {
+void Server::Tick(const std::vector<Live *> &ready) {
     std::vector<vx_stage_item> items;
     std::vector<Live *> shared;
     for (int s = 0; s < 6; s++) {
         items.clear();
         shared.clear();
-        for (auto &L : active) L->sm->StageBegin(L->dt, s);
-        for (auto &L : active) {
+        for (Live *L : ready) L->sm->StageBegin(L->dt, s);
+        for (Live *L : ready) {
             vx_stage_item it;
             if (L->sm->FusedItem(L->dt, s, &it)) {
                 items.push_back(it);
-                shared.push_back(L.get());
+                shared.push_back(L);
             } else {
                 for (auto &m : L->sm->meshes) m->Advance(L->dt, s);
             }
@@ -130,18 +132,20 @@ void Server::Tick(std::vector<std::unique_ptr<Live>> &active) {
             batched_launches++;
             for (Live *L : shared) batch->check(vx_ctx_wait(L->sm->dev->ctx, batch->ctx), "run waits for batch");
         }
-        for (auto &L : active) L->sm->StageEnd(L->dt, s);
+        for (Live *L : ready) L->sm->StageEnd(L->dt, s);
     }
-    for (auto &L : active) {
+    for (Live *L : ready) {
         L->done++;
-        L->rec.batched = L->rec.batched || std::find(shared.begin(), shared.end(), L.get()) != shared.end();
-        if (L->done < L->req.steps) L->sm->CflAsync();
+        L->rec.batched = L->rec.batched || std::find(shared.begin(), shared.end(), L) != shared.end();
+        L->sm->CflAsync();  // next dt, or the end-of-run marker
+        L->waiting = true;
     }
     ticks++;
 }
 
 void Server::Run() {
     const auto t0 = std::chrono::steady_clock::now();
+    while ((int)workers.size() < std::max(1, builders)) workers.emplace_back([this] { Build(); });
     std::vector<std::unique_ptr<Live>> active;
     size_t held = 0;   // footprints of admitted runs (estimates until built)
     int building = 0;
@@ -195,10 +199,18 @@ void Server::Run() {
             continue;
         }
         peak_active = std::max(peak_active, (int)active.size());
-        Tick(active);
-        // finished runs leave; the others take their next dt
+        // runs whose last step has finished: they leave, or take their next dt
+        std::vector<Live *> ready;
         for (size_t k = 0; k < active.size();) {
             Live &L = *active[k];
+            if (L.waiting && !L.sm->CflReady()) {
+                k++;
+                continue;
+            }
+            if (L.waiting) {
+                L.dt = L.sm->CflResult(L.req.cfl);
+                L.waiting = false;
+            }
             if (L.done >= L.req.steps) {
                 L.sm->dev->check(vx_sync(L.sm->dev->ctx), "finish");
                 L.rec.finished = now_s(t0);
@@ -210,11 +222,16 @@ void Server::Run() {
                 records.push_back(L.rec);
                 held -= L.bytes;
                 active.erase(active.begin() + k);
-            } else {
-                L.dt = L.sm->CflResult(L.req.cfl);
-                k++;
+                continue;
             }
+            ready.push_back(&L);
+            k++;
         }
+        if (ready.empty()) {
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            continue;
+        }
+        Tick(ready);
     }
     if (!failure.empty()) throw Error("serve: " + failure);
 }

@@ -45,6 +45,7 @@ public:
     size_t peak_bytes = 0;  // largest sum of the footprints held at once
     int peak_active = 0;
     long ticks = 0, batched_launches = 0;
+    int builders = 3;  // threads that build admitted runs (set before Run)
 
 private:
     struct Live {
@@ -54,6 +55,7 @@ private:
         std::unique_ptr<SolverManager> sm;
         double dt = 0.0;
         int done = 0;
+        bool waiting = false;  // a step is in flight (its CFL bound pending)
         size_t bytes = 0;
         std::string error;
     };
@@ -67,9 +69,9 @@ private:
     std::condition_variable cv;
     std::deque<std::unique_ptr<Live>> to_build, built;
     bool stop = false;
-    std::thread worker;
+    std::vector<std::thread> workers;
     void Build();
-    void Tick(std::vector<std::unique_ptr<Live>> &active);
+    void Tick(const std::vector<Live *> &ready);
 };
 
 size_t ServeEstimate(const vx_scenario_params &p);

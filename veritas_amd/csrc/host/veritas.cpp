@@ -595,6 +595,12 @@ void SolverManager::CflAsync() {
                "cfl async");
 }
 
+bool SolverManager::CflReady() {
+    int r = 0;
+    dev->check(vx_cfl_ready(dev->ctx, &r), "cfl ready");
+    return r != 0;
+}
+
 double SolverManager::CflResult(double cfl) {
     double b = 0.0;
     dev->check(vx_cfl_result(dev->ctx, &b), "cfl result");

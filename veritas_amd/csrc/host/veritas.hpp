@@ -303,6 +303,7 @@ public:
     void StageEnd(double dt, int step);    // stage clock and the Maxwell stage
     void CflAsync();                       // vx_cfl_bound_async for the next dt
     double CflResult(double cfl);          // cfl * the bound (CalculateDt)
+    bool CflReady();                       // the pending bound has arrived
     // mixed-precision form (single walled patch): FP32 WENO weights and Zalesak
     // ratios in the fused stage, compensated FP64 moment sums; FP64 otherwise
     void SetMixed(bool on);

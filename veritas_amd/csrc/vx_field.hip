@@ -829,6 +829,17 @@ int vx_cfl_bound_async(vx_ctx *ctx, const vx_field_state *fs, int n_species, con
     return VX_OK;
 }
 
+int vx_cfl_ready(vx_ctx *ctx, int *ready) {
+    if (!ctx->cfl_pending || !ready) return fail(ctx, VX_EINVAL, "cfl: no bound pending");
+    hipError_t e = hipEventQuery(ctx->cfl_ev);
+    if (e == hipErrorNotReady) {
+        *ready = 0;
+        return VX_OK;
+    }
+    *ready = 1;
+    return check_hip(ctx, e, "cfl query");
+}
+
 int vx_cfl_result(vx_ctx *ctx, double *out) {
     if (!ctx->cfl_pending || !out) return fail(ctx, VX_EINVAL, "cfl: no bound pending");
     int rc = check_hip(ctx, hipEventSynchronize(ctx->cfl_ev), "cfl wait");

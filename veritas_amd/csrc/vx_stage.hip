@@ -443,7 +443,8 @@ void launch_stage(vx_ctx *ctx, StageArgs &A, bool mp) {
 // strips (less warm-up).  The first call for a shape times the candidates on
 // the call's own launch, which is idempotent (it reads f_s, f^n and the fluxes
 // of earlier stages and writes only f_(s+1) and this stage's fluxes, the same
-// values for every strip height), and keeps the fastest.  While a step graph
+// values for every strip height), and keeps the fastest (a shorter strip must
+// beat the taller ones by more than 3 %).  While a step graph
 // is being recorded the choice falls back to the shortest strip that still
 // gives 2048 wavefronts.
 int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
@@ -473,7 +474,9 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
             if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
             float ms = 0.0f;
             if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
-            if (ms < best_ms) {
+            // candidates run from tall to short strips: a shorter strip must be
+            // more than 3 % faster to win (ties go to fewer warm-up rows)
+            if (ms < 0.97f * best_ms) {
                 best_ms = ms;
                 best = ti;
             }
