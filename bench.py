@@ -251,7 +251,7 @@ def main():
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 WENO weights and Zalesak ratios, compensated FP64 moments "
                          "(reports its error against the FP64 path at 25/50/100%% of --error-steps)")
-    ap.add_argument("--error-steps", type=int, default=500)
+    ap.add_argument("--error-steps", type=int, default=500, help="mixed: run length of the error report (0: none)")
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--np", type=int, default=4096)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -324,7 +324,7 @@ def main():
     run.close()
     if mixed:
         out["config"]["precision"] = "FP32 WENO weights and Zalesak ratios; FP64 fluxes, updates, compensated moments, fields"
-        out["error_vs_fp64"] = mixed_error(va, params, a.error_steps)
+        out["error_vs_fp64"] = mixed_error(va, params, a.error_steps) if a.error_steps > 0 else None
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(params, a.cpu_steps)
     else:

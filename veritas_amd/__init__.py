@@ -62,7 +62,7 @@ def host_lib():
         L.vxh_set_graphs.argtypes = [vp, i]
         L.vxh_set_mixed.argtypes = [vp, i]
         L.vxh_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
-        L.vxh_tile_rows.argtypes = [vp, i, i]
+        L.vxh_tile_rows.argtypes = [vp, i, i, i]
         L.vxh_server_create.restype = vp
         L.vxh_server_create.argtypes = [i, d, i]
         L.vxh_server_destroy.argtypes = [vp]
@@ -145,11 +145,12 @@ class Run:
         self._ck(host_lib().vxh_set_graphs(self.h, int(bool(on))))
 
     def tile_rows(self, n_x=None, n_p=None):
-        """Rows per strip the fused stage uses for an n_x x n_p patch (0 until
-        its first launch times the candidates)."""
+        """Rows per strip the fused FP64 stage uses at each of the six RK stages
+        for an n_x x n_p patch (0 until that stage's first launch times the
+        candidates)."""
         n_x = self.params.nx if n_x is None else n_x
         n_p = self.params.np if n_p is None else n_p
-        return host_lib().vxh_tile_rows(self.h, n_x, n_p)
+        return [host_lib().vxh_tile_rows(self.h, n_x, n_p, s) for s in range(6)]
 
     def graph_stats(self):
         out = (ctypes.c_int * 2)()

@@ -24,8 +24,9 @@ struct vx_ctx {
     // p-moment logarithm table of the patch being reduced (vx_moments_patch)
     double *mom_table = nullptr;
     size_t mom_table_n = 0;
-    // strip heights of the fused stage chosen per patch shape: {n_x, n_p, rows, mixed}
-    std::vector<std::array<int, 4>> ti_cache;
+    // strip heights of the fused stage chosen per patch shape and stage:
+    // {n_x, n_p, rows, mixed, stage}
+    std::vector<std::array<int, 5>> ti_cache;
     // device bytes held through vx_malloc (vx_ctx_live_bytes)
     std::unordered_map<void *, size_t> allocs;
     size_t live_bytes = 0;

@@ -33,6 +33,7 @@
 // FP32 weight w, so its error is of order 1e-7 |fL - fR|, not 1e-7 |f|.
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -437,7 +438,8 @@ void launch_stage(vx_ctx *ctx, StageArgs &A, bool mp) {
     else launch_stage_t<false>(ctx, A, grid);
 }
 
-// Rows per strip (the tile height) of a patch shape.  A strip costs its rows
+// Rows per strip (the tile height) of a patch shape and RK stage (the later
+// stages hold more registers and read more planes).  A strip costs its rows
 // plus 6 pipeline warm-up rows, and a wavefront walks them one after another,
 // so small patches want short strips (more wavefronts) and large ones long
 // strips (less warm-up).  The first call for a shape times the candidates on
@@ -450,8 +452,23 @@ void launch_stage(vx_ctx *ctx, StageArgs &A, bool mp) {
 int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
     const int nx = A0.p.n_x, np = A0.p.n_p;
     for (auto &c : ctx->ti_cache)
-        if (c[0] == nx && c[1] == np && c[3] == (int)mp) return c[2];
+        if (c[0] == nx && c[1] == np && c[3] == (int)mp && c[4] == A0.step) return c[2];
     static const int cand[] = {64, 32, 16, 8, 4, 2};
+    // VX_TILE_ROWS pins the choice (one value, or six comma-separated values,
+    // one per RK stage), e.g. to profile the configuration a timed run chose
+    if (const char *env = std::getenv("VX_TILE_ROWS")) {
+        int v[6], n = 0;
+        for (const char *q = env; *q && n < 6;) {
+            v[n++] = std::atoi(q);
+            while (*q && *q != ',') q++;
+            if (*q == ',') q++;
+        }
+        const int ti = n == 6 ? v[A0.step] : (n >= 1 ? v[0] : 0);
+        if (ti >= 1 && ti <= 1024) {
+            ctx->ti_cache.push_back({nx, np, ti, (int)mp, A0.step});
+            return ti;
+        }
+    }
     int guess = 2;
     for (int ti : cand)
         if ((long)A0.nbx * ((nx + ti - 1) / ti) >= 2048) {
@@ -484,7 +501,7 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    ctx->ti_cache.push_back({nx, np, best, (int)mp});
+    ctx->ti_cache.push_back({nx, np, best, (int)mp, A0.step});
     return best;
 }
 
@@ -651,9 +668,9 @@ int vx_fct_stage_fused_batch(vx_ctx *ctx, const vx_stage_item *items, int n, int
     return rc ? rc : batch_launch(ctx, mixed, step, true);
 }
 
-int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p) {
+int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step) {
     for (auto &c : ctx->ti_cache)
-        if (c[0] == n_x && c[1] == n_p && c[3] == 0) return c[2];
+        if (c[0] == n_x && c[1] == n_p && c[3] == 0 && c[4] == step) return c[2];
     return 0;
 }
 
