@@ -89,4 +89,20 @@ __device__ __forceinline__ double face_weno(double f1, double f2, double f3, dou
     return wL * fL + wR * fR;
 }
 
+// Whole-wavefront lane shifts by one lane (DPP wave_shl:1 / wave_shr:1, plain
+// VALU moves instead of LDS permutes): shdn gives lane l the value of lane
+// l + 1, shup the value of lane l - 1.  The end lane (63, resp. 0) receives 0;
+// it is a halo lane, as are its neighbours that consume it.
+template <int CTRL>
+__device__ __forceinline__ double wave_shift1(double v) {
+    const int lo = __double2loint(v), hi = __double2hiint(v);
+    const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+    const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
+    return __hiloint2double(rhi, rlo);
+}
+constexpr int DPP_WAVE_SHL1 = 0x130;
+constexpr int DPP_WAVE_SHR1 = 0x138;
+__device__ __forceinline__ double shdn1(double v) { return wave_shift1<DPP_WAVE_SHL1>(v); }
+__device__ __forceinline__ double shup1(double v) { return wave_shift1<DPP_WAVE_SHR1>(v); }
+
 }  // namespace vxd

@@ -16,9 +16,6 @@ using namespace vxd;
 
 namespace {
 
-constexpr int TI = 16;  // x rows per tile
-constexpr int TJ = 64;  // p columns per tile (one wavefront)
-
 struct Sub0Args {
     vx_patch p;
     vx_fields fl;
@@ -27,11 +24,6 @@ struct Sub0Args {
     double a[6];     // RK coefficients times dt (Rectangle.cpp:1396-1517)
     double aSum;     // a[0] + ... + a[ns-1], summed left to right
 };
-
-__device__ __forceinline__ double load_or0(const double *pl, int pitch, int nx, int np, int i,
-                                           int j) {
-    return (i >= -2 && i < nx + 2 && j >= -2 && j < np + 2) ? pl[pidx(pitch, i, j)] : 0.0;
-}
 
 // EMFieldSolver::GetASquared(GetFinestIndex(i)) scaled by q*q (Rectangle.cpp:1280)
 __device__ __forceinline__ double face_as(const Sub0Args &A, int i) {
@@ -46,109 +38,134 @@ __device__ __forceinline__ double patch_E(const vx_patch &p, const vx_fields &fl
     return vxd::patch_E(fl.E, fl.n_f, p.rtb, p.x_pos, i);
 }
 
-// Sub-step 0: fluxes of stage `step`, RK combination.  One 256-thread block
-// covers NS positions i in [i0, i0+16), j in [j0, j0+64).
-__global__ __launch_bounds__(256) void k_sub0(Sub0Args A) {
+// Sub-step 0 fluxes (Rectangle.cpp:1269-1517 up to the predictor) in
+// register-streaming form, for a patch with any edges: a wavefront owns 64
+// consecutive p positions, lane l <-> j = jw - 3 + l, and produces the 58
+// positions of lanes 3..60 (p neighbours up to 2 lanes away come from DPP lane
+// shifts); it marches along x over a strip of TI positions, position ix taking
+// rows ix-2 .. ix+1 of f_s, so a strip starts 3 rows early.  Every position
+// (i, j) in [-2, n_x+2) x [-2, n_p+2) has one owner, which writes FxDS/FpDS
+// there, and FxH_s/FpH_s (FxL0/FpL0 at stage 0) where the reference's loops
+// compute them; later stages read the stored FxL0/FpL0 (the coarse-fine flux
+// program may have replaced them at interface faces).
+constexpr int S0W = 58;  // positions per wavefront
+
+struct Sub0Tiles {
+    int TI, nstrip, nbx;
+};
+
+template <int STEP>
+__global__ __launch_bounds__(256) void k_sub0s(Sub0Args A, Sub0Tiles T) {
     const vx_patch &P = A.p;
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
-    const int i0 = -2 + (int)blockIdx.y * TI;
-    const int j0 = -2 + (int)blockIdx.x * TJ;
-    const int tid = threadIdx.x;
-
-    __shared__ double F1[TI + 3][TJ + 4];  // f slot 1, i in [i0-2, i0+17), j in [j0-2, j0+66)
-    __shared__ double G[TI + 3][TJ + 4];   // Gamma(p_j, as_i), i in [i0-1, i0+18), j in [j0-1, j0+67)
-    __shared__ double EX[TI][TJ + 2], FX[TI][TJ + 2];          // x-faces, j in [j0-1, j0+65)
-    __shared__ double EP[TI + 2][TJ], FP[TI + 2][TJ];          // p-faces, i in [i0-1, i0+17)
-    __shared__ double AS[TI + 3];  // q^2 A^2 at x-faces i0-1 .. i0+17
-    __shared__ double EM[TI + 2];  // q E at columns i0-1 .. i0+16
-
-    const double *f1 = P.f[1];
-    for (int e = tid; e < (TI + 3) * (TJ + 3); e += 256) {
-        int r = e / (TJ + 3), c = e % (TJ + 3);
-        F1[r][c] = load_or0(f1, pitch, nx, np, i0 - 2 + r, j0 - 2 + c);
-    }
-    if (tid < TI + 3) AS[tid] = face_as(A, i0 - 1 + tid);
-    if (tid >= 64 && tid < 64 + TI + 2) {
-        int r = tid - 64, i = i0 - 1 + r;
-        EM[r] = (i >= -1 && i < nx + 1) ? P.q * patch_E(P, A.fl, i) : 0.0;  // columns ep reads
-    }
-    __syncthreads();
-    for (int e = tid; e < (TI + 3) * (TJ + 3); e += 256) {
-        int r = e / (TJ + 3), c = e % (TJ + 3);
-        double pj = momentum(P.pmin, P.dp, j0 - 1 + c, P.p_pos);
-        G[r][c] = gamma_(pj, AS[r], P.mc2inv);
-    }
-    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int bx = (int)blockIdx.x % T.nbx, by = (int)blockIdx.x / T.nbx;
+    const int strip = by * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (strip >= T.nstrip) return;  // uniform per wavefront
+    const int i0 = -2 + strip * T.TI;
+    const int i1 = min(i0 + T.TI, nx + 2);  // positions [i0, i1)
+    const int jw = -2 + bx * S0W;
+    const int j = jw - 3 + lane;
+    const bool own_j = lane >= 3 && lane < 3 + S0W && j < np + 2;
+    const bool in_j = j >= -2 && j < np + 2;
+    const bool jr_ex = j >= -1 && j < np + 1;
+    const bool jr_ep = j >= -1 && j < np + 2;
+    const bool jr_fp = j >= 0 && j < np + 1;
     const double dpcc = P.dp_inv * P.cc;
     const double ccdx = P.cc * P.dx_inv;
-    // x-face velocities and WENO values (Rectangle.cpp:1278-1293)
-    for (int e = tid; e < TI * (TJ + 2); e += 256) {
-        int r = e / (TJ + 2), c = e % (TJ + 2);
-        int i = i0 + r, j = j0 - 1 + c;
-        double ex = 0.0, fx = 0.0;
-        if (i >= -1 && i < nx + 2 && j >= -1 && j < np + 1)
-            ex = dpcc * (G[r + 1][c + 1] - G[r + 1][c]);
-        if (i >= 0 && i < nx + 1 && j >= -1 && j < np + 1)
-            fx = weno(F1[r][c + 1], F1[r + 1][c + 1], F1[r + 2][c + 1], F1[r + 3][c + 1], ex > 0.0);
-        EX[r][c] = ex;
-        FX[r][c] = fx;
-    }
-    // p-face forces and WENO values (Rectangle.cpp:1294-1312)
-    for (int e = tid; e < (TI + 2) * TJ; e += 256) {
-        int r = e / TJ, c = e % TJ;
-        int i = i0 - 1 + r, j = j0 + c;
-        double ep = 0.0, fp = 0.0;
-        if (i >= -1 && i < nx + 1 && j >= -1 && j < np + 2)
-            ep = EM[r] - ccdx * (G[r + 1][c + 1] - G[r][c + 1]);
-        if (i >= -1 && i < nx + 1 && j >= 0 && j < np + 1)
-            fp = weno(F1[r + 1][c], F1[r + 1][c + 1], F1[r + 1][c + 2], F1[r + 1][c + 3], ep > 0.0);
-        EP[r][c] = ep;
-        FP[r][c] = fp;
-    }
-    __syncthreads();
-
     const double w3 = 1 / 48.0;
-    const int c = tid & 63;
-    const int j = j0 + c;
-    for (int rr = tid >> 6; rr < TI; rr += 4) {
-        const int i = i0 + rr;
-        if (i >= nx + 2 || j >= np + 2) continue;
-        const long id = pidx(pitch, i, j);
-        // high- and low-order x flux (Rectangle.cpp:1313-1352)
-        double fxh = 0.0, fxl = 0.0;
-        if (i >= 0 && i < nx + 1 && j >= -1 && j < np + 1) {
-            double am = EX[rr][c + 1], ap1 = EX[rr][c + 2], am1 = EX[rr][c];
-            double fm = FX[rr][c + 1], fp1 = FX[rr][c + 2], fm1 = FX[rr][c];
-            fxh = P.dx_inv * (fm * am + w3 * (fp1 - fm1) * (ap1 - am1));
-            fxl = P.dx_inv * ((am > 0.0 ? F1[rr + 1][c + 2] : F1[rr + 2][c + 2]) * am);
-            P.FxH[A.step][id] = fxh;
-            if (A.step == 0) P.FxL0[id] = fxl;
+    const double pj = momentum(P.pmin, P.dp, j, P.p_pos);
+    const double *f1 = P.f[1];
+    auto ld = [&](int i) { return (i >= -2 && i < nx + 2 && in_j) ? f1[pidx(pitch, i, j)] : 0.0; };
+
+    double fs_m3 = 0, fs_m2 = 0, fs_m1 = 0, fs_0 = 0;
+    double ep_m2 = 0, ep_m1 = 0, ep_0 = 0, fp_m2 = 0, fp_m1 = 0, fp_0 = 0;
+    const int r0 = i0 - 2;
+    double as_m1 = face_as(A, r0 - 1), as_0 = face_as(A, r0);
+    double G_m1 = gamma_(pj, as_m1, P.mc2inv);
+    double G_0 = (as_0 == as_m1) ? G_m1 : gamma_(pj, as_0, P.mc2inv);
+    double pfs_a = ld(r0), pfs_b = ld(r0 + 1);
+    constexpr int NH = STEP > 0 ? STEP : 1;
+    double hx[NH], hp[NH], lx = 0.0, lp = 0.0;
+    auto load_h = [&](int pos) {  // stored planes at position (pos, j)
+        const bool v = pos >= -2 && pos < nx + 2 && in_j;
+#pragma unroll
+        for (int k = 0; k < NH; k++) {
+            hx[k] = (k < STEP && v) ? P.FxH[k][pidx(pitch, pos, j)] : 0.0;
+            hp[k] = (k < STEP && v) ? P.FpH[k][pidx(pitch, pos, j)] : 0.0;
         }
-        // high- and low-order p flux (Rectangle.cpp:1353-1394)
-        double fph = 0.0, fpl = 0.0;
-        if (i >= -1 && i < nx + 1 && j >= 0 && j < np + 1) {
-            double am = EP[rr + 1][c], ap1 = EP[rr + 2][c], am1 = EP[rr][c];
-            double fm = FP[rr + 1][c], fp1 = FP[rr + 2][c], fm1 = FP[rr][c];
-            fph = P.dp_inv * (fm * am + w3 * (fp1 - fm1) * (ap1 - am1));
-            fpl = P.dp_inv * ((am > 0.0 ? F1[rr + 2][c + 1] : F1[rr + 2][c + 2]) * am);
-            P.FpH[A.step][id] = fph;
-            if (A.step == 0) P.FpL0[id] = fpl;
+        if (STEP > 0) {
+            lx = v ? P.FxL0[pidx(pitch, pos, j)] : 0.0;
+            lp = v ? P.FpL0[pidx(pitch, pos, j)] : 0.0;
         }
-        // RK flux combination (Rectangle.cpp:1396-1517), all NS positions
-        double lx = A.step == 0 ? fxl : P.FxL0[id];
-        double lp = A.step == 0 ? fpl : P.FpL0[id];
-        double xLS = A.aSum * lx;
-        double pLS = A.aSum * lp;
-        double sx = A.a[0] * (A.step == 0 ? fxh : P.FxH[0][id]);
-        double sp = A.a[0] * (A.step == 0 ? fph : P.FpH[0][id]);
-        for (int k = 1; k < A.ns; k++) {
-            double hx = (k == A.step) ? fxh : P.FxH[k][id];
-            double hp = (k == A.step) ? fph : P.FpH[k][id];
-            sx = sx + A.a[k] * hx;
-            sp = sp + A.a[k] * hp;
+    };
+    load_h(r0 - 1);
+
+    for (int r = r0; r <= i1; r++) {
+        fs_0 = pfs_a;
+        pfs_a = pfs_b;
+        pfs_b = ld(r + 2);
+        double chx[NH], chp[NH];
+#pragma unroll
+        for (int k = 0; k < NH; k++) {
+            chx[k] = hx[k];
+            chp[k] = hp[k];
         }
-        P.FxDS[id] = sx - xLS;
-        P.FpDS[id] = sp - pLS;
+        const double clx = lx, clp = lp;
+        load_h(r);
+        const double as_p1 = face_as(A, r + 1);
+        const double G_p1 = (as_p1 == as_0) ? G_0 : gamma_(pj, as_p1, P.mc2inv);
+
+        // x-face ix = r - 1 (Rectangle.cpp:1278-1293, 1313-1352)
+        const int ix = r - 1;
+        const double Gd = shdn1(G_m1);
+        const double ex = (ix >= -1 && ix < nx + 2 && jr_ex) ? dpcc * (Gd - G_m1) : 0.0;
+        const bool vFx = ix >= 0 && ix < nx + 1 && jr_ex;
+        const double fx = vFx ? weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0) : 0.0;
+        const double ex_u = shdn1(ex), ex_d = shup1(ex), fx_u = shdn1(fx), fx_d = shup1(fx);
+        const double fxh = vFx ? P.dx_inv * (fx * ex + w3 * (fx_u - fx_d) * (ex_u - ex_d)) : 0.0;
+        const double fxl = vFx ? P.dx_inv * ((ex > 0.0 ? fs_m2 : fs_m1) * ex) : 0.0;
+
+        // column r (Rectangle.cpp:1294-1312)
+        const bool cr = r >= -1 && r < nx + 1;
+        const double qE = cr ? P.q * patch_E(P, A.fl, r) : 0.0;
+        ep_0 = (cr && jr_ep) ? qE - ccdx * (G_p1 - G_0) : 0.0;
+        const double s1 = shup1(fs_0), s2 = shup1(s1), s3 = shdn1(fs_0);
+        fp_0 = (cr && jr_fp) ? weno(s2, s1, fs_0, s3, ep_0 > 0.0) : 0.0;
+
+        // column ix = r - 1 (Rectangle.cpp:1353-1394)
+        const bool vFp = ix >= -1 && ix < nx + 1 && jr_fp;
+        const double fph = vFp ? P.dp_inv * (fp_m1 * ep_m1 + w3 * (fp_0 - fp_m2) * (ep_0 - ep_m2)) : 0.0;
+        const double fsd = shup1(fs_m1);  // f(ix, j-1)
+        const double fpl = vFp ? P.dp_inv * ((ep_m1 > 0.0 ? fsd : fs_m1) * ep_m1) : 0.0;
+
+        // position (ix, j): stores and the RK combination (Rectangle.cpp:1396-1517)
+        if (ix >= i0 && ix < i1 && own_j) {
+            const long id = pidx(pitch, ix, j);
+            if (vFx) {
+                P.FxH[STEP][id] = fxh;
+                if (STEP == 0) P.FxL0[id] = fxl;
+            }
+            if (vFp) {
+                P.FpH[STEP][id] = fph;
+                if (STEP == 0) P.FpL0[id] = fpl;
+            }
+            const double xLS = A.aSum * (STEP == 0 ? fxl : clx);
+            const double pLS = A.aSum * (STEP == 0 ? fpl : clp);
+            double sx = A.a[0] * (STEP == 0 ? fxh : chx[0]);
+            double sp = A.a[0] * (STEP == 0 ? fph : chp[0]);
+#pragma unroll
+            for (int k = 1; k <= STEP; k++) {
+                sx = sx + A.a[k] * (k == STEP ? fxh : chx[k < NH ? k : 0]);
+                sp = sp + A.a[k] * (k == STEP ? fph : chp[k < NH ? k : 0]);
+            }
+            P.FxDS[id] = sx - xLS;
+            P.FpDS[id] = sp - pLS;
+        }
+
+        fs_m3 = fs_m2; fs_m2 = fs_m1; fs_m1 = fs_0;
+        as_0 = as_p1; G_m1 = G_0; G_0 = G_p1;
+        ep_m2 = ep_m1; ep_m1 = ep_0; fp_m2 = fp_m1; fp_m1 = fp_0;
     }
 }
 
@@ -306,10 +323,27 @@ int vx_fct_sub0_fluxes(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     A.fl = *fl;
     A.step = step;
     rk_coefficients(step, dt, A.a, A.ns, A.aSum);
-    dim3 grid((unsigned)((p->n_p + 4 + TJ - 1) / TJ), (unsigned)((p->n_x + 4 + TI - 1) / TI));
+    // strips of the streaming kernel: the tallest that still gives 2048 wavefronts
+    Sub0Tiles T;
+    T.nbx = (p->n_p + 4 + S0W - 1) / S0W;
+    T.TI = 4;
+    for (int ti : {64, 32, 16, 8})
+        if ((long)T.nbx * ((p->n_x + 4 + ti - 1) / ti) >= 2048) {
+            T.TI = ti;
+            break;
+        }
+    T.nstrip = (p->n_x + 4 + T.TI - 1) / T.TI;
+    dim3 grid((unsigned)(T.nbx * ((T.nstrip + 3) / 4)));
     {
         LaunchTimer lt(ctx, "k_sub0");
-        k_sub0<<<grid, 256, 0, ctx->stream>>>(A);
+        switch (step) {
+            case 0: k_sub0s<0><<<grid, 256, 0, ctx->stream>>>(A, T); break;
+            case 1: k_sub0s<1><<<grid, 256, 0, ctx->stream>>>(A, T); break;
+            case 2: k_sub0s<2><<<grid, 256, 0, ctx->stream>>>(A, T); break;
+            case 3: k_sub0s<3><<<grid, 256, 0, ctx->stream>>>(A, T); break;
+            case 4: k_sub0s<4><<<grid, 256, 0, ctx->stream>>>(A, T); break;
+            default: k_sub0s<5><<<grid, 256, 0, ctx->stream>>>(A, T); break;
+        }
     }
     return check_launch(ctx, "k_sub0");
 }

@@ -104,22 +104,6 @@ __device__ __forceinline__ float shup1f(float v) {
     return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
 }
 
-// Whole-wavefront lane shifts by one lane (DPP wave_shl:1 / wave_shr:1, plain
-// VALU moves instead of LDS permutes): shdn gives lane l the value of lane
-// l + 1, shup the value of lane l - 1.  The end lane (63, resp. 0) receives 0;
-// it is a halo lane, as are its neighbours that consume it.
-template <int CTRL>
-__device__ __forceinline__ double wave_shift1(double v) {
-    const int lo = __double2loint(v), hi = __double2hiint(v);
-    const int rlo = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
-    const int rhi = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
-    return __hiloint2double(rhi, rlo);
-}
-constexpr int DPP_WAVE_SHL1 = 0x130;
-constexpr int DPP_WAVE_SHR1 = 0x138;
-__device__ __forceinline__ double shdn1(double v) { return wave_shift1<DPP_WAVE_SHL1>(v); }
-__device__ __forceinline__ double shup1(double v) { return wave_shift1<DPP_WAVE_SHR1>(v); }
-
 __device__ __forceinline__ double ldp(const double *pl, int pitch, int nx, int np, int i, int j) {
     return (i >= -2 && i < nx + 2 && j >= -2 && j < np + 2) ? pl[pidx(pitch, i, j)] : 0.0;
 }
