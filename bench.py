@@ -54,6 +54,22 @@ def measured_traffic(kernel, nx, np_):
     return None
 
 
+VALU_PEAK = 1024 * 2.4e9 / 4  # wave64 FP64 VALU instructions/s: 256 CUs x 4 SIMDs, 4 cycles each
+
+
+def measured_valu(kernel, nx, np_, mixed):
+    """VALU wave-instructions per launch of `kernel` from the committed
+    rocprofv3 SQ_INSTS_VALU summary of this workload, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_valu_two_stream_{nx}.md")
+    key = "valu_instructions_per_launch_mixed:" if mixed else "valu_instructions_per_launch:"
+    if nx != np_ or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith(key):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(params, steps):
     """Time the CPU oracle (test infrastructure, used here only as the
     baseline) on `steps` steps of the same workload."""
@@ -321,6 +337,13 @@ def main():
                      "avg_launch_ms": kms / nlaunch if nlaunch else None,
                      "launches": nlaunch, "kernel_share": kms / ms if ms else None},
     }
+    valu = measured_valu(kernel, params.nx, params.np, mixed)
+    if valu and avg_launch_s == avg_launch_s:
+        # the bound that binds this kernel: FP64 VALU issue (profiles/k_stage_valu_*.md)
+        out["valu_roofline"] = {"bound": "valu-issue", "achieved": valu / avg_launch_s / 1e9,
+                                "peak": VALU_PEAK / 1e9, "unit": "G wave-instructions/s",
+                                "frac": valu / avg_launch_s / VALU_PEAK,
+                                "instructions_per_launch": valu}
     run.close()
     if mixed:
         out["config"]["precision"] = "FP32 WENO weights and Zalesak ratios; FP64 fluxes, updates, compensated moments, fields"
