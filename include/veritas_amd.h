@@ -295,6 +295,19 @@ int vx_program_run(vx_ctx *ctx, const vx_patch_table *table, const vx_program *p
 int vx_program_run_fluxes(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog,
                           int step, double dt, int r, const vx_fields *fl);
 
+/* Work-list forms of the sub-step kernels (no counterpart in the reference,
+ * which loops Level::FCTTimeStep over its rectangles, Level.cpp:12-17): one
+ * launch covers every patch of a table, each workgroup finding its patch in a
+ * per-table list built on the first call (a table's shapes never change).
+ * Each patch's results are bit-identical to its own launch: within a sub-step
+ * the patches only write their own planes.  The table must hold the current
+ * plane pointers (vx_patch_table_update). */
+int vx_fct_sub0_fluxes_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const vx_fields *fl);
+int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, int32_t n_f);
+int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
+int vx_fct_sub2_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
+int vx_fct_sub3_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
+
 /* Split form of vx_fct_sub0 for hierarchies: the fluxes and RK combination
  * (Rectangle.cpp:1269-1517), then the low-order predictor (1518-1534). */
 int vx_fct_sub0_fluxes(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl);

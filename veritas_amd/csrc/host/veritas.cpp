@@ -357,13 +357,10 @@ void Mesh::Sub0(double timeStep, int step) {
         for (auto &r : l->rectangles) r->UnaliasSlots();
     EnsureTable();
     const vx_fields &fl = EMSolver->fields();
-    for (int i = (int)levels.size() - 1; i > -1; i--)
-        for (auto &r : levels[i]->rectangles)
-            dev->check(vx_fct_sub0_fluxes(dev->ctx, &r->desc(), step, timeStep, &fl), "fct sub0 fluxes");
+    // every patch's fluxes in one launch (patches write only their own planes)
+    dev->check(vx_fct_sub0_fluxes_table(dev->ctx, table, step, timeStep, &fl), "fct sub0 fluxes");
     for (unsigned i = 1; i < levels.size(); i++) levels[i]->RunFluxProgram(step, timeStep, fl);
-    for (int i = (int)levels.size() - 1; i > -1; i--)
-        for (auto &r : levels[i]->rectangles)
-            dev->check(vx_fct_sub0_predictor(dev->ctx, &r->desc(), step, timeStep), "fct sub0 predictor");
+    dev->check(vx_fct_sub0_predictor_table(dev->ctx, table, step, timeStep, fl.n_f), "fct sub0 predictor");
 }
 
 void Mesh::EnsureTable() {
@@ -384,14 +381,20 @@ void Mesh::Advance(double timeStep, int step) {
         single->FCTStageFused(timeStep, step, mixed);
         return;
     }
+    // each sub-step is one launch over the mesh's patch table (Level::FCTTimeStep
+    // over every level, Mesh.cpp:64-89; the patches of a sub-step are independent)
+    const int n_f = EMSolver->fields().n_f;
     Sub0(timeStep, step);
     PushData(2);
-    for (int i = (int)levels.size() - 1; i > -1; i--) levels[i]->FCTTimeStep(timeStep, step, 1);
+    EnsureTable();
+    dev->check(vx_fct_sub1_table(dev->ctx, table, n_f), "fct sub1");
     PushBoundaryC();
-    for (int i = (int)levels.size() - 1; i > -1; i--) levels[i]->FCTTimeStep(timeStep, step, 2);
+    dev->check(vx_fct_sub2_table(dev->ctx, table, n_f), "fct sub2");
     PushData(1);
-    if (step == 5)
-        for (int i = (int)levels.size() - 1; i > -1; i--) levels[i]->FCTTimeStep(timeStep, step, 3);
+    if (step == 5) {
+        EnsureTable();
+        dev->check(vx_fct_sub3_table(dev->ctx, table, n_f), "fct sub3");
+    }
 }
 
 // Mesh.cpp:91-106

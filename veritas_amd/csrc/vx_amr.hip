@@ -18,13 +18,6 @@
 using namespace vxi;
 using namespace vxd;
 
-struct vx_patch_table {
-    vx_ctx *ctx = nullptr;
-    vx_patch *d = nullptr;
-    int32_t n = 0;
-    uint64_t serial = 0;          // identity for the program bounds check
-    std::vector<vx_patch> host;   // shapes, for the bounds check
-};
 
 struct vx_program {
     vx_ctx *ctx = nullptr;
@@ -425,6 +418,7 @@ int vx_patch_table_update(vx_ctx *ctx, vx_patch_table *t, const vx_patch *patche
 void vx_patch_table_destroy(vx_patch_table *t) {
     if (!t) return;
     if (t->d) (void)hipFree(t->d);
+    if (t->batch_d) (void)hipFree(t->batch_d);
     delete t;
 }
 

@@ -16,8 +16,8 @@ using namespace vxd;
 
 namespace {
 
-struct Sub0Args {
-    vx_patch p;
+// values of one sub-step-0 launch that every patch shares
+struct Sub0Coef {
     vx_fields fl;
     int step;
     int ns;          // step + 1 terms in the RK combination
@@ -26,10 +26,10 @@ struct Sub0Args {
 };
 
 // EMFieldSolver::GetASquared(GetFinestIndex(i)) scaled by q*q (Rectangle.cpp:1280)
-__device__ __forceinline__ double face_as(const Sub0Args &A, int i) {
-    long fi = (long)A.p.rtb * (long)(i + A.p.x_pos);
-    long c = fi < 0 ? 0 : (fi > A.fl.n_f ? A.fl.n_f : fi);
-    return A.p.q * A.p.q * A.fl.a2[c];
+__device__ __forceinline__ double face_as(const vx_patch &P, const vx_fields &fl, int i) {
+    long fi = (long)P.rtb * (long)(i + P.x_pos);
+    long c = fi < 0 ? 0 : (fi > fl.n_f ? fl.n_f : fi);
+    return P.q * P.q * fl.a2[c];
 }
 
 // Rectangle::GetEfield(i) (Rectangle.cpp:1055-1067); E is periodic on the
@@ -55,11 +55,11 @@ struct Sub0Tiles {
 };
 
 template <int STEP>
-__global__ __launch_bounds__(256) void k_sub0s(Sub0Args A, Sub0Tiles T) {
-    const vx_patch &P = A.p;
+__device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A, const Sub0Tiles &T,
+                                           const int tile) {
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
     const int lane = threadIdx.x & 63;
-    const int bx = (int)blockIdx.x % T.nbx, by = (int)blockIdx.x / T.nbx;
+    const int bx = tile % T.nbx, by = tile / T.nbx;
     const int strip = by * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     if (strip >= T.nstrip) return;  // uniform per wavefront
     const int i0 = -2 + strip * T.TI;
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(256) void k_sub0s(Sub0Args A, Sub0Tiles T) {
     double fs_m3 = 0, fs_m2 = 0, fs_m1 = 0, fs_0 = 0;
     double ep_m2 = 0, ep_m1 = 0, ep_0 = 0, fp_m2 = 0, fp_m1 = 0, fp_0 = 0;
     const int r0 = i0 - 2;
-    double as_m1 = face_as(A, r0 - 1), as_0 = face_as(A, r0);
+    double as_m1 = face_as(P, A.fl, r0 - 1), as_0 = face_as(P, A.fl, r0);
     double G_m1 = gamma_(pj, as_m1, P.mc2inv);
     double G_0 = (as_0 == as_m1) ? G_m1 : gamma_(pj, as_0, P.mc2inv);
     double pfs_a = ld(r0), pfs_b = ld(r0 + 1);
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(256) void k_sub0s(Sub0Args A, Sub0Tiles T) {
         }
         const double clx = lx, clp = lp;
         load_h(r);
-        const double as_p1 = face_as(A, r + 1);
+        const double as_p1 = face_as(P, A.fl, r + 1);
         const double G_p1 = (as_p1 == as_0) ? G_0 : gamma_(pj, as_p1, P.mc2inv);
 
         // x-face ix = r - 1 (Rectangle.cpp:1278-1293, 1313-1352)
@@ -169,6 +169,33 @@ __global__ __launch_bounds__(256) void k_sub0s(Sub0Args A, Sub0Tiles T) {
     }
 }
 
+template <int STEP>
+__global__ __launch_bounds__(256) void k_sub0s(vx_patch P, Sub0Coef A, Sub0Tiles T) {
+    sub0s_tile<STEP>(P, A, T, (int)blockIdx.x);
+}
+
+// The first patch k of the work list whose workgroups start at or before w
+// (uniform per workgroup).
+template <int KIND>
+__device__ __forceinline__ int batch_patch(const vx_batch_info *bi, int n, int w) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (bi[mid].start[KIND] <= w) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int STEP>
+__global__ __launch_bounds__(256) void k_sub0s_batch(const vx_patch *tab, const vx_batch_info *bi, int n,
+                                                     Sub0Coef A) {
+    const int w = (int)blockIdx.x;
+    const int k = batch_patch<0>(bi, n, w);
+    const Sub0Tiles T{bi[k].TI, bi[k].nstrip, bi[k].nbx};
+    sub0s_tile<STEP>(tab[k], A, T, w - bi[k].start[0]);
+}
+
 // Low-order predictor f2 on interior cells (Rectangle.cpp:1518-1534) in
 // gather form.  For cell (i, j) the serial scatter loop touches it in the
 // order: iteration (i,j) adds FxLS(i,j) then FpLS(i,j); iteration (i,j+1)
@@ -188,9 +215,9 @@ __device__ __forceinline__ LoopRange loop_range(const vx_patch &P) {
     return L;
 }
 
-__global__ __launch_bounds__(256) void k_lo_predictor(vx_patch P, double aSum) {
-    int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void lo_predictor(const vx_patch &P, double aSum, int bx, int by) {
+    int j = bx * 64 + (threadIdx.x & 63);
+    int i = by * 4 + (threadIdx.x >> 6);
     if (i >= P.n_x || j >= P.n_p) return;
     LoopRange L = loop_range(P);
     const int pitch = P.pitch;
@@ -205,9 +232,9 @@ __global__ __launch_bounds__(256) void k_lo_predictor(vx_patch P, double aSum) {
 }
 
 // Zalesak R+- for i in [-1, n_x+1), j in [-1, n_p+1) (Rectangle.cpp:1538-1579)
-__global__ __launch_bounds__(256) void k_limiter_r(vx_patch P) {
-    int j = -1 + (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
-    int i = -1 + (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+__device__ __forceinline__ void limiter_r(const vx_patch &P, int bx, int by) {
+    int j = -1 + bx * 64 + (int)(threadIdx.x & 63);
+    int i = -1 + by * 4 + (int)(threadIdx.x >> 6);
     if (i >= P.n_x + 1 || j >= P.n_p + 1) return;
     const int pitch = P.pitch;
     const double *FxDS = P.FxDS, *FpDS = P.FpDS, *f0 = P.f[0], *f2 = P.f[2];
@@ -233,9 +260,9 @@ __global__ __launch_bounds__(256) void k_limiter_r(vx_patch P) {
 
 // C on the NS range: 1 everywhere, the limiter formula on interior faces
 // i in [1, n_x), j in [0, n_p) (Rectangle.cpp:1581-1594)
-__global__ __launch_bounds__(256) void k_limiter_c(vx_patch P) {
-    int j = -2 + (int)blockIdx.x * 64 + (int)(threadIdx.x & 63);
-    int i = -2 + (int)blockIdx.y * 4 + (int)(threadIdx.x >> 6);
+__device__ __forceinline__ void limiter_c(const vx_patch &P, int bx, int by) {
+    int j = -2 + bx * 64 + (int)(threadIdx.x & 63);
+    int i = -2 + by * 4 + (int)(threadIdx.x >> 6);
     if (i >= P.n_x + 2 || j >= P.n_p + 2) return;
     const int pitch = P.pitch;
     long idx = pidx(pitch, i, j);
@@ -251,9 +278,9 @@ __global__ __launch_bounds__(256) void k_limiter_c(vx_patch P) {
 
 // f1 = f2 + limited antidiffusive fluxes on interior cells
 // (Rectangle.cpp:1596-1612), same gather order as the predictor.
-__global__ __launch_bounds__(256) void k_corrector(vx_patch P) {
-    int j = blockIdx.x * 64 + (threadIdx.x & 63);
-    int i = blockIdx.y * 4 + (threadIdx.x >> 6);
+__device__ __forceinline__ void corrector(const vx_patch &P, int bx, int by) {
+    int j = bx * 64 + (threadIdx.x & 63);
+    int i = by * 4 + (threadIdx.x >> 6);
     if (i >= P.n_x || j >= P.n_p) return;
     LoopRange L = loop_range(P);
     const int pitch = P.pitch;
@@ -272,6 +299,55 @@ __global__ __launch_bounds__(256) void k_corrector(vx_patch P) {
         t -= P.Cx[k] * P.FxDS[k];
     }
     P.f[1][idx] = t;
+}
+
+__global__ __launch_bounds__(256) void k_lo_predictor(vx_patch P, double aSum) {
+    lo_predictor(P, aSum, (int)blockIdx.x, (int)blockIdx.y);
+}
+__global__ __launch_bounds__(256) void k_limiter_r(vx_patch P) { limiter_r(P, (int)blockIdx.x, (int)blockIdx.y); }
+__global__ __launch_bounds__(256) void k_limiter_c(vx_patch P) { limiter_c(P, (int)blockIdx.x, (int)blockIdx.y); }
+__global__ __launch_bounds__(256) void k_corrector(vx_patch P) { corrector(P, (int)blockIdx.x, (int)blockIdx.y); }
+
+// Work-list forms: workgroup w of the launch is workgroup w - start of patch k,
+// laid out as that patch's own 2-D grid of 64 x 4 cells (grid_cells).
+template <int KIND, int EXTRA>
+__device__ __forceinline__ void batch_cell(const vx_batch_info *bi, int n, int &k, int &bx, int &by,
+                                           const vx_patch *tab) {
+    const int w = (int)blockIdx.x;
+    k = batch_patch<KIND>(bi, n, w);
+    const int local = w - bi[k].start[KIND];
+    const int gx = (tab[k].n_p + EXTRA + 63) / 64;
+    bx = local % gx;
+    by = local / gx;
+}
+
+__global__ __launch_bounds__(256) void k_lo_predictor_batch(const vx_patch *tab, const vx_batch_info *bi, int n,
+                                                            double aSum) {
+    int k, bx, by;
+    batch_cell<1, 0>(bi, n, k, bx, by, tab);
+    lo_predictor(tab[k], aSum, bx, by);
+}
+__global__ __launch_bounds__(256) void k_limiter_r_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
+    int k, bx, by;
+    batch_cell<2, 2>(bi, n, k, bx, by, tab);
+    limiter_r(tab[k], bx, by);
+}
+__global__ __launch_bounds__(256) void k_limiter_c_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
+    int k, bx, by;
+    batch_cell<3, 4>(bi, n, k, bx, by, tab);
+    limiter_c(tab[k], bx, by);
+}
+__global__ __launch_bounds__(256) void k_corrector_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
+    int k, bx, by;
+    batch_cell<1, 0>(bi, n, k, bx, by, tab);
+    corrector(tab[k], bx, by);
+}
+// f0 <- f1 over the whole plane of every patch (Rectangle.cpp:1614-1622)
+__global__ __launch_bounds__(256) void k_copy_plane_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
+    const int k = batch_patch<4>(bi, n, (int)blockIdx.x);
+    const long e = (long)((int)blockIdx.x - bi[k].start[4]) * 256 + threadIdx.x;
+    const vx_patch &P = tab[k];
+    if (e < (long)(P.n_x + 4) * P.pitch) P.f[0][e] = P.f[1][e];
 }
 
 // zero the 2-wide ghost ring of one slot (sentinel neighbours on all sides)
@@ -309,6 +385,55 @@ dim3 grid_cells(int nx, int np, int extra_i, int extra_j) {
     return dim3((unsigned)((np + extra_j + 63) / 64), (unsigned)((nx + extra_i + 3) / 4));
 }
 
+// strips of the streaming sub-step-0 kernel: the tallest that still gives
+// 2048 wavefronts
+Sub0Tiles sub0_tiles(const vx_patch &p) {
+    Sub0Tiles T;
+    T.nbx = (p.n_p + 4 + S0W - 1) / S0W;
+    T.TI = 4;
+    for (int ti : {64, 32, 16, 8})
+        if ((long)T.nbx * ((p.n_x + 4 + ti - 1) / ti) >= 2048) {
+            T.TI = ti;
+            break;
+        }
+    T.nstrip = (p.n_x + 4 + T.TI - 1) / T.TI;
+    return T;
+}
+
+// the work list of a patch table (built once: a table's shapes never change)
+int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
+    if (!t || t->n < 1) return fail(ctx, VX_EINVAL, "batched sub-step: empty patch table");
+    for (const vx_patch &p : t->host) {
+        int rc = validate_span(ctx, &p, n_f);
+        if (rc) return rc;
+    }
+    if (t->batch_d) return VX_OK;
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "batched sub-step: work list built while recording");
+    t->batch.assign(t->n + 1, vx_batch_info{});
+    int run[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k <= t->n; k++) {
+        vx_batch_info &b = t->batch[k];
+        for (int q = 0; q < 5; q++) b.start[q] = run[q];
+        if (k == t->n) break;
+        const vx_patch &p = t->host[k];
+        const Sub0Tiles T = sub0_tiles(p);
+        b.TI = T.TI;
+        b.nstrip = T.nstrip;
+        b.nbx = T.nbx;
+        dim3 g0 = grid_cells(p.n_x, p.n_p, 0, 0), g2 = grid_cells(p.n_x, p.n_p, 2, 2),
+             g4 = grid_cells(p.n_x, p.n_p, 4, 4);
+        run[0] += T.nbx * ((T.nstrip + 3) / 4);
+        run[1] += (int)(g0.x * g0.y);
+        run[2] += (int)(g2.x * g2.y);
+        run[3] += (int)(g4.x * g4.y);
+        run[4] += (int)(((long)(p.n_x + 4) * p.pitch + 255) / 256);
+    }
+    const size_t bytes = sizeof(vx_batch_info) * t->batch.size();
+    int rc = check_hip(ctx, hipMalloc((void **)&t->batch_d, bytes), "work list");
+    if (!rc) rc = check_hip(ctx, hipMemcpy(t->batch_d, t->batch.data(), bytes, hipMemcpyHostToDevice), "work list");
+    return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -318,31 +443,21 @@ int vx_fct_sub0_fluxes(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     if (rc) return rc;
     if (step < 0 || step > 5 || !fl) return fail(ctx, VX_EINVAL, "sub0: bad step %d", step);
     if ((rc = validate_span(ctx, p, fl->n_f))) return rc;
-    Sub0Args A;
-    A.p = *p;
+    Sub0Coef A;
     A.fl = *fl;
     A.step = step;
     rk_coefficients(step, dt, A.a, A.ns, A.aSum);
-    // strips of the streaming kernel: the tallest that still gives 2048 wavefronts
-    Sub0Tiles T;
-    T.nbx = (p->n_p + 4 + S0W - 1) / S0W;
-    T.TI = 4;
-    for (int ti : {64, 32, 16, 8})
-        if ((long)T.nbx * ((p->n_x + 4 + ti - 1) / ti) >= 2048) {
-            T.TI = ti;
-            break;
-        }
-    T.nstrip = (p->n_x + 4 + T.TI - 1) / T.TI;
+    const Sub0Tiles T = sub0_tiles(*p);
     dim3 grid((unsigned)(T.nbx * ((T.nstrip + 3) / 4)));
     {
         LaunchTimer lt(ctx, "k_sub0");
         switch (step) {
-            case 0: k_sub0s<0><<<grid, 256, 0, ctx->stream>>>(A, T); break;
-            case 1: k_sub0s<1><<<grid, 256, 0, ctx->stream>>>(A, T); break;
-            case 2: k_sub0s<2><<<grid, 256, 0, ctx->stream>>>(A, T); break;
-            case 3: k_sub0s<3><<<grid, 256, 0, ctx->stream>>>(A, T); break;
-            case 4: k_sub0s<4><<<grid, 256, 0, ctx->stream>>>(A, T); break;
-            default: k_sub0s<5><<<grid, 256, 0, ctx->stream>>>(A, T); break;
+            case 0: k_sub0s<0><<<grid, 256, 0, ctx->stream>>>(*p, A, T); break;
+            case 1: k_sub0s<1><<<grid, 256, 0, ctx->stream>>>(*p, A, T); break;
+            case 2: k_sub0s<2><<<grid, 256, 0, ctx->stream>>>(*p, A, T); break;
+            case 3: k_sub0s<3><<<grid, 256, 0, ctx->stream>>>(*p, A, T); break;
+            case 4: k_sub0s<4><<<grid, 256, 0, ctx->stream>>>(*p, A, T); break;
+            default: k_sub0s<5><<<grid, 256, 0, ctx->stream>>>(*p, A, T); break;
         }
     }
     return check_launch(ctx, "k_sub0");
@@ -412,6 +527,65 @@ int vx_fct_stage_walls(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     if ((rc = vx_ghost_zero(ctx, p, 1))) return rc;
     if (step == 5) rc = vx_fct_sub3(ctx, p);
     return rc;
+}
+
+
+// ---- work-list (batched) forms: one launch per sub-step for every patch of a table
+int vx_fct_sub0_fluxes_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const vx_fields *fl) {
+    if (step < 0 || step > 5 || !fl) return fail(ctx, VX_EINVAL, "sub0 table: bad step %d", step);
+    int rc = table_batch(ctx, t, fl->n_f);
+    if (rc) return rc;
+    Sub0Coef A;
+    A.fl = *fl;
+    A.step = step;
+    rk_coefficients(step, dt, A.a, A.ns, A.aSum);
+    dim3 grid((unsigned)t->batch[t->n].start[0]);
+    {
+        LaunchTimer lt(ctx, "k_sub0");
+        switch (step) {
+            case 0: k_sub0s_batch<0><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+            case 1: k_sub0s_batch<1><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+            case 2: k_sub0s_batch<2><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+            case 3: k_sub0s_batch<3><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+            case 4: k_sub0s_batch<4><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+            default: k_sub0s_batch<5><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+        }
+    }
+    return check_launch(ctx, "k_sub0s_batch");
+}
+
+int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, int32_t n_f) {
+    if (step < 0 || step > 5) return fail(ctx, VX_EINVAL, "predictor table: bad step %d", step);
+    int rc = table_batch(ctx, t, n_f);
+    if (rc) return rc;
+    double a[6], aSum;
+    int ns;
+    rk_coefficients(step, dt, a, ns, aSum);
+    k_lo_predictor_batch<<<(unsigned)t->batch[t->n].start[1], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, aSum);
+    return check_launch(ctx, "k_lo_predictor_batch");
+}
+
+int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
+    int rc = table_batch(ctx, t, n_f);
+    if (rc) return rc;
+    k_limiter_r_batch<<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n);
+    if ((rc = check_launch(ctx, "k_limiter_r_batch"))) return rc;
+    k_limiter_c_batch<<<(unsigned)t->batch[t->n].start[3], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n);
+    return check_launch(ctx, "k_limiter_c_batch");
+}
+
+int vx_fct_sub2_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
+    int rc = table_batch(ctx, t, n_f);
+    if (rc) return rc;
+    k_corrector_batch<<<(unsigned)t->batch[t->n].start[1], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n);
+    return check_launch(ctx, "k_corrector_batch");
+}
+
+int vx_fct_sub3_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
+    int rc = table_batch(ctx, t, n_f);
+    if (rc) return rc;
+    k_copy_plane_batch<<<(unsigned)t->batch[t->n].start[4], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n);
+    return check_launch(ctx, "k_copy_plane_batch");
 }
 
 }  // extern "C"

@@ -53,6 +53,23 @@ struct vx_graph {
     hipGraphExec_t exec = nullptr;
 };
 
+// Device table of a mesh's patches (vx_amr.hip), also the work list of the
+// batched sub-step launches (vx_fct.hip): batch_d holds, per patch, where its
+// workgroups start in each batched launch and its strip shape.
+struct vx_batch_info {
+    int start[5];        // first workgroup of the patch: sub-step-0 tiles, cells +0, +2, +4, plane copy
+    int TI, nstrip, nbx; // strips of the streaming sub-step-0 kernel
+};
+struct vx_patch_table {
+    vx_ctx *ctx = nullptr;
+    vx_patch *d = nullptr;
+    int32_t n = 0;
+    uint64_t serial = 0;          // identity for the program bounds check
+    std::vector<vx_patch> host;   // shapes, for the bounds check
+    vx_batch_info *batch_d = nullptr;  // n + 1 entries (the last one: totals)
+    std::vector<vx_batch_info> batch;  // host copy
+};
+
 struct vx_moment_plan {
     int n_species = 0, n_levels = 0, n_f = 0;
     void *tables = nullptr;  // device: MomentSrc[] then level_start[]
