@@ -177,3 +177,22 @@ def test_large_finest_grid_poisson_path(nx):
     assert_same("f1", g.get("f1"), o.get("f1"))
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("nx,np_,fused", [(8, 38, True), (8, 38, False), (130, 60, True), (130, 60, False),
+                                          (34, 300, True), (8, 8, True), (8, 8, False), (510, 62, True)])
+def test_ragged_and_tiny_shapes(nx, np_, fused):
+    """Shapes that leave partial strips, partial 58-cell lanes, single strips
+    and tiny planes (edge cases of the tile mapping; the reference requires
+    sizes divisible by the refinement ratio, Settings.cpp:35-50): 6 steps
+    bitwise."""
+    p = two_stream(nx=nx, np=np_, length=4 * math.pi, amplitude=0.05)
+    g, o = va.Run(p, fused=fused), OracleRun(p)
+    dt = o.calc_dt()
+    g.advance(dt, 6)
+    o.advance(dt, 6)
+    assert_same(f"{nx}x{np_} f1", g.get("f1", 2), o.get("f1", 2))
+    assert_same(f"{nx}x{np_} f0", g.get("f0", 2), o.get("f0", 2))
+    compare_fields(g, o, f"{nx}x{np_}")
+    g.close()
+    o.close()

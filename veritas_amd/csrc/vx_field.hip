@@ -417,9 +417,10 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
         if (i < N) {
             v = P.g[0] * yb[i];
             for (int m = 1; m < NG; m++) {
+                // periodic indices; below 17 entries the taps wrap more than once
                 int ip = i + m, im = i - m;
-                ip -= ip >= N ? N : 0;
-                im += im < 0 ? N : 0;
+                while (ip >= N) ip -= N;
+                while (im < 0) im += N;
                 v += P.g[m] * (yb[ip] + yb[im]);
             }
         }
@@ -634,7 +635,7 @@ __global__ void k_neutralize(vx_field_state F) {
 
 int validate_fs(vx_ctx *ctx, const vx_field_state *fs) {
     if (!fs) return fail(ctx, VX_EINVAL, "null field state");
-    if (fs->n_f < 2 * NG || fs->n_prepad < 2 || fs->n_postpad < 2 ||
+    if (fs->n_f < 8 || fs->n_prepad < 2 || fs->n_postpad < 2 ||
         fs->n_e != fs->n_f + fs->n_prepad + fs->n_postpad)
         return fail(ctx, VX_EINVAL, "bad field sizes n_f=%d pre=%d post=%d n_e=%d", fs->n_f,
                     fs->n_prepad, fs->n_postpad, fs->n_e);
