@@ -319,6 +319,18 @@ int vx_fct_sub0_predictor(vx_ctx *ctx, const vx_patch *p, int step, double dt);
  * [lo_x, hi_x) x [lo_p, hi_p) exceeds `criteria` (uint8 device array). */
 int vx_flag_cells(vx_ctx *ctx, const vx_patch *p, const double *weights5, double criteria,
                   int32_t lo_x, int32_t hi_x, int32_t lo_p, int32_t hi_p, uint8_t *flags);
+/* The same flags, compacted on the device (device regrid assistance; no
+ * counterpart in the reference): `cells` receives the flagged cells' indices
+ * (i - lo_x) * (hi_p - lo_p) + (j - lo_p) in increasing order (the row-major
+ * scan order of Mesh.cpp:212-295), `count` their number (at most max_cells,
+ * else an error), and row_sig / col_sig the signatures of the range, the
+ * number of flagged cells in each x row (hi_x - lo_x entries) and p column
+ * (hi_p - lo_p entries) (the counts of Mesh::computeSignatures,
+ * Mesh.cpp:386-447).  Host pointers; any of cells, row_sig, col_sig may be
+ * NULL. */
+int vx_flag_compact(vx_ctx *ctx, const vx_patch *p, const double *weights5, double criteria,
+                    int32_t lo_x, int32_t hi_x, int32_t lo_p, int32_t hi_p, int32_t *cells,
+                    int32_t max_cells, int32_t *count, int32_t *row_sig, int32_t *col_sig);
 
 /* ---- timing helpers (HIP events on the context stream) ---- */
 int vx_event_record(vx_ctx *ctx, int slot);

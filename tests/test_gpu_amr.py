@@ -73,6 +73,30 @@ def test_amr_initial_hierarchy_matches(amr_pair):
         assert np.array_equal(g.flags(lvl), o.flags(lvl)), f"flags of hierarchy level {lvl}"
 
 
+def test_device_flag_signatures(amr_pair):
+    """The flags come back compacted on the device (in the reference's scan
+    order: equal to the oracle's list above); the row and column signatures
+    computed there equal the counts of that list over each patch's range."""
+    p, g, o = amr_pair
+    lay = g.layout()
+    nlev = int(lay[:, 0].max()) + 1
+    checked = 0
+    for lvl in range(nlev - 1):  # levels that flag cells for a finer one (0 = coarsest)
+        fl = g.flags(lvl)
+        assert np.array_equal(fl, o.flags(lvl))
+        npatch = int((lay[:, 0] == nlev - 1 - lvl).sum())
+        for k in range(npatch):
+            sg = g.signatures(lvl, k)
+            x0, p0, rows, cols = sg["lo_x"], sg["lo_p"], sg["rows"], sg["cols"]
+            inside = ((fl[:, 0] >= x0) & (fl[:, 0] < x0 + len(rows)) & (fl[:, 1] >= p0) &
+                      (fl[:, 1] < p0 + len(cols)))
+            sub = fl[inside]
+            assert np.array_equal(rows, np.bincount(sub[:, 0] - x0, minlength=len(rows)))
+            assert np.array_equal(cols, np.bincount(sub[:, 1] - p0, minlength=len(cols)))
+            checked += int(rows.sum())
+    assert checked > 0
+
+
 def test_amr_steps_and_regrid_match(amr_pair):
     p, g, o = amr_pair
     dt = o.calc_dt(0.4)

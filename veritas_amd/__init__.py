@@ -63,6 +63,8 @@ def host_lib():
         L.vxh_set_mixed.argtypes = [vp, i]
         L.vxh_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vxh_tile_rows.argtypes = [vp, i, i, i]
+        L.vxh_signatures.restype = i
+        L.vxh_signatures.argtypes = [vp, i, i, i, vp, i]
         L.vxh_server_create.restype = vp
         L.vxh_server_create.argtypes = [i, d, i]
         L.vxh_server_destroy.argtypes = [vp]
@@ -247,6 +249,19 @@ class Run:
         lay = self.layout(sp)
         row = lay[lay[:, 0] == level][idx]
         return out.reshape(int(row[3]) + 2 * ghosts, int(row[4]) + 2 * ghosts)
+
+    def signatures(self, level, idx, sp=0):
+        """Row and column flag counts of patch `idx` of level `level` (0 =
+        coarsest) from the device compaction of the last flags() call."""
+        L = host_lib()
+        n = L.vxh_signatures(self.h, sp, level, idx, None, 0)
+        if n < 0:
+            self._ck(-1)
+        a = (ctypes.c_int * max(n, 1))()
+        L.vxh_signatures(self.h, sp, level, idx, a, n)
+        v = np.array(a[:n], dtype=np.int64)
+        nr, nc = int(v[2]), int(v[3])
+        return {"lo_x": int(v[0]), "lo_p": int(v[1]), "rows": v[4:4 + nr], "cols": v[4 + nr:4 + nr + nc]}
 
     def flags(self, level, sp=0):
         """Refinement flags of hierarchy level `level` (0 = coarsest) as (i, j) rows."""

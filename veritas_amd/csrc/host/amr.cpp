@@ -566,6 +566,24 @@ void Rectangle::getError(std::vector<coords> &flagged) {
     const int lp = std::max(p_pos, offset), up_ = std::min(p_pos + n_p, settings.GetPSize(depth, particleType) - offset);
     if (ux <= lx || up_ <= lp) return;
     const size_t n = (size_t)(ux - lx) * (up_ - lp);
+    const bool hook = settings.scn.has_refinement_override();
+    if (!hook) {
+        // device compaction: only the flagged cells (and the range's signatures)
+        // come back, in the row-major order of the scan below
+        std::vector<int32_t> cells(n);
+        int32_t count = 0;
+        sig_x.assign(ux - lx, 0);
+        sig_p.assign(up_ - lp, 0);
+        dev->check(vx_flag_compact(dev->ctx, &P, errorWeights, settings.refinementCriteria, lx - x_pos, ux - x_pos,
+                                   lp - p_pos, up_ - p_pos, cells.data(), (int32_t)n, &count, sig_x.data(),
+                                   sig_p.data()),
+                   "flag compaction");
+        const int wd = up_ - lp;
+        for (int32_t k = 0; k < count; k++) flagged.push_back(std::make_pair(lx + cells[k] / wd, lp + cells[k] % wd));
+        sig_lo_x = lx;
+        sig_lo_p = lp;
+        return;
+    }
     void *d = nullptr;
     dev->check(vx_malloc(dev->ctx, n, &d), "alloc flags");
     std::vector<uint8_t> h(n);
@@ -574,7 +592,6 @@ void Rectangle::getError(std::vector<coords> &flagged) {
     if (rc == VX_OK) rc = vx_memcpy_d2h(dev->ctx, h.data(), d, n);
     vx_free(dev->ctx, d);
     dev->check(rc, "flags");
-    const bool hook = settings.scn.has_refinement_override();
     size_t e = 0;
     for (int i = lx; i < ux; i++)
         for (int j = lp; j < up_; j++, e++)

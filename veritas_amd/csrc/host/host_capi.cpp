@@ -199,6 +199,25 @@ int vxh_flags(void *h, int sp, int lvl, int *out, int n) {
     }
 }
 
+// signatures kept by the last getError of patch idx of level lvl (0 = the
+// coarsest, as vxh_flags): out = lo_x, lo_p, n_rows, n_cols, rows..., cols...;
+// returns the length
+int vxh_signatures(void *h, int sp, int lvl, int idx, int *out, int n) {
+    Run *r = (Run *)h;
+    try {
+        auto &levels = r->sm->meshes.at(sp)->levels;
+        const vxh::Rectangle &R = *levels.at(r->settings->maxDepth - lvl)->rectangles.at(idx);
+        std::vector<int> v = {R.sig_lo_x, R.sig_lo_p, (int)R.sig_x.size(), (int)R.sig_p.size()};
+        v.insert(v.end(), R.sig_x.begin(), R.sig_x.end());
+        v.insert(v.end(), R.sig_p.begin(), R.sig_p.end());
+        if (out && n >= (int)v.size()) std::memcpy(out, v.data(), sizeof(int) * v.size());
+        return (int)v.size();
+    } catch (std::exception &e) {
+        r->err = e.what();
+        return -1;
+    }
+}
+
 int vxh_regrid(void *h) { return guarded(h, [&](Run *r) { r->sm->reGrid(); }); }
 
 // SolverManager::fileOutput / OutputRectangles: text files under output/ of

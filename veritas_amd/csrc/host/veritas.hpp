@@ -111,6 +111,10 @@ public:
     void CalculateConnectivitySame(Rectangle *rect);                    // Rectangle.cpp:671-766
     void CalculateConnectivityFromFiner(Rectangle *current, Rectangle *rect);  // 768-864
     void getError(std::vector<std::pair<int, int>> &flagged);          // 866-890
+    // flagged cells per x row / p column of the last getError's range (from the
+    // device compaction), the range starting at (sig_lo_x, sig_lo_p)
+    std::vector<int> sig_x, sig_p;
+    int sig_lo_x = 0, sig_lo_p = 0;
     void UploadNested();
     Rectangle(int n_x, int n_p, int x_pos, int p_pos, int depth, Settings &settings,
               std::shared_ptr<Device> dev, bool up, bool down, bool left, bool right,

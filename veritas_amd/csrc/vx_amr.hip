@@ -278,6 +278,37 @@ __global__ __launch_bounds__(256) void k_flags(vx_patch P, double w0, double w1,
     flags[e] = err > crit ? 1 : 0;
 }
 
+// Ordered compaction of the flags: per 1024-cell block, the count (k_flag_count),
+// then each flagged cell's index at its block's offset plus its rank in the
+// block (wavefront ballots), so the list is in increasing cell index, i.e. the
+// reference's row-major scan order (Mesh.cpp:212-295 collects flags that way).
+constexpr int FB = 1024;
+
+__global__ __launch_bounds__(FB) void k_flag_count(const uint8_t *flags, long n, int *cnt) {
+    const long e = blockIdx.x * (long)FB + threadIdx.x;
+    const int c = __syncthreads_count(e < n && flags[e]);
+    if (threadIdx.x == 0) cnt[blockIdx.x] = c;
+}
+
+__global__ __launch_bounds__(FB) void k_flag_scatter(const uint8_t *flags, long n, const int *off, int w,
+                                                     int *list, int *row_sig, int *col_sig) {
+    __shared__ int wave_tot[FB / 64];
+    const long e = blockIdx.x * (long)FB + threadIdx.x;
+    const bool f = e < n && flags[e];
+    const unsigned long long m = __ballot(f);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rank = __popcll(m & ((1ull << lane) - 1));
+    if (lane == 0) wave_tot[wv] = __popcll(m);
+    __syncthreads();
+    int base = off[blockIdx.x];
+    for (int k = 0; k < wv; k++) base += wave_tot[k];
+    if (f) {
+        list[base + rank] = (int)e;
+        atomicAdd(&row_sig[e / w], 1);  // integer counts: exact in any order
+        atomicAdd(&col_sig[e % w], 1);
+    }
+}
+
 int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r);
 
 int fill_common(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int r, ProgArgs &A) {
@@ -498,6 +529,63 @@ int vx_flag_cells(vx_ctx *ctx, const vx_patch *p, const double *w, double criter
     k_flags<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(*p, w[0], w[1], w[2], w[3], w[4], criteria,
                                                                  lo_x, hi_x, lo_p, hi_p, flags);
     return check_launch(ctx, "k_flags");
+}
+
+
+int vx_flag_compact(vx_ctx *ctx, const vx_patch *p, const double *w, double criteria, int32_t lo_x, int32_t hi_x,
+                    int32_t lo_p, int32_t hi_p, int32_t *cells, int32_t max_cells, int32_t *count,
+                    int32_t *row_sig, int32_t *col_sig) {
+    int rc = validate_patch(ctx, p);
+    if (rc) return rc;
+    if (!w || !count) return fail(ctx, VX_EINVAL, "flag compaction: null weights or count");
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "flag compaction while a step graph is recorded");
+    if (lo_x < 0 || lo_p < 0 || hi_x > p->n_x || hi_p > p->n_p)
+        return fail(ctx, VX_EINVAL, "flags: range [%d,%d)x[%d,%d) outside the patch", lo_x, hi_x, lo_p, hi_p);
+    *count = 0;
+    if (hi_x <= lo_x || hi_p <= lo_p) return VX_OK;
+    const int wd = hi_p - lo_p, hx = hi_x - lo_x;
+    const long n = (long)hx * wd;
+    const int nb = (int)((n + FB - 1) / FB);
+    // device scratch: flags | block counts, offsets | list | row, column signatures
+    const size_t bytes = (size_t)n + sizeof(int) * (2 * (size_t)nb + (size_t)n + hx + wd) + 64;
+    char *d = nullptr;
+    if ((rc = check_hip(ctx, hipMalloc((void **)&d, bytes), "flag compaction scratch"))) return rc;
+    uint8_t *flags = (uint8_t *)d;
+    int *cnt = (int *)(d + ((n + 15) / 16) * 16), *off = cnt + nb, *list = off + nb, *rs = list + n, *cs = rs + hx;
+    std::vector<int> hc(nb), ho(nb);
+    k_flags<<<(unsigned)((n + 255) / 256), 256, 0, ctx->stream>>>(*p, w[0], w[1], w[2], w[3], w[4], criteria,
+                                                                 lo_x, hi_x, lo_p, hi_p, flags);
+    rc = check_launch(ctx, "k_flags");
+    if (!rc) {
+        k_flag_count<<<(unsigned)nb, FB, 0, ctx->stream>>>(flags, n, cnt);
+        rc = check_launch(ctx, "k_flag_count");
+    }
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(hc.data(), cnt, sizeof(int) * nb, hipMemcpyDeviceToHost, ctx->stream),
+                            "flag counts");
+    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "flag counts");
+    long total = 0;
+    for (int b = 0; b < nb; b++) {
+        ho[b] = (int)total;
+        total += hc[b];
+    }
+    if (!rc && cells && total > max_cells) rc = fail(ctx, VX_EINVAL, "flag compaction: %ld flagged cells > %d", total, max_cells);
+    if (!rc) rc = check_hip(ctx, hipMemcpyAsync(off, ho.data(), sizeof(int) * nb, hipMemcpyHostToDevice, ctx->stream),
+                            "flag offsets");
+    if (!rc) rc = check_hip(ctx, hipMemsetAsync(rs, 0, sizeof(int) * (hx + wd), ctx->stream), "signatures");
+    if (!rc) {
+        k_flag_scatter<<<(unsigned)nb, FB, 0, ctx->stream>>>(flags, n, off, wd, list, rs, cs);
+        rc = check_launch(ctx, "k_flag_scatter");
+    }
+    if (!rc && cells && total > 0)
+        rc = check_hip(ctx, hipMemcpyAsync(cells, list, sizeof(int) * total, hipMemcpyDeviceToHost, ctx->stream), "list");
+    if (!rc && row_sig)
+        rc = check_hip(ctx, hipMemcpyAsync(row_sig, rs, sizeof(int) * hx, hipMemcpyDeviceToHost, ctx->stream), "rows");
+    if (!rc && col_sig)
+        rc = check_hip(ctx, hipMemcpyAsync(col_sig, cs, sizeof(int) * wd, hipMemcpyDeviceToHost, ctx->stream), "cols");
+    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "flag compaction");
+    (void)hipFree(d);
+    if (!rc) *count = (int32_t)total;
+    return rc;
 }
 
 }  // extern "C"
