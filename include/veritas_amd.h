@@ -66,6 +66,14 @@ typedef struct vx_patch {
     double *chargeR, *currentR; /* n_x * rtb each */
 } vx_patch;
 
+/* Rectangle::f slot `slot` (0: f^n, 1: current stage, 2: predictor;
+ * Rectangle.hpp:91-108 interleaves them, here each is a plane) of one patch
+ * to or from a host array of (n_x + 2g) x (n_p + 2g) doubles, row-major, p
+ * fastest, keeping g = 0, 1 or 2 ghost layers; waits for the copy.  For
+ * output, initial data and golden files. */
+int vx_patch_upload(vx_ctx *ctx, const vx_patch *p, int slot, int ghosts, const double *host);
+int vx_patch_download(vx_ctx *ctx, const vx_patch *p, int slot, int ghosts, double *host);
+
 /* 1-D finest-grid field data the patches read during a stage. */
 typedef struct vx_fields {
     int32_t n_f;       /* Settings::x_size_finest */

@@ -91,6 +91,8 @@ def host_lib():
         L.vxh_timing_end.argtypes = [vp, ctypes.POINTER(d), ctypes.POINTER(i)]
         L.vxh_get.restype = i
         L.vxh_get.argtypes = [vp, ctypes.c_char_p, i, vp, i]
+        L.vxh_put.restype = i
+        L.vxh_put.argtypes = [vp, ctypes.c_char_p, i, vp, i]
         L.vxh_layout.restype = i
         L.vxh_layout.argtypes = [vp, i, vp, i]
         L.vxh_patch_get.restype = i
@@ -203,6 +205,15 @@ class Run:
         ms, n = ctypes.c_double(), ctypes.c_int()
         self._ck(host_lib().vxh_timing_end(self.h, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
+
+    def put(self, what, values, ghosts=0):
+        """Write f slot "f0", "f1" or "f2" of the base patch from an array of
+        (nx + 2g) x (np + 2g) values; ghost layers beyond g are left alone."""
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        rc = host_lib().vxh_put(self.h, what.encode(), ghosts, a.ctypes.data, a.size)
+        if rc == -2:
+            raise KeyError(f"{what} with {a.size} values and {ghosts} ghost layers")
+        self._ck(rc)
 
     def get(self, what, ghosts=0):
         L = host_lib()

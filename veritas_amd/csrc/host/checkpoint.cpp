@@ -53,6 +53,13 @@ void Rectangle::UploadSlot(int slot, const std::vector<double> &full) {
     dev->check(vx_memcpy_h2d(dev->ctx, P.f[slot], full.data(), sizeof(double) * full.size()), "upload slot");
 }
 
+void Rectangle::UploadSlot(int slot, int g, const double *host) {
+    UnaliasSlots();
+    const size_t n = (size_t)(n_x + 2 * g) * (n_p + 2 * g);
+    for (size_t k = 0; k < n; k++) fScale = std::max(fScale, std::fabs(host[k]));
+    dev->check(vx_patch_upload(dev->ctx, &P, slot, g, host), "upload slot");
+}
+
 void Mesh::Restore(const std::vector<level> &h) {
     hierarchy = h;
     promoteHierarchyToMesh(true);

@@ -23,6 +23,9 @@ int copy_out(const std::vector<double> &v, double *out, int n) {
 
 vxh::Rectangle &base(Run *r) { return *r->sm->meshes[0]->levels.back()->rectangles[0]; }
 
+// f slot named "f0", "f1" or "f2", else -1
+int slot_of(const std::string &w) { return w.size() == 2 && w[0] == 'f' && w[1] >= '0' && w[1] <= '2' ? w[1] - '0' : -1; }
+
 std::vector<double> dl(Run *r, const double *d, size_t n) {
     std::vector<double> v(n);
     r->sm->dev->check(vx_memcpy_d2h(r->sm->dev->ctx, v.data(), d, sizeof(double) * n), "download");
@@ -171,6 +174,7 @@ int vxh_patch_get(void *h, int sp, int lvl, int idx, const char *what, int ghost
         if (lvl < 0 || lvl >= (int)levels.size() || idx < 0 || idx >= (int)levels[lvl]->rectangles.size())
             return -2;
         vxh::Rectangle &p = *levels[lvl]->rectangles[idx];
+        if (slot_of(what) >= 0) return copy_out(p.DownloadSlot(slot_of(what), ghosts), out, n);
         const double *pl = p.Plane(what);
         if (!pl) return -2;
         return copy_out(p.Download(pl, ghosts), out, n);
@@ -229,12 +233,24 @@ int vxh_output_flush(void *h) { return guarded(h, [&](Run *r) { r->sm->FlushOutp
 int vxh_save(void *h, const char *path) { return guarded(h, [&](Run *r) { r->sm->SaveCheckpoint(path); }); }
 int vxh_load(void *h, const char *path) { return guarded(h, [&](Run *r) { r->sm->LoadCheckpoint(path); }); }
 
+// f slot "f0"/"f1"/"f2" of the base patch from n = (n_x+2g)(n_p+2g) values
+// (ghost layers beyond g keep their contents); -2 unknown name or size
+int vxh_put(void *h, const char *what, int ghosts, const double *in, int n) {
+    Run *r = (Run *)h;
+    int s = slot_of(what);
+    vxh::Rectangle &p = base(r);
+    if (s < 0 || ghosts < 0 || ghosts > 2 || (size_t)n != (size_t)(p.n_x + 2 * ghosts) * (p.n_p + 2 * ghosts))
+        return -2;
+    return guarded(h, [&](Run *) { p.UploadSlot(s, ghosts, in); });
+}
+
 // Named arrays, same names and layouts as the oracle's C API.
 int vxh_get(void *h, const char *what, int ghosts, double *out, int n) {
     Run *r = (Run *)h;
     try {
         std::string w(what);
         vxh::Rectangle &p = base(r);
+        if (slot_of(w) >= 0) return copy_out(p.DownloadSlot(slot_of(w), ghosts), out, n);
         if (const double *pl = p.Plane(w)) return copy_out(p.Download(pl, ghosts), out, n);
         const vx_field_state &fs = r->sm->EMSolver->state();
         int N = fs.n_f;

@@ -308,6 +308,12 @@ void Rectangle::CalculateRhoAndJ() {
         dev->check(vx_moments_patch(dev->ctx, &P, &EMSolver->fields()), "moments");
 }
 
+std::vector<double> Rectangle::DownloadSlot(int slot, int g) const {
+    std::vector<double> out((size_t)(n_x + 2 * g) * (n_p + 2 * g));
+    dev->check(vx_patch_download(dev->ctx, &P, slot, g, out.data()), "download slot");
+    return out;
+}
+
 std::vector<double> Rectangle::Download(const double *plane, int g) const {
     std::vector<double> full(vx_plane_elems(n_x, n_p));
     dev->check(vx_memcpy_d2h(dev->ctx, full.data(), plane, sizeof(double) * full.size()), "download");

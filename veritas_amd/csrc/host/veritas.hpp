@@ -145,9 +145,10 @@ public:
     bool AllWalls() const { return up && down && left && right; }
     const vx_patch &desc() const { return P; }
     std::vector<double> Download(const double *plane, int ghosts) const;
-    std::vector<double> DownloadSlot(int slot, int ghosts) const { return Download(P.f[slot], ghosts); }
+    std::vector<double> DownloadSlot(int slot, int ghosts) const;  // vx_patch_download
     const double *Plane(const std::string &name) const;
     void UploadSlot(int slot, const std::vector<double> &full);  // whole plane incl. ghosts
+    void UploadSlot(int slot, int ghosts, const double *host);  // (n_x+2g) x (n_p+2g), vx_patch_upload
     std::vector<double> CalculateEnergy();  // Rectangle.cpp:284-305 on the device
     double *energyR_d = nullptr;
     double fScale = 0.0;  // max |f| of the host uploads (scale of the mixed-precision FP32 copies)

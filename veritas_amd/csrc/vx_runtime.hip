@@ -245,4 +245,29 @@ int vx_ctx_wait(vx_ctx *waiter, vx_ctx *signaller) {
 
 size_t vx_ctx_live_bytes(vx_ctx *ctx) { return ctx ? ctx->live_bytes : 0; }
 
+
+// Slot `slot` of a patch <-> a host array of (n_x + 2g) x (n_p + 2g) doubles,
+// row-major (g = 0, 1, 2 ghost layers kept), one strided copy each way.
+static int patch_copy(vx_ctx *ctx, const vx_patch *p, int slot, int ghosts, double *host, bool up) {
+    if (!p || !host || slot < 0 || slot > 2 || ghosts < 0 || ghosts > 2 || !p->f[slot])
+        return fail(ctx, VX_EINVAL, "patch copy: bad slot %d or ghosts %d", slot, ghosts);
+    if (int rc = not_recording(ctx, "patch copy")) return rc;
+    const size_t w = (size_t)(p->n_p + 2 * ghosts), h = (size_t)(p->n_x + 2 * ghosts);
+    double *dev = p->f[slot] + (size_t)(2 - ghosts) * p->pitch + (2 - ghosts);
+    hipError_t e = up ? hipMemcpy2DAsync(dev, sizeof(double) * p->pitch, host, sizeof(double) * w,
+                                         sizeof(double) * w, h, hipMemcpyHostToDevice, ctx->stream)
+                      : hipMemcpy2DAsync(host, sizeof(double) * w, dev, sizeof(double) * p->pitch,
+                                         sizeof(double) * w, h, hipMemcpyDeviceToHost, ctx->stream);
+    int rc = check_hip(ctx, e, up ? "patch upload" : "patch download");
+    return rc ? rc : vx_sync(ctx);
+}
+
+int vx_patch_upload(vx_ctx *ctx, const vx_patch *p, int slot, int ghosts, const double *host) {
+    return patch_copy(ctx, p, slot, ghosts, const_cast<double *>(host), true);
+}
+
+int vx_patch_download(vx_ctx *ctx, const vx_patch *p, int slot, int ghosts, double *host) {
+    return patch_copy(ctx, p, slot, ghosts, host, false);
+}
+
 }  // extern "C"
