@@ -409,22 +409,39 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
         if (base + k < N) yb[base + k] = run;
     }
     __syncthreads();
+    // the thread's CH entries share one window of CH + 32 taps:
+    // w[q] = y[base - 16 + q], periodic (below 34 entries the taps can wrap
+    // more than once)
+    constexpr int WN = CH + 2 * (NG - 1);
     double cp[CH];
+    if (base < N) {
+        double w[WN];
+        if (N >= 2 * NG) {
 #pragma unroll
-    for (int k = 0; k < CH; k++) {
-        int i = base + k;
-        double v = 0.0;
-        if (i < N) {
-            v = P.g[0] * yb[i];
-            for (int m = 1; m < NG; m++) {
-                // periodic indices; below 17 entries the taps wrap more than once
-                int ip = i + m, im = i - m;
-                while (ip >= N) ip -= N;
-                while (im < 0) im += N;
-                v += P.g[m] * (yb[ip] + yb[im]);
+            for (int q = 0; q < WN; q++) {
+                int u = base - (NG - 1) + q;
+                u = u < 0 ? u + N : (u >= N ? u - N : u);
+                w[q] = yb[u];
+            }
+        } else {
+#pragma unroll
+            for (int q = 0; q < WN; q++) {
+                int u = base - (NG - 1) + q;
+                while (u >= N) u -= N;
+                while (u < 0) u += N;
+                w[q] = yb[u];
             }
         }
-        cp[k] = v;
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            double v = P.g[0] * w[NG - 1 + k];
+#pragma unroll
+            for (int m = 1; m < NG; m++) v += P.g[m] * (w[NG - 1 + k + m] + w[NG - 1 + k - m]);
+            cp[k] = base + k < N ? v : 0.0;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < CH; k++) cp[k] = 0.0;
     }
     __syncthreads();
     if (threadIdx.x == 0) part[0] = cp[0];

@@ -112,6 +112,17 @@ __device__ __forceinline__ void add_two(double &acc, double &cmp, double v) {
 // chain ch in p order and lane 32 + ch the current summands.  Skipped cells
 // (nested, or past n_p) contribute -0.0, which is an exact identity for
 // addition, so every chain performs the reference's additions in its order.
+//
+// Latency: everything a chunk's summands read from global memory (the cell
+// values, the logarithm table, the nested flags) is loaded one chunk ahead
+// into registers and staged in LDS, so the summands and the ordered sums read
+// LDS only and never wait on the next chunk's loads.  The ordered sums of
+// chunk k run in the same straight-line block as the summands of chunk k + 1
+// (summand buffers alternate), so the compiler fills the latency of each
+// dependent addition with the next chunk's arithmetic.
+#ifndef SCHED_FENCE
+#define SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
+#endif
 template <int RTB, int C, bool COMP>
 __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     constexpr int NCH = C * RTB;
@@ -123,7 +134,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     __shared__ double S[4][NCH][68];   // sub-cell values, p index jb-1+e
     __shared__ double H[4][68];        // Gamma(p)+c1 p for the chain being prepared
     __shared__ double Lb[4][NCH][68];  // c2 log(h(j+1)/h(j)), p index jb-1+e
-    __shared__ double TR[4][NCH][65], TJ[4][NCH][65];
+    __shared__ double T[2][4][2][NCH][65];  // summands: buffer, wave, charge/current, chain, p
     if (i0 >= P.n_x) return;  // uniform per wavefront
     const bool use_tab = A.tab[P.n_p + 2] != 0.0;  // uniform A^2: tabulated logarithms
     const int pitch = P.pitch;
@@ -172,25 +183,129 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
 #pragma unroll
         for (int k = 0; k < RTB; k++) out[k] = v[k] + corr;
     };
-    // chunk prefetch: values for p in [jb-1, jb+65) held in registers one
-    // chunk ahead so the global-memory latency overlaps the ordered sums
-    double pre[NCH][2];
-    auto prefetch = [&](int j0) {
+    // a chunk in registers: values for p in [j0, j0+66), the table entries
+    // tab[j0+1 .. j0+67) (clamped to the table) and the nested flag of each
+    // column's p-cell j0+1+lane; two such sets are in flight (two chunks ahead)
+    struct Pre {
+        double v[NCH][2], t[2];
+        int n[C];
+    };
+    auto prefetch = [&](int j0, Pre &pr) {
         for (int c = 0; c < C; c++) {
             double v[RTB];
             column(c, j0 + lane, v);
-            for (int k = 0; k < RTB; k++) pre[c * RTB + k][0] = v[k];
+            for (int k = 0; k < RTB; k++) pr.v[c * RTB + k][0] = v[k];
             if (lane < 2) column(c, j0 + 64 + lane, v);
-            for (int k = 0; k < RTB; k++) pre[c * RTB + k][1] = lane < 2 ? v[k] : 0.0;
+            for (int k = 0; k < RTB; k++) pr.v[c * RTB + k][1] = lane < 2 ? v[k] : 0.0;
+        }
+        if (use_tab) {
+            const int t0 = min(j0 + 1 + lane, P.n_p + 2), t1 = min(j0 + 65 + lane, P.n_p + 2);
+            pr.t[0] = A.tab[t0];
+            pr.t[1] = lane < 2 ? A.tab[t1] : 0.0;
+        }
+        if (P.nested) {
+            const int jn = min(j0 + 1 + lane, P.n_p - 1);
+            for (int c = 0; c < C; c++) pr.n[c] = P.nested[pidx(pitch, min(i0 + c, P.n_x - 1), jn)];
+        } else {
+            for (int c = 0; c < C; c++) pr.n[c] = 0;
         }
     };
-    prefetch(-1);
-    for (int jb = 0; jb < P.n_p; jb += 64) {
+    // summands of p-cell j = jb + lane for every chain (Rectangle.cpp:214-253),
+    // into T[b] (or the lanes' compensated sums); no branches and LDS reads only
+    auto summands = [&](int jb, int b, const int *nflag) {
+        const int j = jb + lane;
+        const bool live = j < P.n_p;
+#pragma unroll
         for (int ch = 0; ch < NCH; ch++) {
-            S[wave][ch][lane] = pre[ch][0];
-            if (lane < 2) S[wave][ch][64 + lane] = pre[ch][1];
+            const int c = ch / RTB;
+            const bool skip = !live || i0 + c >= P.n_x || nflag[c] != 0;
+            const double *L = use_tab ? &Lb[wave][0][0] : &Lb[wave][ch][0];
+            const double tm = S[wave][ch][lane], t0 = S[wave][ch][lane + 1], tp = S[wave][ch][lane + 2];
+            const double gm = L[lane + 1], gm1 = L[lane], gp1 = L[lane + 2];
+            const double jv = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
+            const double tr = skip ? -0.0 : t0, tj = skip ? -0.0 : jv;
+            if (COMP) {
+                add_two(accR[ch], cmpR[ch], tr);
+                add_two(accJ[ch], cmpJ[ch], tj);
+            } else {
+                T[b][wave][0][ch][lane] = tr;
+                T[b][wave][1][ch][lane] = tj;
+            }
         }
-        if (jb + 64 < P.n_p) prefetch(jb + 63);
+    };
+    // plain form: the ordered sums of buffer b ^ 1 (the previous chunk) with
+    // the summands of chunk jb (into buffer b) placed between their groups of
+    // eight additions, so that the additions' latency is filled with them
+    auto fused = [&](int jb, int b, const int *nflag) {
+        const int j = jb + lane;
+        const bool live = j < P.n_p;
+        const int chr = (lane & 31) < NCH ? (lane & 31) : 0;
+        const double *row = &T[b ^ 1][wave][lane >> 5][chr][0];
+        double a = acc, v[16];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = row[q];
+#pragma unroll
+        for (int g = 0; g < 8; g++) {
+            double *cur = &v[(g & 1) * 8], *nxt = &v[(~g & 1) * 8];
+            if (g + 1 < 8) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) nxt[q] = row[8 * (g + 1) + q];
+            }
+#pragma unroll
+            for (int ch = g * NCH / 8; ch < (g + 1) * NCH / 8; ch++) {
+                const int c = ch / RTB;
+                const bool skip = !live || i0 + c >= P.n_x || nflag[c] != 0;
+                const double *L = use_tab ? &Lb[wave][0][0] : &Lb[wave][ch][0];
+                const double tm = S[wave][ch][lane], t0 = S[wave][ch][lane + 1], tp = S[wave][ch][lane + 2];
+                const double gm = L[lane + 1], gm1 = L[lane], gp1 = L[lane + 2];
+                const double jv = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
+                T[b][wave][0][ch][lane] = skip ? -0.0 : t0;
+                T[b][wave][1][ch][lane] = skip ? -0.0 : jv;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) a += cur[q];
+            SCHED_FENCE();
+        }
+        acc = a;
+    };
+    // chain (lane & 31) of charge (lanes < 32) or current adds the 64 summands
+    // of T[b] in p order; the other lanes repeat chain 0 and are not stored
+    auto ordered = [&](int b) {
+        const int ch = (lane & 31) < NCH ? (lane & 31) : 0;
+        const double *row = &T[b][wave][lane >> 5][ch][0];
+        double a = acc, v[16];
+#pragma unroll
+        for (int q = 0; q < 8; q++) v[q] = row[q];
+#pragma unroll
+        for (int e = 0; e < 64; e += 8) {
+            // the next group's reads go out before this group's additions
+            double *cur = &v[(e >> 3 & 1) * 8], *nxt = &v[(~e >> 3 & 1) * 8];
+            if (e + 8 < 64) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) nxt[q] = row[e + 8 + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) a += cur[q];
+        }
+        acc = a;
+    };
+    if (!COMP) {
+        // the ordered sums of the (absent) chunk before the first add -0.0
+        for (int e = lane; e < 2 * NCH * 65; e += 64) (&T[1][wave][0][0][0])[e] = -0.0;
+    }
+    // chunk jb from pr (summand buffer bb), whose registers then take chunk jb + 128
+    auto chunk = [&](int jb, int bb, Pre &pr) {
+        for (int ch = 0; ch < NCH; ch++) {
+            S[wave][ch][lane] = pr.v[ch][0];
+            if (lane < 2) S[wave][ch][64 + lane] = pr.v[ch][1];
+        }
+        if (use_tab) {
+            Lb[wave][0][lane] = pr.t[0];
+            if (lane < 2) Lb[wave][0][64 + lane] = pr.t[1];
+        }
+        int nflag[C];
+        for (int c = 0; c < C; c++) nflag[c] = pr.n[c];
+        if (jb + 128 < P.n_p) prefetch(jb + 127, pr);
         // logarithm terms, computed once per distinct A^2 among the chains
         for (int ch = 0; ch < NCH && !use_tab; ch++) {
             if (ch > 0 && asq[ch] == asq[ch - 1]) {
@@ -206,52 +321,23 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             wave_sync();
         }
         wave_sync();
-        // summands of p-cell j = jb + lane for every chain (Rectangle.cpp:214-253)
-        const int j = jb + lane;
-        const bool live = j < P.n_p;
-#pragma unroll
-        for (int ch = 0; ch < NCH; ch++) {
-            int c = ch / RTB;
-            int i = i0 + c;
-            bool skip = !live || i >= P.n_x || (P.nested && P.nested[pidx(pitch, i, j)]);
-            double tr = -0.0, tj = -0.0;
-            if (!skip) {
-                double tm = S[wave][ch][lane], t0 = S[wave][ch][lane + 1], tp = S[wave][ch][lane + 2];
-                double gm, gm1, gp1;
-                if (use_tab) {
-                    gm = A.tab[j + 1];
-                    gm1 = A.tab[j];
-                    gp1 = A.tab[j + 2];
-                } else {
-                    gm = Lb[wave][ch][lane + 1];
-                    gm1 = Lb[wave][ch][lane];
-                    gp1 = Lb[wave][ch][lane + 2];
-                }
-                tr = t0;
-                tj = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
-            }
-            if (COMP) {
-                add_two(accR[ch], cmpR[ch], tr);
-                add_two(accJ[ch], cmpJ[ch], tj);
-            } else {
-                TR[wave][ch][lane] = tr;
-                TJ[wave][ch][lane] = tj;
-            }
-        }
+        if (COMP) summands(jb, bb, nflag);
+        else fused(jb, bb, nflag);
         wave_sync();
-        // ordered accumulation
-        if (!COMP && (lane & 31) < NCH) {
-            const double *row = (lane < 32) ? &TR[wave][lane][0] : &TJ[wave][lane - 32][0];
-            double a = acc;
-            for (int e = 0; e < 64; e += 8) {
-                double v0 = row[e], v1 = row[e + 1], v2 = row[e + 2], v3 = row[e + 3];
-                double v4 = row[e + 4], v5 = row[e + 5], v6 = row[e + 6], v7 = row[e + 7];
-                a += v0; a += v1; a += v2; a += v3; a += v4; a += v5; a += v6; a += v7;
-            }
-            acc = a;
+    };
+    Pre pa, pb;
+    prefetch(-1, pa);
+    if (64 < P.n_p) prefetch(63, pb);
+    int last = 0;  // summand buffer of the last chunk
+    for (int jb = 0; jb < P.n_p; jb += 128) {
+        chunk(jb, 0, pa);
+        last = 0;
+        if (jb + 64 < P.n_p) {
+            chunk(jb + 64, 1, pb);
+            last = 1;
         }
-        wave_sync();
     }
+    if (!COMP) ordered(last);
     if (COMP) {
         // one compensated reduction across the lanes per chain
 #pragma unroll
