@@ -14,6 +14,7 @@
 
 struct vx_ctx {
     int device = 0;
+    int n_cu = 256;  // compute units of the device (launch-shape choices)
     hipStream_t stream = nullptr;
     hipEvent_t ev[8] = {};
     std::string err;

@@ -114,16 +114,18 @@ __device__ __forceinline__ void add_two(double &acc, double &cmp, double v) {
 // addition, so every chain performs the reference's additions in its order.
 //
 // Latency: everything a chunk's summands read from global memory (the cell
-// values, the logarithm table, the nested flags) is loaded one chunk ahead
+// values, the logarithm table, the nested flags) is loaded two chunks ahead
 // into registers and staged in LDS, so the summands and the ordered sums read
-// LDS only and never wait on the next chunk's loads.  The ordered sums of
-// chunk k run in the same straight-line block as the summands of chunk k + 1
-// (summand buffers alternate), so the compiler fills the latency of each
-// dependent addition with the next chunk's arithmetic.
-#ifndef SCHED_FENCE
-#define SCHED_FENCE() __builtin_amdgcn_sched_barrier(0)
-#endif
-template <int RTB, int C, bool COMP>
+// LDS only and never wait on the next chunks' loads.
+//
+// FUSE (plain form only): the ordered sums of chunk k run in the same
+// straight-line block as the summands of chunk k + 1 (two summand buffers),
+// so the latency of each dependent addition is filled with the next chunk's
+// arithmetic; this holds many values in registers, and is the choice for
+// launches of at most one wavefront per SIMD.  Otherwise the summands and the
+// ordered sums of a chunk run one after the other through one buffer, in
+// fewer registers and less LDS, so that more wavefronts share each SIMD.
+template <int RTB, int C, bool COMP, bool FUSE>
 __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     constexpr int NCH = C * RTB;
     static_assert(NCH <= 32, "chains must fit in half a wavefront");
@@ -134,7 +136,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     __shared__ double S[4][NCH][68];   // sub-cell values, p index jb-1+e
     __shared__ double H[4][68];        // Gamma(p)+c1 p for the chain being prepared
     __shared__ double Lb[4][NCH][68];  // c2 log(h(j+1)/h(j)), p index jb-1+e
-    __shared__ double T[2][4][2][NCH][65];  // summands: buffer, wave, charge/current, chain, p
+    __shared__ double T[FUSE ? 2 : 1][4][2][NCH][65];  // summands: buffer, wave, charge/current, chain, p
     if (i0 >= P.n_x) return;  // uniform per wavefront
     const bool use_tab = A.tab[P.n_p + 2] != 0.0;  // uniform A^2: tabulated logarithms
     const int pitch = P.pitch;
@@ -233,7 +235,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             }
         }
     };
-    // plain form: the ordered sums of buffer b ^ 1 (the previous chunk) with
+    // FUSE: the ordered sums of buffer b ^ 1 (the previous chunk) with
     // the summands of chunk jb (into buffer b) placed between their groups of
     // eight additions, so that the additions' latency is filled with them
     auto fused = [&](int jb, int b, const int *nflag) {
@@ -264,7 +266,6 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             }
 #pragma unroll
             for (int q = 0; q < 8; q++) a += cur[q];
-            SCHED_FENCE();
         }
         acc = a;
     };
@@ -273,25 +274,18 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     auto ordered = [&](int b) {
         const int ch = (lane & 31) < NCH ? (lane & 31) : 0;
         const double *row = &T[b][wave][lane >> 5][ch][0];
-        double a = acc, v[16];
-#pragma unroll
-        for (int q = 0; q < 8; q++) v[q] = row[q];
+        double a = acc;
 #pragma unroll
         for (int e = 0; e < 64; e += 8) {
-            // the next group's reads go out before this group's additions
-            double *cur = &v[(e >> 3 & 1) * 8], *nxt = &v[(~e >> 3 & 1) * 8];
-            if (e + 8 < 64) {
-#pragma unroll
-                for (int q = 0; q < 8; q++) nxt[q] = row[e + 8 + q];
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) a += cur[q];
+            double v0 = row[e], v1 = row[e + 1], v2 = row[e + 2], v3 = row[e + 3];
+            double v4 = row[e + 4], v5 = row[e + 5], v6 = row[e + 6], v7 = row[e + 7];
+            a += v0; a += v1; a += v2; a += v3; a += v4; a += v5; a += v6; a += v7;
         }
         acc = a;
     };
-    if (!COMP) {
+    if (!COMP && FUSE) {
         // the ordered sums of the (absent) chunk before the first add -0.0
-        for (int e = lane; e < 2 * NCH * 65; e += 64) (&T[1][wave][0][0][0])[e] = -0.0;
+        for (int e = lane; e < 2 * NCH * 65; e += 64) (&T[FUSE ? 1 : 0][wave][0][0][0])[e] = -0.0;
     }
     // chunk jb from pr (summand buffer bb), whose registers then take chunk jb + 128
     auto chunk = [&](int jb, int bb, Pre &pr) {
@@ -321,8 +315,15 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             wave_sync();
         }
         wave_sync();
-        if (COMP) summands(jb, bb, nflag);
-        else fused(jb, bb, nflag);
+        if (COMP) {
+            summands(jb, bb, nflag);
+        } else if (FUSE) {
+            fused(jb, bb, nflag);
+        } else {
+            summands(jb, 0, nflag);
+            wave_sync();
+            ordered(0);
+        }
         wave_sync();
     };
     Pre pa, pb;
@@ -337,7 +338,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             last = 1;
         }
     }
-    if (!COMP) ordered(last);
+    if (!COMP && FUSE) ordered(last);
     if (COMP) {
         // one compensated reduction across the lanes per chain
 #pragma unroll
@@ -505,13 +506,18 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
     LaunchTimer lt(ctx, "k_moments");
     const unsigned g1 = (unsigned)((p->n_x + 15) / 16), g2 = (unsigned)((p->n_x + 7) / 8),
                    g4 = (unsigned)((p->n_x + 3) / 4);
-    switch (p->rtb * 2 + (comp ? 1 : 0)) {
-        case 2: k_moments<1, 4, false><<<g1, 256, 0, ctx->stream>>>(A); break;
-        case 3: k_moments<1, 4, true><<<g1, 256, 0, ctx->stream>>>(A); break;
-        case 4: k_moments<2, 2, false><<<g2, 256, 0, ctx->stream>>>(A); break;
-        case 5: k_moments<2, 2, true><<<g2, 256, 0, ctx->stream>>>(A); break;
-        case 8: k_moments<4, 1, false><<<g4, 256, 0, ctx->stream>>>(A); break;
-        case 9: k_moments<4, 1, true><<<g4, 256, 0, ctx->stream>>>(A); break;
+    // wavefronts of the launch (4 sub-columns each) against the device's SIMDs
+    const bool fuse = (size_t)(p->n_x * p->rtb + 3) / 4 <= (size_t)4 * ctx->n_cu;
+    switch (p->rtb * 4 + (comp ? 2 : 0) + (fuse ? 1 : 0)) {
+        case 4: k_moments<1, 4, false, false><<<g1, 256, 0, ctx->stream>>>(A); break;
+        case 5: k_moments<1, 4, false, true><<<g1, 256, 0, ctx->stream>>>(A); break;
+        case 6: case 7: k_moments<1, 4, true, false><<<g1, 256, 0, ctx->stream>>>(A); break;
+        case 8: k_moments<2, 2, false, false><<<g2, 256, 0, ctx->stream>>>(A); break;
+        case 9: k_moments<2, 2, false, true><<<g2, 256, 0, ctx->stream>>>(A); break;
+        case 10: case 11: k_moments<2, 2, true, false><<<g2, 256, 0, ctx->stream>>>(A); break;
+        case 16: k_moments<4, 1, false, false><<<g4, 256, 0, ctx->stream>>>(A); break;
+        case 17: k_moments<4, 1, false, true><<<g4, 256, 0, ctx->stream>>>(A); break;
+        case 18: case 19: k_moments<4, 1, true, false><<<g4, 256, 0, ctx->stream>>>(A); break;
         default: return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d not in {1,2,4}", p->rtb);
     }
     return check_launch(ctx, "k_moments");

@@ -23,6 +23,9 @@ int vx_ctx_create(int device, vx_ctx **out) {
             delete c;
             return VX_EHIP;
         }
+    int ncu = 0;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
+        c->n_cu = ncu;
     *out = c;
     return VX_OK;
 }
