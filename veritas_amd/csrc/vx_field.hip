@@ -329,54 +329,64 @@ __global__ __launch_bounds__(MT) void k_pm_conv(PoissonMB P) {
 
 // Register/LDS variant for n_f <= 4 * PT: thread t owns the CH consecutive
 // entries [t*CH, t*CH + CH); neighbours (residual, convolution) come from LDS.
+//
+// Barriers: a workgroup sum or scan exchanges the 16 wavefront partials through
+// LDS once (one barrier); every thread then forms the same sequential sums of
+// the partials itself.  Consecutive exchanges alternate between two partial
+// buffers, so a buffer is rewritten only after a later barrier has ordered
+// every read of it.
 constexpr int SMALL_MAX = 4 * PT;
 
-// exclusive scan of per-thread totals; returns this thread's offset
-__device__ double block_offsets(double tot, double *part) {
+struct SmallLds {
+    double v[SMALL_MAX];  // neighbour exchange: y for the convolution, x for the residual
+    double part[2][PW];   // wavefront partials, alternating
+    double edge[5];       // c at entries 0, 1, 2, N-2, N-1 (row 0 of the matrix)
+};
+
+// Workgroup exclusive scan of the per-thread totals (thread order), and the
+// workgroup total; every thread gets the same total bits.
+__device__ __forceinline__ double block_scan(double tot, double (&part)[2][PW], int &buf, double *total) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    double inc = wave_incl_scan(tot);
-    if (lane == 63) part[w] = inc;
+    const double inc = wave_incl_scan(tot);
+    if (lane == 63) part[buf][w] = inc;
     __syncthreads();
-    if (t == 0) {
-        double run = 0.0;
-        for (int k = 0; k < PW; k++) {
-            double v = part[k];
-            part[PW + k] = run;
-            run += v;
-        }
+    double off = 0.0, all = 0.0;
+#pragma unroll
+    for (int k = 0; k < PW; k++) {
+        const double v = part[buf][k];
+        if (k == w) off = all;
+        all += v;
     }
-    __syncthreads();
-    double off = part[PW + w] + (inc - tot);
-    __syncthreads();
-    return off;
+    buf ^= 1;
+    if (total) *total = all;
+    return off + (inc - tot);
 }
 
-__device__ double block_total(double v, double *part) {
+__device__ __forceinline__ double block_sum(double v, double (&part)[2][PW], int &buf) {
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     v = wave_sum(v);
-    if (lane == 0) part[w] = v;
+    if (lane == 0) part[buf][w] = v;
     __syncthreads();
-    if (t == 0) {
-        double tot = 0.0;
-        for (int k = 0; k < PW; k++) tot += part[k];
-        part[2 * PW] = tot;
-    }
-    __syncthreads();
-    double r = part[2 * PW];
-    __syncthreads();
-    return r;
+    double all = 0.0;
+#pragma unroll
+    for (int k = 0; k < PW; k++) all += part[buf][k];
+    buf ^= 1;
+    return all;
 }
 
+// One approximate solve of A out = in (the steps of the multi-workgroup
+// variant above): S = exclusive prefix of (-sum_{k>=1} in_k, in_1, in_2, ...),
+// D = mean(S) - S (D_0 = 0), y = inclusive prefix of D, c = M^-1 y - (M^-1 y)_0,
+// and c_0 from row 0 of the matrix.
 template <int CH>
-__device__ void approx_small(const PoissonArgs &P, const double *in, double *out, double *yb,
-                             double *part) {
+__device__ void approx_small(const PoissonArgs &P, const double *in, double *out, SmallLds &L, int &buf) {
     const int N = P.fs.n_f;
     const int base = threadIdx.x * CH;
     double loc = 0.0;
 #pragma unroll
     for (int k = 0; k < CH; k++)
         if (base + k >= 1 && base + k < N) loc += in[k];
-    double tail = block_total(loc, part);
+    const double tail = block_sum(loc, L.part, buf);
     double rp[CH], S[CH];
     double tot = 0.0;
 #pragma unroll
@@ -385,7 +395,7 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
         rp[k] = (i == 0) ? -tail : (i < N ? in[k] : 0.0);
         tot += rp[k];
     }
-    double run = block_offsets(tot, part);
+    double run = block_scan(tot, L.part, buf, nullptr);
     double ssum = 0.0;
 #pragma unroll
     for (int k = 0; k < CH; k++) {
@@ -393,7 +403,7 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
         run += rp[k];
         if (base + k < N) ssum += S[k];
     }
-    double d0 = block_total(ssum, part) * P.invN;
+    const double d0 = block_sum(ssum, L.part, buf) * P.invN;
     double D[CH];
     tot = 0.0;
 #pragma unroll
@@ -402,11 +412,11 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
         D[k] = (i == 0 || i >= N) ? 0.0 : d0 - S[k];
         tot += D[k];
     }
-    run = block_offsets(tot, part);
+    run = block_scan(tot, L.part, buf, nullptr);
 #pragma unroll
     for (int k = 0; k < CH; k++) {
         run += D[k];
-        if (base + k < N) yb[base + k] = run;
+        if (base + k < N) L.v[base + k] = run;
     }
     __syncthreads();
     // the thread's CH entries share one window of CH + 32 taps:
@@ -421,7 +431,7 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
             for (int q = 0; q < WN; q++) {
                 int u = base - (NG - 1) + q;
                 u = u < 0 ? u + N : (u >= N ? u - N : u);
-                w[q] = yb[u];
+                w[q] = L.v[u];
             }
         } else {
 #pragma unroll
@@ -429,7 +439,7 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
                 int u = base - (NG - 1) + q;
                 while (u >= N) u -= N;
                 while (u < 0) u += N;
-                w[q] = yb[u];
+                w[q] = L.v[u];
             }
         }
 #pragma unroll
@@ -443,32 +453,38 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
 #pragma unroll
         for (int k = 0; k < CH; k++) cp[k] = 0.0;
     }
-    __syncthreads();
-    if (threadIdx.x == 0) part[0] = cp[0];
-    __syncthreads();
-    double c0 = part[0];
-    __syncthreads();
+    // row 0 needs c at entries 0, 1, 2, N-2 and N-1
 #pragma unroll
     for (int k = 0; k < CH; k++) {
-        int i = base + k;
-        out[k] = (i > 0) ? cp[k] - c0 : cp[k];
-        if (i < N) yb[i] = out[k];
+        const int i = base + k;
+        if (i == 0) L.edge[0] = cp[k];
+        if (i == 1) L.edge[1] = cp[k];
+        if (i == 2) L.edge[2] = cp[k];
+        if (i == N - 2) L.edge[3] = cp[k];
+        if (i == N - 1) L.edge[4] = cp[k];
     }
     __syncthreads();
-    if (threadIdx.x == 0)
-        out[0] = in[0] - (((P.w1 * yb[1] + P.w2 * yb[2]) + P.w2 * yb[N - 2]) + P.w1 * yb[N - 1]);
-    __syncthreads();
+    const double c0 = L.edge[0];
+#pragma unroll
+    for (int k = 0; k < CH; k++) out[k] = (base + k > 0) ? cp[k] - c0 : cp[k];
+    if (threadIdx.x == 0) {
+        const double y1 = L.edge[1] - c0, y2 = L.edge[2] - c0, yn2 = L.edge[3] - c0, yn1 = L.edge[4] - c0;
+        out[0] = in[0] - (((P.w1 * y1 + P.w2 * y2) + P.w2 * yn2) + P.w1 * yn1);
+    }
 }
 
+// r = b - A x with a double-double accumulation of the reference's
+// coefficients (rows 1..N-1: the periodic stencil without column 0; row 0:
+// x_0 + w1 x_1 + w2 x_2 + w2 x_{N-2} + w1 x_{N-1})
 template <int CH>
-__device__ void residual_small(const PoissonArgs &P, const double *b, const double *x, double *r,
-                               double *xb) {
+__device__ void residual_small(const PoissonArgs &P, const double *b, const double *x, double *r, SmallLds &L) {
     const int N = P.fs.n_f;
     const int base = threadIdx.x * CH;
 #pragma unroll
     for (int k = 0; k < CH; k++)
-        if (base + k < N) xb[base + k] = x[k];
+        if (base + k < N) L.v[base + k] = x[k];
     __syncthreads();
+    const double *xb = L.v;
 #pragma unroll
     for (int k = 0; k < CH; k++) {
         int i = base + k;
@@ -499,7 +515,6 @@ __device__ void residual_small(const PoissonArgs &P, const double *b, const doub
         }
         r[k] = sh + sl;
     }
-    __syncthreads();
 }
 
 template <int CH>
@@ -507,9 +522,8 @@ __global__ __launch_bounds__(PT) void k_poisson_small(PoissonArgs P) {
     const vx_field_state &F = P.fs;
     const int N = F.n_f;
     const int t = threadIdx.x, base = t * CH;
-    __shared__ double wb[SMALL_MAX];  // neighbour exchange (residual and convolution phases never overlap)
-    __shared__ double part[2 * PW + 2];
-    double *xb = wb, *yb = wb;
+    __shared__ SmallLds L;
+    int buf = 0;
     double b[CH], x[CH], r[CH], c[CH];
     const double te = vx_eps0_inv;
 #pragma unroll
@@ -522,20 +536,26 @@ __global__ __launch_bounds__(PT) void k_poisson_small(PoissonArgs P) {
         }
         b[k] = v;
     }
-    approx_small<CH>(P, b, x, yb, part);
+    approx_small<CH>(P, b, x, L, buf);
     for (int it = 0; it < 3; it++) {
-        residual_small<CH>(P, b, x, r, xb);
-        approx_small<CH>(P, r, c, yb, part);
+        // the previous solve's last barrier ordered its reads of L.v
+        residual_small<CH>(P, b, x, r, L);
+        // the first barrier of the solve below orders the residual's reads of
+        // L.v before the solve writes it
+        approx_small<CH>(P, r, c, L, buf);
 #pragma unroll
         for (int k = 0; k < CH; k++) x[k] = x[k] + c[k];
     }
+    // E from the final potential: every thread's reads of L.v in the last
+    // solve came before its final barrier
 #pragma unroll
     for (int k = 0; k < CH; k++)
         if (base + k < N) {
-            xb[base + k] = x[k];
+            L.v[base + k] = x[k];
             F.phi[base + k] = x[k];
         }
     __syncthreads();
+    const double *xb = L.v;
     double ex0 = *F.Ex0;
     double e_m1 = efield(xb, N, F.field_coef, ex0, -1);
     double e_0 = efield(xb, N, F.field_coef, ex0, 0);

@@ -39,6 +39,8 @@ struct MomArgs {
     // then depend on p only), tab[t] = c2 log(h(t)/h(t-1)) for t in [0, n_p+2)
     // with h(j) = Gamma(p_j) + c1 p_j, and tab[n_p+2] = 1; otherwise
     // tab[n_p+2] = 0 and the kernel evaluates them per column group.
+    // tab[n_p+3 .. n_p+3+TAB_KEY) holds what the logarithms were formed from
+    // (A^2, the p grid, the constants), so an unchanged table is not rebuilt.
     const double *tab;
 };
 
@@ -50,6 +52,8 @@ __device__ __forceinline__ double cell_a2(const vx_fields &fl, long i) {
     double ay = fl.Ay1[i], az = fl.Az1[i];
     return (ay * ay) + (az * az);
 }
+
+constexpr int TAB_KEY = 8;  // A^2, pmin, dp, p_pos, mc2inv, c1, c2, n_p
 
 // Logarithm table of one patch (see MomArgs::tab); one workgroup.
 template <int RTB>
@@ -65,11 +69,19 @@ __global__ __launch_bounds__(1024) void k_moment_table(MomArgs A, double *tab) {
         if (t == 0) tab[P.n_p + 2] = 0.0;
         return;
     }
+    const double key[TAB_KEY] = {a0, P.pmin, P.dp, (double)P.p_pos, P.mc2inv, A.c1, A.c2, (double)P.n_p};
+    double *kt = tab + P.n_p + 3;
+    bool hit = tab[P.n_p + 2] == 1.0;
+    for (int k = 0; k < TAB_KEY; k++) hit = hit && kt[k] == key[k];
+    if (__syncthreads_or(hit ? 1 : 0) != 0) return;  // the same table is in place
+    // (every thread read the same bits; the barrier also orders those reads
+    // before the writes below)
     auto h = [&](int j) {
         double pm = momentum(P.pmin, P.dp, j, P.p_pos);
         return gamma_(pm, a0, P.mc2inv) + A.c1 * pm;
     };
     for (int e = t; e < P.n_p + 2; e += blockDim.x) tab[e] = A.c2 * vx_log(h(e) / h(e - 1));
+    if (t < TAB_KEY) kt[t] = key[t];
     if (t == 0) tab[P.n_p + 2] = 1.0;
 }
 
@@ -484,7 +496,7 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
     if ((rc = validate_span(ctx, p, fl->n_f))) return rc;
     MomArgs A;
     fill_args(A, p, fl);
-    const size_t need = (size_t)p->n_p + 3;
+    const size_t need = (size_t)p->n_p + 3 + TAB_KEY;
 
     if (ctx->mom_table_n < need) {
         if (ctx->capturing) return fail(ctx, VX_EINVAL, "moments: table growth while a step graph is recorded");
@@ -493,6 +505,9 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
         ctx->mom_table_n = 0;
         if (hipMalloc(&ctx->mom_table, sizeof(double) * need) != hipSuccess)
             return fail(ctx, VX_ENOMEM, "moments: logarithm table");
+        // no table in place yet (flag 0)
+        if (hipMemsetAsync(ctx->mom_table, 0, sizeof(double) * need, ctx->stream) != hipSuccess)
+            return fail(ctx, VX_EHIP, "moments: logarithm table clear");
         ctx->mom_table_n = need;
     }
     A.tab = ctx->mom_table;
