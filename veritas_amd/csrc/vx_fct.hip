@@ -231,11 +231,8 @@ __device__ __forceinline__ void lo_predictor(const vx_patch &P, double aSum, int
     P.f[2][pidx(pitch, i, j)] = t;
 }
 
-// Zalesak R+- for i in [-1, n_x+1), j in [-1, n_p+1) (Rectangle.cpp:1538-1579)
-__device__ __forceinline__ void limiter_r(const vx_patch &P, int bx, int by) {
-    int j = -1 + bx * 64 + (int)(threadIdx.x & 63);
-    int i = -1 + by * 4 + (int)(threadIdx.x >> 6);
-    if (i >= P.n_x + 1 || j >= P.n_p + 1) return;
+// Zalesak R+- of cell (i, j) (Rectangle.cpp:1538-1579)
+__device__ __forceinline__ void ratios(const vx_patch &P, int i, int j, double &rp, double &rm) {
     const int pitch = P.pitch;
     const double *FxDS = P.FxDS, *FpDS = P.FpDS, *f0 = P.f[0], *f2 = P.f[2];
     long idx = pidx(pitch, i, j);
@@ -254,23 +251,49 @@ __device__ __forceinline__ void limiter_r(const vx_patch &P, int bx, int by) {
                   valmin(valmin(d0, d2), valmin(e0, e2)))));
     double Qm = -wMin + a2;
     double Qp = wMax - a2;
-    P.Rp[idx] = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
-    P.Rm[idx] = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
+    rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
+    rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
 }
 
-// C on the NS range: 1 everywhere, the limiter formula on interior faces
-// i in [1, n_x), j in [0, n_p) (Rectangle.cpp:1581-1594)
-__device__ __forceinline__ void limiter_c(const vx_patch &P, int bx, int by) {
-    int j = -2 + bx * 64 + (int)(threadIdx.x & 63);
-    int i = -2 + by * 4 + (int)(threadIdx.x >> 6);
-    if (i >= P.n_x + 2 || j >= P.n_p + 2) return;
-    const int pitch = P.pitch;
-    long idx = pidx(pitch, i, j);
+// Both limiter passes in one: R+- on [-1, n_x+1) x [-1, n_p+1), then C on the
+// NS range [-2, n_x+2) x [-2, n_p+2): 1 everywhere, the limiter formula on
+// interior faces i in [1, n_x), j in [0, n_p) (Rectangle.cpp:1538-1594).
+// Workgroup (bx, by) forms R on an LR x 64 block of cells (one per thread,
+// rows i0 .., columns j0 ..) in LDS and owns its last LR - 1 rows and 63
+// columns: it stores R there (the cross-patch C sync reads R) and forms C
+// there from the LDS block, whose first row and column are the halo.  Blocks
+// overlap by one row and one column; an overlapped cell repeats its owner's
+// arithmetic, so both hold the same bits.
+constexpr int LR = 16;  // rows of a limiter block (LR x 64 threads)
+
+__host__ __device__ inline int lim_gx(int n_p) { return (n_p + 4 + 62) / 63; }
+__host__ __device__ inline int lim_gy(int n_x) { return (n_x + 4 + LR - 2) / (LR - 1); }
+
+__device__ __forceinline__ void limiter_rc(const vx_patch &P, int bx, int by) {
+    __shared__ double sRp[LR][64], sRm[LR][64];
+    const int i0 = -3 + by * (LR - 1), j0 = -3 + bx * 63;
+    const int t = (int)threadIdx.x, r = t >> 6, c = t & 63;
+    const int i = i0 + r, j = j0 + c;
+    const bool own = r >= 1 && c >= 1;
+    if (i >= -1 && i < P.n_x + 1 && j >= -1 && j < P.n_p + 1) {
+        double rp, rm;
+        ratios(P, i, j, rp, rm);
+        sRp[r][c] = rp;
+        sRm[r][c] = rm;
+        if (own) {
+            const long idx = pidx(P.pitch, i, j);
+            P.Rp[idx] = rp;
+            P.Rm[idx] = rm;
+        }
+    }
+    __syncthreads();
+    if (!own || i >= P.n_x + 2 || j >= P.n_p + 2) return;
+    const long idx = pidx(P.pitch, i, j);
     double cx = 1.0, cp = 1.0;
     if (i >= 1 && i < P.n_x && j >= 0 && j < P.n_p) {
-        long idxi = pidx(pitch, i - 1, j), idxj = pidx(pitch, i, j - 1);
-        cx = P.FxDS[idx] > 0.0 ? valmin(P.Rp[idx], P.Rm[idxi]) : valmin(P.Rp[idxi], P.Rm[idx]);
-        cp = P.FpDS[idx] > 0.0 ? valmin(P.Rp[idx], P.Rm[idxj]) : valmin(P.Rp[idxj], P.Rm[idx]);
+        // R(i, j) = s[r][c], R(i-1, j) = s[r-1][c], R(i, j-1) = s[r][c-1]
+        cx = P.FxDS[idx] > 0.0 ? valmin(sRp[r][c], sRm[r - 1][c]) : valmin(sRp[r - 1][c], sRm[r][c]);
+        cp = P.FpDS[idx] > 0.0 ? valmin(sRp[r][c], sRm[r][c - 1]) : valmin(sRp[r][c - 1], sRm[r][c]);
     }
     P.Cx[idx] = cx;
     P.Cp[idx] = cp;
@@ -304,8 +327,7 @@ __device__ __forceinline__ void corrector(const vx_patch &P, int bx, int by) {
 __global__ __launch_bounds__(256) void k_lo_predictor(vx_patch P, double aSum) {
     lo_predictor(P, aSum, (int)blockIdx.x, (int)blockIdx.y);
 }
-__global__ __launch_bounds__(256) void k_limiter_r(vx_patch P) { limiter_r(P, (int)blockIdx.x, (int)blockIdx.y); }
-__global__ __launch_bounds__(256) void k_limiter_c(vx_patch P) { limiter_c(P, (int)blockIdx.x, (int)blockIdx.y); }
+__global__ __launch_bounds__(LR * 64) void k_limiter_rc(vx_patch P) { limiter_rc(P, (int)blockIdx.x, (int)blockIdx.y); }
 __global__ __launch_bounds__(256) void k_corrector(vx_patch P) { corrector(P, (int)blockIdx.x, (int)blockIdx.y); }
 
 // Work-list forms: workgroup w of the launch is workgroup w - start of patch k,
@@ -327,15 +349,12 @@ __global__ __launch_bounds__(256) void k_lo_predictor_batch(const vx_patch *tab,
     batch_cell<1, 0>(bi, n, k, bx, by, tab);
     lo_predictor(tab[k], aSum, bx, by);
 }
-__global__ __launch_bounds__(256) void k_limiter_r_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
-    int k, bx, by;
-    batch_cell<2, 2>(bi, n, k, bx, by, tab);
-    limiter_r(tab[k], bx, by);
-}
-__global__ __launch_bounds__(256) void k_limiter_c_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
-    int k, bx, by;
-    batch_cell<3, 4>(bi, n, k, bx, by, tab);
-    limiter_c(tab[k], bx, by);
+__global__ __launch_bounds__(LR * 64) void k_limiter_rc_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
+    const int w = (int)blockIdx.x;
+    const int k = batch_patch<2>(bi, n, w);
+    const int local = w - bi[k].start[2];
+    const int gx = lim_gx(tab[k].n_p);
+    limiter_rc(tab[k], local % gx, local / gx);
 }
 __global__ __launch_bounds__(256) void k_corrector_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
     int k, bx, by;
@@ -420,11 +439,10 @@ int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
         b.TI = T.TI;
         b.nstrip = T.nstrip;
         b.nbx = T.nbx;
-        dim3 g0 = grid_cells(p.n_x, p.n_p, 0, 0), g2 = grid_cells(p.n_x, p.n_p, 2, 2),
-             g4 = grid_cells(p.n_x, p.n_p, 4, 4);
+        dim3 g0 = grid_cells(p.n_x, p.n_p, 0, 0), g4 = grid_cells(p.n_x, p.n_p, 4, 4);
         run[0] += T.nbx * ((T.nstrip + 3) / 4);
         run[1] += (int)(g0.x * g0.y);
-        run[2] += (int)(g2.x * g2.y);
+        run[2] += lim_gx(p.n_p) * lim_gy(p.n_x);
         run[3] += (int)(g4.x * g4.y);
         run[4] += (int)(((long)(p.n_x + 4) * p.pitch + 255) / 256);
     }
@@ -483,11 +501,8 @@ int vx_fct_sub0(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fi
 int vx_fct_sub1(vx_ctx *ctx, const vx_patch *p) {
     int rc = validate_patch(ctx, p);
     if (rc) return rc;
-    k_limiter_r<<<grid_cells(p->n_x, p->n_p, 2, 2), 256, 0, ctx->stream>>>(*p);
-    rc = check_launch(ctx, "k_limiter_r");
-    if (rc) return rc;
-    k_limiter_c<<<grid_cells(p->n_x, p->n_p, 4, 4), 256, 0, ctx->stream>>>(*p);
-    return check_launch(ctx, "k_limiter_c");
+    k_limiter_rc<<<dim3((unsigned)lim_gx(p->n_p), (unsigned)lim_gy(p->n_x)), LR * 64, 0, ctx->stream>>>(*p);
+    return check_launch(ctx, "k_limiter_rc");
 }
 
 int vx_fct_sub2(vx_ctx *ctx, const vx_patch *p) {
@@ -568,10 +583,8 @@ int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double
 int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
     int rc = table_batch(ctx, t, n_f);
     if (rc) return rc;
-    k_limiter_r_batch<<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n);
-    if ((rc = check_launch(ctx, "k_limiter_r_batch"))) return rc;
-    k_limiter_c_batch<<<(unsigned)t->batch[t->n].start[3], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n);
-    return check_launch(ctx, "k_limiter_c_batch");
+    k_limiter_rc_batch<<<(unsigned)t->batch[t->n].start[2], LR * 64, 0, ctx->stream>>>(t->d, t->batch_d, t->n);
+    return check_launch(ctx, "k_limiter_rc_batch");
 }
 
 int vx_fct_sub2_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
