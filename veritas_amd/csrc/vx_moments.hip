@@ -21,6 +21,7 @@ using namespace vxd;
 namespace {
 
 constexpr int MAX_RTB = 4;  // coarsest level of a 3-level hierarchy at ratio 2
+constexpr int TAB_KEY = 8;  // A^2, pmin, dp, p_pos, mc2inv, c1, c2, n_p
 
 // Rectangle.cpp:76-89
 __constant__ double kInterp[12] = {
@@ -35,12 +36,16 @@ struct MomArgs {
     double inv_rtb;       // 1.0 / relativeToBottom
     double dpq, jscale;   // dp*q and -q*q/mass
     double rel[3 * MAX_RTB];  // interpolatCoefsREL (Rectangle.cpp:103-110)
-    // When every sub-column of the patch has the same A^2 (the logarithm terms
-    // then depend on p only), tab[t] = c2 log(h(t)/h(t-1)) for t in [0, n_p+2)
-    // with h(j) = Gamma(p_j) + c1 p_j, and tab[n_p+2] = 1; otherwise
-    // tab[n_p+2] = 0 and the kernel evaluates them per column group.
-    // tab[n_p+3 .. n_p+3+TAB_KEY) holds what the logarithms were formed from
-    // (A^2, the p grid, the constants), so an unchanged table is not rebuilt.
+    // Two tables of logarithm terms, each for one value a0 of A^2: table 0 for
+    // the patch's first sub-column (every column when A^2 is uniform), table 1
+    // for A^2 = 0 (the plasma ahead of a laser pulse).  Table q starts at
+    // tab + q (n_p + 3 + TAB_KEY): entries t in [0, n_p+2) hold
+    // c2 log(h(t)/h(t-1)) with h(j) = Gamma(p_j, a0) + c1 p_j, entry n_p+2 is 1
+    // once built, and the TAB_KEY entries after it what they were formed from
+    // (a0 first, then the p grid and the constants), so an unchanged table is
+    // not rebuilt.  A wavefront whose chains all have A^2 == a0 of a table
+    // reads it (the same bits it would form itself); the others form their
+    // logarithms per distinct A^2.
     const double *tab;
 };
 
@@ -53,22 +58,16 @@ __device__ __forceinline__ double cell_a2(const vx_fields &fl, long i) {
     return (ay * ay) + (az * az);
 }
 
-constexpr int TAB_KEY = 8;  // A^2, pmin, dp, p_pos, mc2inv, c1, c2, n_p
 
-// Logarithm table of one patch (see MomArgs::tab); one workgroup.
+__host__ __device__ inline long tab_stride(int n_p) { return (long)n_p + 3 + TAB_KEY; }
+
+// Logarithm tables of one patch (see MomArgs::tab); workgroup q builds table q.
 template <int RTB>
 __global__ __launch_bounds__(1024) void k_moment_table(MomArgs A, double *tab) {
     const vx_patch &P = A.p;
     const int t = threadIdx.x;
-    const double a0 = P.q * P.q * cell_a2(A.fl, (long)P.x_pos * RTB);
-    int same = 1;
-    for (int s = t; s < P.n_x * RTB; s += blockDim.x)
-        same &= (P.q * P.q * cell_a2(A.fl, (long)P.x_pos * RTB + s)) == a0;
-    same = __syncthreads_and(same);
-    if (!same) {
-        if (t == 0) tab[P.n_p + 2] = 0.0;
-        return;
-    }
+    tab += blockIdx.x * tab_stride(P.n_p);
+    const double a0 = blockIdx.x == 0 ? P.q * P.q * cell_a2(A.fl, (long)P.x_pos * RTB) : 0.0;
     const double key[TAB_KEY] = {a0, P.pmin, P.dp, (double)P.p_pos, P.mc2inv, A.c1, A.c2, (double)P.n_p};
     double *kt = tab + P.n_p + 3;
     bool hit = tab[P.n_p + 2] == 1.0;
@@ -150,7 +149,6 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     __shared__ double Lb[4][NCH][68];  // c2 log(h(j+1)/h(j)), p index jb-1+e
     __shared__ double T[FUSE ? 2 : 1][4][2][NCH][65];  // summands: buffer, wave, charge/current, chain, p
     if (i0 >= P.n_x) return;  // uniform per wavefront
-    const bool use_tab = A.tab[P.n_p + 2] != 0.0;  // uniform A^2: tabulated logarithms
     const int pitch = P.pitch;
     const double *g = P.f[1];
     double asq[NCH];
@@ -159,6 +157,15 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         int i = i0 + c < P.n_x ? i0 + c : P.n_x - 1;
         asq[ch] = P.q * P.q * cell_a2(A.fl, (long)(i + P.x_pos) * RTB + k);
     }
+    // every chain of this wavefront has the A^2 of one of the tables: tabulated logarithms
+    const double *tab = nullptr;
+    for (int q = 1; q >= 0 && !tab; q--) {
+        const double *tq = A.tab + q * tab_stride(P.n_p);
+        bool all = tq[P.n_p + 2] != 0.0;
+        for (int ch = 0; ch < NCH; ch++) all = all && asq[ch] == tq[P.n_p + 3];
+        if (all) tab = tq;
+    }
+    const bool use_tab = tab != nullptr;
     double acc = 0.0;  // lanes < NCH: charge chain; lanes 32..32+NCH-1: current chain
     // COMP: per chain, this lane's running sum over its p-cells and its compensation
     double accR[NCH], cmpR[NCH], accJ[NCH], cmpJ[NCH];
@@ -214,8 +221,8 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         }
         if (use_tab) {
             const int t0 = min(j0 + 1 + lane, P.n_p + 2), t1 = min(j0 + 65 + lane, P.n_p + 2);
-            pr.t[0] = A.tab[t0];
-            pr.t[1] = lane < 2 ? A.tab[t1] : 0.0;
+            pr.t[0] = tab[t0];
+            pr.t[1] = lane < 2 ? tab[t1] : 0.0;
         }
         if (P.nested) {
             const int jn = min(j0 + 1 + lane, P.n_p - 1);
@@ -496,7 +503,7 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
     if ((rc = validate_span(ctx, p, fl->n_f))) return rc;
     MomArgs A;
     fill_args(A, p, fl);
-    const size_t need = (size_t)p->n_p + 3 + TAB_KEY;
+    const size_t need = 2 * (size_t)tab_stride(p->n_p);
 
     if (ctx->mom_table_n < need) {
         if (ctx->capturing) return fail(ctx, VX_EINVAL, "moments: table growth while a step graph is recorded");
@@ -512,9 +519,9 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
     }
     A.tab = ctx->mom_table;
     switch (p->rtb) {
-        case 1: k_moment_table<1><<<1, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
-        case 2: k_moment_table<2><<<1, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
-        case 4: k_moment_table<4><<<1, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
+        case 1: k_moment_table<1><<<2, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
+        case 2: k_moment_table<2><<<2, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
+        case 4: k_moment_table<4><<<2, 1024, 0, ctx->stream>>>(A, ctx->mom_table); break;
         default: return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d not in {1,2,4}", p->rtb);
     }
     if ((rc = check_launch(ctx, "k_moment_table"))) return rc;
