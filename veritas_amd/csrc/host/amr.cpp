@@ -622,9 +622,10 @@ void Rectangle::UploadNested() {
 
 // ---------------------------------------------------------------- Level programs
 void Level::FreePrograms() {
-    for (auto &p : prog) {
-        vx_program_destroy(p);
-        p = nullptr;
+    for (int k = 0; k < 7; k++) {
+        vx_program_destroy(prog[k]);
+        prog[k] = nullptr;
+        prog_built[k] = false;
     }
     vx_program_destroy(flux_prog);
     flux_prog = nullptr;
@@ -632,9 +633,34 @@ void Level::FreePrograms() {
 
 Level::~Level() { FreePrograms(); }
 
-// Level.cpp:88-126
+// tasks of update type `type` (Level.cpp:88-126) of every patch of level L, appended to T
+static void emit_level(TaskList &T, Level &L, int type, int r) {
+    GhostEmitter g{T, r};
+    CEmitter c{T, r};
+    for (auto &R : L.rectangles) {
+        switch (type) {
+            case 0: g.Interior(*R); break;
+            case 1: g.SameLevel(*R); break;
+            case 2: g.DifferentLevel(*R); break;
+            case 3: g.Corners(*R); break;
+            case 4: c.SameBoundary(VX_OP_C_SAME, *R); break;
+            case 5: c.DifferentBoundary(*R); break;
+            case 6: c.SameBoundary(VX_OP_C_UPDATE, *R); break;
+        }
+    }
+}
+
+// Level.cpp:88-126 (the mesh runs its merged programs; a direct call builds
+// this level's program of the type on first use)
 void Level::PushData(int updateType, int val) {
     if (updateType < 0 || updateType > 6) throw Error("Level::PushData: bad update type");
+    if (!table || !dev) throw Error("Level::PushData: level not attached to a mesh");
+    if (!prog_built[updateType]) {
+        TaskList T;
+        emit_level(T, *this, updateType, settings.refinementRatio);
+        prog[updateType] = T.upload(*dev);
+        prog_built[updateType] = true;
+    }
     vx_program *p = prog[updateType];
     if (!p) return;  // nothing to do on this level
     if (updateType <= 3)
@@ -796,6 +822,9 @@ void Mesh::promoteHierarchyToMesh(bool init) {
 
 void Mesh::FreePrograms() {
     for (auto &l : levels) l->FreePrograms();
+    vx_program_destroy(push_prog);
+    vx_program_destroy(pushc_prog);
+    push_prog = pushc_prog = nullptr;
     vx_patch_table_destroy(table);
     table = nullptr;
 }
@@ -821,27 +850,29 @@ void Mesh::BuildPrograms() {
     for (Rectangle *r : order) d.push_back(r->desc());
     dev->check(vx_patch_table_create(dev->ctx, d.data(), (int32_t)d.size(), &table), "patch table");
     const int r = settings.refinementRatio;
+    // Mesh::PushData's order of level programs (Mesh.cpp:91-106) as one task
+    // list, and PushBoundaryC's (Mesh.cpp:904-917): a task joins the current
+    // batch only when it touches no cell that the batch reads or writes, so
+    // the merged program applies every task in the sequential order's effect
+    {
+        TaskList P;
+        emit_level(P, *levels[0], 1, r);
+        for (unsigned i = 1; i < levels.size(); i++) {
+            emit_level(P, *levels[i], 0, r);
+            emit_level(P, *levels[i], 1, r);
+        }
+        emit_level(P, *levels.back(), 3, r);
+        for (unsigned i = levels.size() - 1; i > 0; i--) emit_level(P, *levels[i - 1], 2, r);
+        push_prog = P.upload(*dev);
+        TaskList C;
+        for (int type = 4; type <= 6; type++)
+            for (auto &l : levels) emit_level(C, *l, type, r);
+        pushc_prog = C.upload(*dev);
+    }
     for (auto &lp : levels) {
         Level &L = *lp;
         L.dev = dev.get();
         L.table = table;
-        for (int type = 0; type < 7; type++) {
-            TaskList T;
-            GhostEmitter g{T, r};
-            CEmitter c{T, r};
-            for (auto &R : L.rectangles) {
-                switch (type) {
-                    case 0: g.Interior(*R); break;
-                    case 1: g.SameLevel(*R); break;
-                    case 2: g.DifferentLevel(*R); break;
-                    case 3: g.Corners(*R); break;
-                    case 4: c.SameBoundary(VX_OP_C_SAME, *R); break;
-                    case 5: c.DifferentBoundary(*R); break;
-                    case 6: c.SameBoundary(VX_OP_C_UPDATE, *R); break;
-                }
-            }
-            L.prog[type] = T.upload(*dev);
-        }
         // coarse faces on a finer patch's edge (Rectangle.cpp:1313-1394)
         TaskList F;
         for (auto &K : L.rectangles) {

@@ -403,24 +403,22 @@ void Mesh::Advance(double timeStep, int step) {
     }
 }
 
-// Mesh.cpp:91-106
+// Mesh.cpp:91-106: level 0's type 1, then types 0 and 1 of the coarser levels,
+// the coarsest level's type 3, and type 2 from the second-coarsest level to
+// the finest, run as one merged program (built in BuildPrograms)
 void Mesh::PushData(int val) {
     EnsureTable();
-    levels[0]->PushData(1, val);
-    for (unsigned i = 1; i < levels.size(); i++) {
-        levels[i]->PushData(0, val);
-        levels[i]->PushData(1, val);
-    }
-    levels[levels.size() - 1]->PushData(3, val);
-    for (unsigned i = levels.size() - 1; i > 0; i--) levels[i - 1]->PushData(2, val);
+    if (push_prog)
+        dev->check(vx_program_run(dev->ctx, table, push_prog, VX_MODE_GHOST, val, settings.refinementRatio),
+                   "push data");
 }
 
-// Mesh.cpp:904-917
+// Mesh.cpp:904-917: types 4, 5 and 6 over every level, one merged program
 void Mesh::PushBoundaryC() {
     EnsureTable();
-    for (auto &l : levels) l->PushData(4);
-    for (auto &l : levels) l->PushData(5);
-    for (auto &l : levels) l->PushData(6);
+    if (pushc_prog)
+        dev->check(vx_program_run(dev->ctx, table, pushc_prog, VX_MODE_GHOST, 1, settings.refinementRatio),
+                   "limiter sync");
 }
 
 // ---------------------------------------------------------------- EMFieldSolver

@@ -187,6 +187,7 @@ public:
     Device *dev = nullptr;
     vx_patch_table *table = nullptr;
     vx_program *prog[7] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    bool prog_built[7] = {false, false, false, false, false, false, false};  // built on first use
     vx_program *flux_prog = nullptr;  // coarse-fine faces of this level's patches
     void FreePrograms();
 
@@ -228,6 +229,9 @@ private:
     // device patch table and task programs of the current hierarchy (host/amr.cpp)
     vx_patch_table *table = nullptr;
     std::vector<Rectangle *> order;  // patches in table order
+    // PushData's and PushBoundaryC's sequences of level programs, each merged
+    // into one program (batches span levels and types where no cell conflicts)
+    vx_program *push_prog = nullptr, *pushc_prog = nullptr;
     void promoteHierarchyToMesh(bool init);      // Mesh.cpp:794-875
     void InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &levels_n);  // Mesh.cpp:116-130
     void BuildPrograms();
