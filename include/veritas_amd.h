@@ -144,11 +144,11 @@ typedef struct vx_stage_item {
  * on that stream (vx_ctx_wait).  No counterpart in the reference: the
  * serving loop packs the patches of concurrent runs into shared launches. */
 int vx_fct_stage_fused_batch(vx_ctx *ctx, const vx_stage_item *items, int n, int step);
-/* Rows per strip that vx_fct_stage_fused uses at RK stage `step` for patches
- * of n_x x n_p cells (timed on the first call for that shape and stage; 0
- * before it).  No counterpart in the reference: it reports the tile shape the
- * library chose. */
-int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step);
+/* Rows per strip that the fused stage uses at RK stage `step` for patches of
+ * n_x x n_p cells, in FP64 (mixed = 0) or mixed precision (mixed = 1); timed
+ * on the first call for that shape, stage and precision, 0 before it.  No
+ * counterpart in the reference: it reports the tile shape the library chose. */
+int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed);
 int vx_fct_stage_fused_available(void);
 
 /* Ghost fill of slot `slot` of a patch whose neighbours are all the domain-edge

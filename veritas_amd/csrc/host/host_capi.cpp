@@ -90,7 +90,8 @@ int vxh_graph_stats(void *h, int *out) {
 
 // strip height chosen for the fused stage of an n_x x n_p patch (0: not yet chosen)
 int vxh_tile_rows(void *h, int n_x, int n_p, int step) {
-    return vx_stage_tile_rows(((Run *)h)->sm->dev->ctx, n_x, n_p, step);
+    Run *r = (Run *)h;
+    return vx_stage_tile_rows(r->sm->dev->ctx, n_x, n_p, step, r->sm->meshes[0]->mixed ? 1 : 0);
 }
 
 int vxh_advance(void *h, double dt, int nsteps) {

@@ -652,9 +652,9 @@ int vx_fct_stage_fused_batch(vx_ctx *ctx, const vx_stage_item *items, int n, int
     return rc ? rc : batch_launch(ctx, mixed, step, true);
 }
 
-int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step) {
+int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed) {
     for (auto &c : ctx->ti_cache)
-        if (c[0] == n_x && c[1] == n_p && c[3] == 0 && c[4] == step) return c[2];
+        if (c[0] == n_x && c[1] == n_p && c[3] == (mixed ? 1 : 0) && c[4] == step) return c[2];
     return 0;
 }
 
