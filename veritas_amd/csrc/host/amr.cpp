@@ -9,6 +9,10 @@
 // (one launch) starts whenever a task would read or write a cell written by,
 // or write a cell read by, an earlier task of the current batch, so the
 // parallel batches reproduce the sequential order of the reference exactly.
+// The regrid transfer needs no tracking: its phases (interpolation into a new
+// level, copies from the old patches of the same level, the missing cells)
+// each write every cell at most once and read only cells that earlier phases
+// wrote, so one batch per phase keeps the order.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
@@ -33,32 +37,73 @@ uint64_t cell_key(int patch, int arr, int i, int j) {
 struct TaskList {
     std::vector<vx_cell_task> tasks;
     std::vector<int32_t> starts{0};
+    // Conflicts with the current batch (number `cur`): f cells of a patch in
+    // dense stamp arrays over [-4, n_x+4) x [-4, n_p+4) (a cell is in the
+    // batch's read or write set when its stamp equals `cur`, so a new batch
+    // clears them by incrementing `cur`); limiter-C cells, and any cell outside
+    // that window, in hash sets.  Without shapes every cell uses the sets.
+    struct Stamps {
+        std::vector<int32_t> rd, wr;
+    };
+    std::vector<std::pair<int, int>> shape;  // (n_x, n_p) per table index
+    std::vector<Stamps> stamps;              // per table index, allocated on first use
+    int32_t cur = 1;
     std::unordered_set<uint64_t> rd, wr;
-    std::vector<uint64_t> r_, w_;
+    std::vector<uint64_t> r_, w_;  // hashed cells of the task being recorded
 
+    void set_shapes(const std::vector<Rectangle *> &order) {
+        shape.clear();
+        for (Rectangle *x : order) shape.push_back({x->n_x, x->n_p});
+        stamps.assign(shape.size(), Stamps{});
+    }
+    int32_t *slot(int patch, int arr, int i, int j, bool write) {
+        if (arr != ARR_F || patch < 0 || patch >= (int)shape.size()) return nullptr;
+        const int nx = shape[patch].first, np = shape[patch].second;
+        if (i < -4 || i >= nx + 4 || j < -4 || j >= np + 4) return nullptr;
+        Stamps &S = stamps[patch];
+        if (S.rd.empty()) {
+            S.rd.assign((size_t)(nx + 8) * (np + 8), 0);
+            S.wr.assign((size_t)(nx + 8) * (np + 8), 0);
+        }
+        const size_t k = (size_t)(i + 4) * (np + 8) + (j + 4);
+        return write ? &S.wr[k] : &S.rd[k];
+    }
     void begin() {
         r_.clear();
         w_.clear();
+        rd_cells_.clear();
+        wr_cells_.clear();
     }
-    void read(int patch, int arr, int i, int j) { r_.push_back(cell_key(patch, arr, i, j)); }
-    void write(int patch, int arr, int i, int j) { w_.push_back(cell_key(patch, arr, i, j)); }
+    void read(int patch, int arr, int i, int j) {
+        if (slot(patch, arr, i, j, false)) rd_cells_.push_back({patch, i, j});
+        else r_.push_back(cell_key(patch, arr, i, j));
+    }
+    void write(int patch, int arr, int i, int j) {
+        if (int32_t *p = slot(patch, arr, i, j, true)) wr_cells_.push_back({patch, i, j, p});
+        else w_.push_back(cell_key(patch, arr, i, j));
+    }
     void commit(const vx_cell_task &t) {
         bool conflict = false;
         for (uint64_t k : r_) conflict = conflict || wr.count(k);
         for (uint64_t k : w_) conflict = conflict || wr.count(k) || rd.count(k);
-        if (conflict) {
-            starts.push_back((int32_t)tasks.size());
-            rd.clear();
-            wr.clear();
-        }
+        for (const Cell &c : rd_cells_) conflict = conflict || *slot(c.patch, ARR_F, c.i, c.j, true) == cur;
+        for (const WCell &c : wr_cells_)
+            conflict = conflict || *c.w == cur || *slot(c.patch, ARR_F, c.i, c.j, false) == cur;
+        if (conflict) cut();
         tasks.push_back(t);
         rd.insert(r_.begin(), r_.end());
         wr.insert(w_.begin(), w_.end());
+        for (const Cell &c : rd_cells_) *slot(c.patch, ARR_F, c.i, c.j, false) = cur;
+        for (const WCell &c : wr_cells_) *c.w = cur;
+        rd_cells_.clear();
+        wr_cells_.clear();
     }
+    void push(const vx_cell_task &t) { tasks.push_back(t); }  // untracked (the caller cuts between phases)
     void cut() {  // force a batch boundary
         if ((int32_t)tasks.size() != starts.back()) starts.push_back((int32_t)tasks.size());
         rd.clear();
         wr.clear();
+        cur++;
     }
     vx_program *upload(Device &dev) {
         if (tasks.empty()) return nullptr;
@@ -70,6 +115,17 @@ struct TaskList {
                   "program");
         return p;
     }
+
+private:
+    struct Cell {
+        int patch, i, j;
+    };
+    struct WCell {
+        int patch, i, j;
+        int32_t *w;
+    };
+    std::vector<Cell> rd_cells_;
+    std::vector<WCell> wr_cells_;
 };
 
 vx_cell_task mk(int op, int dst, int di, int dj, int src, int si, int sj, int aux) {
@@ -856,6 +912,7 @@ void Mesh::BuildPrograms() {
     // the merged program applies every task in the sequential order's effect
     {
         TaskList P;
+        P.set_shapes(order);
         emit_level(P, *levels[0], 1, r);
         for (unsigned i = 1; i < levels.size(); i++) {
             emit_level(P, *levels[i], 0, r);
@@ -865,6 +922,7 @@ void Mesh::BuildPrograms() {
         for (unsigned i = levels.size() - 1; i > 0; i--) emit_level(P, *levels[i - 1], 2, r);
         push_prog = P.upload(*dev);
         TaskList C;
+        C.set_shapes(order);
         for (int type = 4; type <= 6; type++)
             for (auto &l : levels) emit_level(C, *l, type, r);
         pushc_prog = C.upload(*dev);
@@ -908,13 +966,9 @@ void Mesh::InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &old) {
             o->table_index = (int)all.size();
             all.push_back(o.get());
         }
-    TaskList T;
-    auto put = [&](Rectangle &D, int di, int dj, const vx_cell_task &t, Rectangle &S, int nreads, const int *ri,
-                   const int *rj) {
-        T.begin();
-        T.write(D.table_index, ARR_F, di, dj);
-        for (int q = 0; q < nreads; q++) T.read(S.table_index, ARR_F, ri[q], rj[q]);
-        T.commit(t);
+    TaskList T;  // one batch per phase, untracked (see the note at the top of this file)
+    auto put = [&](Rectangle &, int, int, const vx_cell_task &t, Rectangle &, int, const int *, const int *) {
+        T.push(t);
     };
     // coarse patch C interpolates the 2r x 2r fine cells of R around coarse cell (i, j)
     auto from_coarse = [&](Rectangle &C, Rectangle &R, bool only_missing) {
@@ -953,17 +1007,22 @@ void Mesh::InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &old) {
     for (unsigned i = 0; i + 1 < old.size(); i++) {
         for (auto &o : old[i + 1]->rectangles)
             for (auto &n : levels[i]->rectangles) from_coarse(*o, *n, false);
+        T.cut();
         for (auto &o : old[i]->rectangles)
             for (auto &n : levels[i]->rectangles) from_same(*o, *n);
+        T.cut();
     }
     if (old.size() > 1) {
         int i = (int)old.size() - 1;
         for (auto &o : old[i]->rectangles)
             for (auto &n : levels[i]->rectangles) from_same(*o, *n);
+        T.cut();
     }
-    for (int i = (int)levels.size() - 1; i > 0; i--)
+    for (int i = (int)levels.size() - 1; i > 0; i--) {
         for (auto &c : levels[i]->rectangles)
             for (auto &f : levels[i - 1]->rectangles) from_coarse(*c, *f, true);
+        T.cut();
+    }
     vx_program *prog = T.upload(*dev);
     if (!prog) return;
     std::vector<vx_patch> d;
