@@ -124,6 +124,34 @@ def test_amr_steps_and_regrid_match(amr_pair):
     assert n_g == n_o
 
 
+def test_regrid_to_the_same_hierarchy_keeps_the_patches(amr_pair):
+    """A regrid right after a regrid sees the same data and finds the same
+    boxes: the GPU run keeps its patches while the oracle rebuilds them with the
+    reference's transfer; the states agree then and three steps later."""
+    p, g, o = amr_pair
+    dt = o.calc_dt(0.4)
+    kept = False
+    for k in range(3):  # regrid (both runs) until the boxes come out unchanged
+        say(f"step: regrid {k}")
+        n0, b0 = g.regrid_stats()
+        lay = g.layout().copy()
+        g.regrid()
+        o.regrid()
+        n1, b1 = g.regrid_stats()
+        assert n1 == n0 + 1
+        compare_mesh(g, o, f"after regrid {k}")
+        if b1 == b0:
+            kept = True
+            assert np.array_equal(g.layout(), lay)
+            break
+    assert kept, "no regrid kept the hierarchy"
+    compare_mesh(g, o, "after an unchanged regrid")
+    g.advance(dt, 3)
+    o.advance(dt, 3)
+    compare_mesh(g, o, "after an unchanged regrid + 3 steps")
+    compare_fields(g, o, "after an unchanged regrid + 3 steps")
+
+
 class g_as_oracle_layout:
     """GPU f1 with the oracle's nested masks (the layouts are equal)."""
 

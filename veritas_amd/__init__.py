@@ -100,6 +100,7 @@ def host_lib():
         L.vxh_flags.restype = i
         L.vxh_flags.argtypes = [vp, i, i, vp, i]
         L.vxh_regrid.argtypes = [vp]
+        L.vxh_regrid_stats.argtypes = [vp, i, vp]
         L.vxh_file_output.argtypes = [vp, d]
         L.vxh_output_rectangles.argtypes = [vp, d]
         L.vxh_output_flush.argtypes = [vp]
@@ -283,6 +284,12 @@ class Run:
         a = (ctypes.c_int * max(n, 1))()
         L.vxh_flags(self.h, sp, level, a, n)
         return np.array(a[:n], dtype=np.int64).reshape(-1, 2)
+
+    def regrid_stats(self, sp=0):
+        """(regrids since set-up, regrids that rebuilt the patches) of species sp."""
+        out = (ctypes.c_int * 2)()
+        self._ck(host_lib().vxh_regrid_stats(self.h, sp, out))
+        return int(out[0]), int(out[1])
 
     def regrid(self):
         """SolverManager::reGrid (SolverManager.cpp:48-58)."""

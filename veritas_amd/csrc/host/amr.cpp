@@ -793,9 +793,15 @@ void cells_under(Settings &s, int sp, int lvl, const rect &b, std::vector<coords
 }
 }  // namespace
 
-// Mesh.cpp:132-210
+// Mesh.cpp:132-210.  When the new hierarchy equals the current one, the
+// reference's rebuild would reproduce the current state: every patch takes its
+// interior back from the old patch of the same box, the ghost cells are refilled
+// by the same PushData from the same data, f^n is copied from f again, and
+// every other plane is rewritten before it is read in the next stage.  The
+// patches are then kept as they are.
 void Mesh::updateHierarchy(bool init) {
     if (hierarchy.empty()) throw Error("hierarchy is empty, cannot update it");
+    const std::vector<level> before = hierarchy;
     const int top = (int)hierarchy.size() - 1;
     for (int l = top; l > -1; l--) {
         std::vector<coords> flagged;
@@ -828,6 +834,9 @@ void Mesh::updateHierarchy(bool init) {
             hierarchy.erase(hierarchy.begin() + l + 1);
         }
     }
+    if (!init) regrids++;
+    if (!init && hierarchy == before && !levels.empty()) return;
+    if (!init) rebuilds++;
     promoteHierarchyToMesh(init);
 }
 

@@ -225,6 +225,15 @@ int vxh_signatures(void *h, int sp, int lvl, int idx, int *out, int n) {
 
 int vxh_regrid(void *h) { return guarded(h, [&](Run *r) { r->sm->reGrid(); }); }
 
+// regrids of species sp's mesh since set-up, and how many of them rebuilt the patches
+int vxh_regrid_stats(void *h, int sp, int *out) {
+    return guarded(h, [&](Run *r) {
+        const vxh::Mesh &m = *r->sm->meshes.at(sp);
+        out[0] = m.regrids;
+        out[1] = m.rebuilds;
+    });
+}
+
 // SolverManager::fileOutput / OutputRectangles: text files under output/ of
 // the working directory, written by a background thread; vxh_output_flush waits.
 int vxh_file_output(void *h, double t) { return guarded(h, [&](Run *r) { r->sm->fileOutput(t); }); }

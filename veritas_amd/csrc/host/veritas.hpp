@@ -219,6 +219,7 @@ public:
     // Mesh.cpp:877-902: f slot 0 of every patch; returns the formatted files
     std::vector<std::pair<std::string, std::string>> rectangleData(double tidx) const;
     void Sub0(double timeStep, int step);  // sub-step 0 of every patch with the coarse-fine fluxes
+    int regrids = 0, rebuilds = 0;  // updateHierarchy calls after set-up, and those that changed the patches
     bool fused = true;  // single walled patch: one fused call per stage
     bool mixed = false; // the fused stage in mixed precision (FP32 weights and ratios)
 
