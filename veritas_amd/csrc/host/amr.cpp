@@ -777,19 +777,33 @@ void boxes_up(Settings &s, int sp, level &boxes, int lvl) {
     }
 }
 
-// Mesh.cpp:315-339: level-lvl cells under a level-(lvl+2) box grown by 2r
+// Mesh.cpp:315-339: level-lvl cells under a level-(lvl+2) box grown by 2r, sorted
+// and unique.  The reference visits every level-(lvl+2) cell and maps it with
+// trunc(i * fx); fx = n(lvl) / n(lvl+2) < 1 and the map is monotone with steps
+// of at most 1, so its image over [a, b] is every integer from trunc(a * fx) to
+// trunc(b * fx): the same list is the rectangle of those ranges (clipped to the
+// level), written in sorted order without visiting the fine cells.
 void cells_under(Settings &s, int sp, int lvl, const rect &b, std::vector<coords> &out) {
     const int nmax = s.GetXSize(s.maxDepth - lvl), pmax = s.GetPSize(s.maxDepth - lvl, sp);
     const double fx = nmax / double(s.GetXSize(s.maxDepth - (lvl + 2)));
     const double fp = pmax / double(s.GetPSize(s.maxDepth - (lvl + 2), sp));
     const int R = (int)s.refinementRatio;
-    for (int i = b.first.first - 2 * R; i <= b.second.first + 2 * R + 1; i++)
-        for (int j = b.first.second - 2 * R; j <= b.second.second + 2 * R; j++) {
-            int ic = i * fx, jc = j * fp;
-            if (ic > -1 && ic < nmax && jc > -1 && jc < pmax) out.push_back({ic, jc});
-        }
-    std::sort(out.begin(), out.end());
-    out.erase(std::unique(out.begin(), out.end()), out.end());
+    const int i0 = b.first.first - 2 * R, i1 = b.second.first + 2 * R + 1;
+    const int j0 = b.first.second - 2 * R, j1 = b.second.second + 2 * R;
+    if (fx > 1.0 || fp > 1.0 || i1 < i0 || j1 < j0) {  // not a coarser level: the reference's loop
+        for (int i = i0; i <= i1; i++)
+            for (int j = j0; j <= j1; j++) {
+                int ic = i * fx, jc = j * fp;
+                if (ic > -1 && ic < nmax && jc > -1 && jc < pmax) out.push_back({ic, jc});
+            }
+        std::sort(out.begin(), out.end());
+        out.erase(std::unique(out.begin(), out.end()), out.end());
+        return;
+    }
+    const int ic0 = std::max(0, (int)(i0 * fx)), ic1 = std::min(nmax - 1, (int)(i1 * fx));
+    const int jc0 = std::max(0, (int)(j0 * fp)), jc1 = std::min(pmax - 1, (int)(j1 * fp));
+    for (int ic = ic0; ic <= ic1; ic++)
+        for (int jc = jc0; jc <= jc1; jc++) out.push_back({ic, jc});
 }
 }  // namespace
 
@@ -804,6 +818,9 @@ void Mesh::updateHierarchy(bool init) {
     const std::vector<level> before = hierarchy;
     const int top = (int)hierarchy.size() - 1;
     for (int l = top; l > -1; l--) {
+        // at the deepest allowed level the reference flags and clusters, then
+        // discards the boxes: nothing to compute
+        if (l == top && l >= settings.maxDepth) continue;
         std::vector<coords> flagged;
         if (l < (int)hierarchy.size() - 2) {  // keep level l+2 nested inside the new level l+1
             std::vector<coords> own, below;
