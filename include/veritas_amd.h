@@ -298,6 +298,26 @@ void vx_program_destroy(vx_program *prog);
  * refinement ratio (2 or 4). */
 int vx_program_run(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int mode,
                    int slot, int r);
+
+/* Regrid transfer in rectangles (Mesh.cpp:116-130 with Rectangle.cpp:892-941
+ * and 1100-1128, whose per-cell loops these replace): slots 0 and 1 of dst on
+ * a rectangle of cells, from slot 1 of src.
+ *   VX_OP_COPY:   dst(di0 + a, dj0 + b) = src(si0 + a, sj0 + b), a < ni, b < nj
+ *   VX_OP_COARSE: coarse cells (ci, cj) = (si0 + a, sj0 + b) of src in level
+ *                 coordinates, a < ni, b < nj, each to its r x r finer cells
+ *                 dst((ci - di0) r + k, (cj - dj0) r + l) = element l + r k of
+ *                 the REF interpolation of that coarse cell (di0, dj0: dst's
+ *                 position in src's level coordinates)
+ * One call is one phase of the transfer (one launch over every range; the
+ * ranges of a call must write disjoint cells and read no cell a range of the
+ * call writes); it waits for the launch.  No counterpart in the reference. */
+typedef struct vx_range_task {
+    int32_t op, dst, src;
+    int32_t di0, dj0, si0, sj0, ni, nj;
+    int32_t pad;
+} vx_range_task;
+int vx_transfer_ranges(vx_ctx *ctx, const vx_patch_table *table, const vx_range_task *ranges, int32_t n,
+                       int32_t r);
 /* Coarse-fine flux program of RK stage `step` (run after vx_fct_sub0_fluxes
  * of every patch and before vx_fct_sub0_predictor), finest coarse level first. */
 int vx_program_run_fluxes(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog,

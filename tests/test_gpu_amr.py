@@ -112,6 +112,9 @@ def test_amr_steps_and_regrid_match(amr_pair):
     say("step: regrid")
     g.regrid()
     o.regrid()
+    n, b = g.regrid_stats()
+    say(f"  regrids {n}, rebuilt {b}")
+    assert b >= 1, "this regrid is expected to change the boxes (it exercises the transfer)"
     compare_mesh(g, o, "after regrid")
     say("step: advance 3 after regrid")
     g.advance(dt, 3)

@@ -981,8 +981,16 @@ void Mesh::BuildPrograms() {
     }
 }
 
-// Mesh.cpp:116-130 with Rectangle.cpp:892-941, 1100-1128 as one transfer
-// program over a table holding the new patches, then the old ones.
+// Mesh.cpp:116-130 with Rectangle.cpp:892-941, 1100-1128: the reference's
+// per-cell transfer loops as rectangles over a table holding the new patches,
+// then the old ones, one launch per phase (vx_transfer_ranges):
+//   for each level i: the new patches interpolated from the old next-coarser
+//   level, then copied from the old patches of level i (the overlaps);
+//   the coarsest level copied from its old patches;
+//   from the coarsest new level down, the finer cells no earlier phase wrote,
+//   interpolated from the new next-coarser level.
+// Each phase writes a cell at most once and reads only cells of old patches or
+// of earlier phases, so the phases reproduce the reference's sequential order.
 void Mesh::InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &old) {
     const int r = settings.refinementRatio;
     std::vector<Rectangle *> all = order;
@@ -992,72 +1000,87 @@ void Mesh::InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &old) {
             o->table_index = (int)all.size();
             all.push_back(o.get());
         }
-    TaskList T;  // one batch per phase, untracked (see the note at the top of this file)
-    auto put = [&](Rectangle &, int, int, const vx_cell_task &t, Rectangle &, int, const int *, const int *) {
-        T.push(t);
+    std::vector<std::vector<vx_range_task>> phases;
+    auto range = [](int op, Rectangle &D, Rectangle &S, int di0, int dj0, int si0, int sj0, int ni, int nj) {
+        vx_range_task t{};
+        t.op = op;
+        t.dst = D.table_index;
+        t.src = S.table_index;
+        t.di0 = di0;
+        t.dj0 = dj0;
+        t.si0 = si0;
+        t.sj0 = sj0;
+        t.ni = ni;
+        t.nj = nj;
+        return t;
     };
-    // coarse patch C interpolates the 2r x 2r fine cells of R around coarse cell (i, j)
-    auto from_coarse = [&](Rectangle &C, Rectangle &R, bool only_missing) {
+    // coarse patch C interpolates the r x r fine cells of R under each coarse cell
+    // (i, j) of C's range within one coarse cell of R; only_missing: the blocks no
+    // earlier phase wrote, as runs along p
+    auto from_coarse = [&](Rectangle &C, Rectangle &R, bool only_missing, std::vector<vx_range_task> &ph) {
         const int xp2 = R.x_pos / r, pp2 = R.p_pos / r, nx2 = R.n_x / r, np2 = R.n_p / r;
         const int lx = std::max(C.x_pos, xp2 - 1), ux = std::min(C.x_pos + C.n_x, xp2 + nx2 + 1);
         const int lp = std::max(C.p_pos, pp2 - 1), up_ = std::min(C.p_pos + C.n_p, pp2 + np2 + 1);
-        int ri[25], rj[25];
-        for (int i = lx; i < ux; i++)
-            for (int j = lp; j < up_; j++) {
-                if (only_missing && R.is_interpolated[R.NS(r * (i - xp2) + 1, r * (j - pp2) + 1)]) continue;
-                int ic = i - C.x_pos, jc = j - C.p_pos, q = 0;
-                for (int a = -2; a <= 2; a++)
-                    for (int b = -2; b <= 2; b++, q++) {
-                        ri[q] = ic + a;
-                        rj[q] = jc + b;
-                    }
-                for (int k = 0; k < r; k++)
-                    for (int l = 0; l < r; l++) {
-                        int di = (i - xp2) * r + k, dj = (j - pp2) * r + l;
-                        put(R, di, dj, mk(VX_OP_COARSE, R.table_index, di, dj, C.table_index, r * i + 1, r * j + 1, l + r * k),
-                            C, 25, ri, rj);
-                        if (!only_missing) R.is_interpolated[R.NS(di, dj)] = 1;
-                    }
+        if (ux <= lx || up_ <= lp) return;
+        if (!only_missing) {
+            ph.push_back(range(VX_OP_COARSE, R, C, xp2, pp2, lx, lp, ux - lx, up_ - lp));
+            for (int i = lx; i < ux; i++)
+                for (int j = lp; j < up_; j++)
+                    for (int k = 0; k < r; k++)
+                        for (int l = 0; l < r; l++) R.is_interpolated[R.NS((i - xp2) * r + k, (j - pp2) * r + l)] = 1;
+            return;
+        }
+        for (int i = lx; i < ux; i++) {
+            int j = lp;
+            while (j < up_) {
+                if (R.is_interpolated[R.NS(r * (i - xp2) + 1, r * (j - pp2) + 1)]) {
+                    j++;
+                    continue;
+                }
+                int j1 = j + 1;
+                while (j1 < up_ && !R.is_interpolated[R.NS(r * (i - xp2) + 1, r * (j1 - pp2) + 1)]) j1++;
+                ph.push_back(range(VX_OP_COARSE, R, C, xp2, pp2, i, j, 1, j1 - j));
+                j = j1;
             }
+        }
     };
-    auto from_same = [&](Rectangle &O, Rectangle &R) {  // Rectangle.cpp:920-941
+    auto from_same = [&](Rectangle &O, Rectangle &R, std::vector<vx_range_task> &ph) {  // Rectangle.cpp:920-941
         const int lx = std::max(O.x_pos, R.x_pos - 2), ux = std::min(O.x_pos + O.n_x, R.x_pos + R.n_x + 2);
         const int lp = std::max(O.p_pos, R.p_pos - 2), up_ = std::min(O.p_pos + O.n_p, R.p_pos + R.n_p + 2);
+        if (ux <= lx || up_ <= lp) return;
+        ph.push_back(range(VX_OP_COPY, R, O, lx - R.x_pos, lp - R.p_pos, lx - O.x_pos, lp - O.p_pos, ux - lx,
+                           up_ - lp));
         for (int i = lx; i < ux; i++)
-            for (int j = lp; j < up_; j++) {
-                int di = i - R.x_pos, dj = j - R.p_pos, si = i - O.x_pos, sj = j - O.p_pos;
-                put(R, di, dj, mk(VX_OP_COPY, R.table_index, di, dj, O.table_index, si, sj, 0), O, 1, &si, &sj);
-                R.is_interpolated[R.NS(di, dj)] = 1;
-            }
+            for (int j = lp; j < up_; j++) R.is_interpolated[R.NS(i - R.x_pos, j - R.p_pos)] = 1;
     };
     for (unsigned i = 0; i + 1 < old.size(); i++) {
+        phases.emplace_back();
         for (auto &o : old[i + 1]->rectangles)
-            for (auto &n : levels[i]->rectangles) from_coarse(*o, *n, false);
-        T.cut();
+            for (auto &n : levels[i]->rectangles) from_coarse(*o, *n, false, phases.back());
+        phases.emplace_back();
         for (auto &o : old[i]->rectangles)
-            for (auto &n : levels[i]->rectangles) from_same(*o, *n);
-        T.cut();
+            for (auto &n : levels[i]->rectangles) from_same(*o, *n, phases.back());
     }
     if (old.size() > 1) {
         int i = (int)old.size() - 1;
+        phases.emplace_back();
         for (auto &o : old[i]->rectangles)
-            for (auto &n : levels[i]->rectangles) from_same(*o, *n);
-        T.cut();
+            for (auto &n : levels[i]->rectangles) from_same(*o, *n, phases.back());
     }
-    for (int i = (int)levels.size() - 1; i > 0; i--) {
-        for (auto &c : levels[i]->rectangles)
-            for (auto &f : levels[i - 1]->rectangles) from_coarse(*c, *f, true);
-        T.cut();
-    }
-    vx_program *prog = T.upload(*dev);
-    if (!prog) return;
+    // the missing cells: each phase reads the level the previous phase completed
     std::vector<vx_patch> d;
     for (Rectangle *x : all) d.push_back(x->desc());
     vx_patch_table *tb = nullptr;
     dev->check(vx_patch_table_create(dev->ctx, d.data(), (int32_t)d.size(), &tb), "transfer table");
-    int rc = vx_program_run(dev->ctx, tb, prog, VX_MODE_TRANSFER, 1, r);
-    if (rc == VX_OK) rc = vx_sync(dev->ctx);
-    vx_program_destroy(prog);
+    int rc = VX_OK;
+    for (auto &ph : phases)
+        if (rc == VX_OK) rc = vx_transfer_ranges(dev->ctx, tb, ph.data(), (int32_t)ph.size(), r);
+    for (int i = (int)levels.size() - 1; i > 0 && rc == VX_OK; i--) {
+        std::vector<vx_range_task> ph;
+        for (auto &c : levels[i]->rectangles)
+            for (auto &f : levels[i - 1]->rectangles) from_coarse(*c, *f, true, ph);
+        rc = vx_transfer_ranges(dev->ctx, tb, ph.data(), (int32_t)ph.size(), r);
+    }
     vx_patch_table_destroy(tb);
     dev->check(rc, "regrid transfer");
 }

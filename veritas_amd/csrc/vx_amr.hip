@@ -10,6 +10,7 @@
 // the per-cell arithmetic keeps the reference's expressions so that the
 // results match the CPU oracle bit for bit.
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "vx_dev.h"
@@ -311,6 +312,40 @@ __global__ __launch_bounds__(FB) void k_flag_scatter(const uint8_t *flags, long 
 
 int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r);
 
+// One thread per cell of a phase of regrid-transfer rectangles: range k covers
+// cells [start[k], start[k+1]) of the launch (binary search on start).
+__global__ __launch_bounds__(256) void k_ranges(ProgArgs A, const vx_range_task *rg, const long *start, int n) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= start[n]) return;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (start[mid] <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const vx_range_task R = rg[lo];
+    const long local = t - start[lo];
+    const vx_patch &D = A.pt[R.dst], &S = A.pt[R.src];
+    double v;
+    int di, dj;
+    if (R.op == VX_OP_COPY) {
+        const int a = (int)(local / R.nj), b = (int)(local % R.nj);
+        di = R.di0 + a;
+        dj = R.dj0 + b;
+        v = get(S.f[1], S.pitch, R.si0 + a, R.sj0 + b);
+    } else {  // VX_OP_COARSE
+        const int rr = A.r * A.r;
+        const int cell = (int)(local / rr), e = (int)(local % rr);
+        const int k = e / A.r, l = e % A.r;
+        const int ci = R.si0 + cell / R.nj, cj = R.sj0 + cell % R.nj;
+        di = (ci - R.di0) * A.r + k;
+        dj = (cj - R.dj0) * A.r + l;
+        v = coarse_element(A, S, S.f[1], A.r * ci + 1, A.r * cj + 1, l + A.r * k);
+    }
+    at(D.f[0], D.pitch, di, dj) = v;
+    at(D.f[1], D.pitch, di, dj) = v;
+}
+
 int fill_common(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int r, ProgArgs &A) {
     if (!table || !prog || !table->d) return fail(ctx, VX_EINVAL, "program: null table or program");
     if (r < 2 || r > MAXR) return fail(ctx, VX_ENOTSUP, "program: refinement ratio %d not in [2, %d]", r, MAXR);
@@ -485,6 +520,71 @@ void vx_program_destroy(vx_program *prog) {
     if (!prog) return;
     if (prog->d) (void)hipFree(prog->d);
     delete prog;
+}
+
+int vx_transfer_ranges(vx_ctx *ctx, const vx_patch_table *table, const vx_range_task *ranges, int32_t n,
+                       int32_t r) {
+    if (n == 0) return VX_OK;
+    if (!table || !table->d || !ranges || n < 0) return fail(ctx, VX_EINVAL, "transfer ranges: bad arguments");
+    if (r < 2 || r > MAXR) return fail(ctx, VX_ENOTSUP, "transfer ranges: refinement ratio %d not in [2, %d]", r, MAXR);
+    if (ctx->capturing) return fail(ctx, VX_EINVAL, "transfer ranges while a step graph is recorded");
+    // host bounds check of every cell read and written, and the cell prefix
+    std::vector<long> start(n + 1, 0);
+    for (int k = 0; k < n; k++) {
+        const vx_range_task &R = ranges[k];
+        bool ok = R.dst >= 0 && R.dst < table->n && R.src >= 0 && R.src < table->n && R.ni > 0 && R.nj > 0;
+        long cells = 0;
+        if (ok) {
+            const vx_patch &D = table->host[R.dst], &S = table->host[R.src];
+            if (R.op == VX_OP_COPY) {
+                ok = inr(D, R.di0, R.dj0) && inr(D, R.di0 + R.ni - 1, R.dj0 + R.nj - 1) && inr(S, R.si0, R.sj0) &&
+                     inr(S, R.si0 + R.ni - 1, R.sj0 + R.nj - 1);
+                cells = (long)R.ni * R.nj;
+            } else if (R.op == VX_OP_COARSE) {
+                const int ic = R.si0 - S.x_pos, jc = R.sj0 - S.p_pos;
+                const int fi = (R.si0 - R.di0) * r, fj = (R.sj0 - R.dj0) * r;
+                ok = R.si0 >= 0 && R.sj0 >= 0 && inr(S, ic - 2, jc - 2) && inr(S, ic + R.ni + 1, jc + R.nj + 1) &&
+                     inr(D, fi, fj) && inr(D, fi + R.ni * r - 1, fj + R.nj * r - 1);
+                cells = (long)R.ni * R.nj * r * r;
+            } else {
+                ok = false;
+            }
+        }
+        if (!ok)
+            return fail(ctx, VX_EINVAL, "transfer range %d (op %d, dst %d, src %d) out of range", k, ranges[k].op,
+                        ranges[k].dst, ranges[k].src);
+        start[k + 1] = start[k] + cells;
+    }
+    ProgArgs A{};
+    A.pt = table->d;
+    A.r = r;
+    A.inv_r2 = 1.0 / std::pow(r, 2);
+    A.r2 = std::pow((double)r, 2.0);
+    A.inv_rr = (1.0 / (r * r));
+    for (int i = 0; i < 3 * MAXR; i++) A.ref[i] = 0.0;
+    for (int i = 0; i < r; i++) {  // Rectangle.cpp:91-101
+        double tl = -0.5 + i / (double)r;
+        double tr = -0.5 + (i + 1.0) / (double)r;
+        A.ref[3 * i] = (tl + tr) * 0.5;
+        A.ref[3 * i + 1] = (tl * tl + tl * tr + tr * tr) / 3.0 - (1.0 / 12);
+        A.ref[3 * i + 2] = (tl * tl * tl + tl * tl * tr + tl * tr * tr + tr * tr * tr) * 0.25;
+    }
+    const size_t rb = sizeof(vx_range_task) * n, sb = sizeof(long) * (n + 1);
+    char *d = nullptr;
+    int rc = check_hip(ctx, hipMalloc((void **)&d, rb + sb), "transfer ranges");
+    if (rc) return rc;
+    std::vector<char> h(rb + sb);
+    std::memcpy(h.data(), ranges, rb);
+    std::memcpy(h.data() + rb, start.data(), sb);
+    rc = check_hip(ctx, hipMemcpyAsync(d, h.data(), rb + sb, hipMemcpyHostToDevice, ctx->stream), "transfer ranges");
+    if (!rc && start[n] > 0) {
+        k_ranges<<<(unsigned)((start[n] + 255) / 256), 256, 0, ctx->stream>>>(A, (const vx_range_task *)d,
+                                                                               (const long *)(d + rb), n);
+        rc = check_launch(ctx, "k_ranges");
+    }
+    const int rs = check_hip(ctx, hipStreamSynchronize(ctx->stream), "transfer ranges");
+    (void)hipFree(d);
+    return rc ? rc : rs;
 }
 
 int vx_program_run(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int mode, int slot, int r) {
