@@ -35,6 +35,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <vector>
 
 #include "vx_dev.h"
@@ -433,10 +436,38 @@ void launch_stage(vx_ctx *ctx, StageArgs &A, bool mp) {
 // beat the taller ones by more than 3 %).  While a step graph
 // is being recorded the choice falls back to the shortest strip that still
 // gives 2048 wavefronts.
+//
+// The choices are kept per device for the whole process (every context on the
+// device shares them: a serving loop's runs of one shape time it once).
+struct TileKey {
+    int device, nx, np, mp, step;
+    bool operator<(const TileKey &o) const {
+        return std::tie(device, nx, np, mp, step) < std::tie(o.device, o.nx, o.np, o.mp, o.step);
+    }
+};
+std::mutex g_tile_mu;
+std::map<TileKey, int> g_tiles;
+
+int tile_lookup(int device, int nx, int np, int mp, int step) {
+    std::lock_guard<std::mutex> g(g_tile_mu);
+    auto it = g_tiles.find(TileKey{device, nx, np, mp, step});
+    return it == g_tiles.end() ? 0 : it->second;
+}
+
+void tile_store(vx_ctx *ctx, int nx, int np, int ti, int mp, int step) {
+    ctx->ti_cache.push_back({nx, np, ti, mp, step});
+    std::lock_guard<std::mutex> g(g_tile_mu);
+    g_tiles[TileKey{ctx->device, nx, np, mp, step}] = ti;
+}
+
 int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
     const int nx = A0.p.n_x, np = A0.p.n_p;
     for (auto &c : ctx->ti_cache)
         if (c[0] == nx && c[1] == np && c[3] == (int)mp && c[4] == A0.step) return c[2];
+    if (const int known = tile_lookup(ctx->device, nx, np, (int)mp, A0.step)) {
+        ctx->ti_cache.push_back({nx, np, known, (int)mp, A0.step});
+        return known;
+    }
     static const int cand[] = {64, 32, 16, 8, 4, 2};
     // VX_TILE_ROWS pins the choice (one value, or six comma-separated values,
     // one per RK stage), e.g. to profile the configuration a timed run chose
@@ -449,7 +480,7 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
         }
         const int ti = n == 6 ? v[A0.step] : (n >= 1 ? v[0] : 0);
         if (ti >= 1 && ti <= 1024) {
-            ctx->ti_cache.push_back({nx, np, ti, (int)mp, A0.step});
+            ctx->ti_cache.push_back({nx, np, ti, (int)mp, A0.step});  // this context only
             return ti;
         }
     }
@@ -485,7 +516,7 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    ctx->ti_cache.push_back({nx, np, best, (int)mp, A0.step});
+    tile_store(ctx, nx, np, best, (int)mp, A0.step);
     return best;
 }
 
@@ -655,6 +686,7 @@ int vx_fct_stage_fused_batch(vx_ctx *ctx, const vx_stage_item *items, int n, int
 int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed) {
     for (auto &c : ctx->ti_cache)
         if (c[0] == n_x && c[1] == n_p && c[3] == (mixed ? 1 : 0) && c[4] == step) return c[2];
+    if (const int known = tile_lookup(ctx->device, n_x, n_p, mixed ? 1 : 0, step)) return known;
     return 0;
 }
 
