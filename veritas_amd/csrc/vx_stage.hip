@@ -324,20 +324,25 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
             double f2u = shdn1(f2_m2), f2d = shup1(f2_m2);
             double rp = 0.0, rm = 0.0;
             if (i >= -1 && i < nx + 1 && jr_R) {
-                double Pp = valmax(0.0, FxD_m2) - valmin(0.0, FxD_m1) + valmax(0.0, FpD_m2) - valmin(0.0, FpDu);
-                double Pm = valmax(0.0, FxD_m1) - valmin(0.0, FxD_m2) + valmax(0.0, FpDu) - valmin(0.0, FpD_m2);
-                double wMax = valmax(valmax(f0_m2, f2_m2), valmax(valmax(f0_m1, f2_m1),
-                              valmax(valmax(f0_m3, f2_m3), valmax(valmax(f0u, f2u), valmax(f0d, f2d)))));
-                double wMin = valmin(valmin(f0_m2, f2_m2), valmin(valmin(f0_m1, f2_m1),
-                              valmin(valmin(f0_m3, f2_m3), valmin(valmin(f0u, f2u), valmin(f0d, f2d)))));
+                // fmax/fmin (one v_max_f64/v_min_f64 each) give the reference's
+                // a > b ? a : b bits here: f is never -0 (every update adds to a
+                // +0 or positive start), R and Q are never -0, nothing is NaN,
+                // and a signed zero among the P terms reaches P only when P is
+                // zero, where R is 0 either way
+                double Pp = fmax(0.0, FxD_m2) - fmin(0.0, FxD_m1) + fmax(0.0, FpD_m2) - fmin(0.0, FpDu);
+                double Pm = fmax(0.0, FxD_m1) - fmin(0.0, FxD_m2) + fmax(0.0, FpDu) - fmin(0.0, FpD_m2);
+                double wMax = fmax(fmax(f0_m2, f2_m2), fmax(fmax(f0_m1, f2_m1),
+                              fmax(fmax(f0_m3, f2_m3), fmax(fmax(f0u, f2u), fmax(f0d, f2d)))));
+                double wMin = fmin(fmin(f0_m2, f2_m2), fmin(fmin(f0_m1, f2_m1),
+                              fmin(fmin(f0_m3, f2_m3), fmin(fmin(f0u, f2u), fmin(f0d, f2d)))));
                 double Qm = -wMin + f2_m2;
                 double Qp = wMax - f2_m2;
                 if (MP) {
                     rp = Pp > 0.0 ? zalesak_mp(Qp, Pp, sinv) : 0.0;
                     rm = Pm > 0.0 ? zalesak_mp(Qm, Pm, sinv) : 0.0;
                 } else {
-                    rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
-                    rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
+                    rp = Pp > 0.0 ? fmin(1.0, Qp / Pp) : 0.0;
+                    rm = Pm > 0.0 ? fmin(1.0, Qm / Pm) : 0.0;
                 }
             }
             Rp_m2 = rp;
@@ -351,11 +356,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
             double Rpd = shup1(Rp_m3), Rmd = shup1(Rm_m3);  // R at (i, j-1)
             const bool jint = j >= 0 && j < np;
             double cxl = (i >= 1 && i < nx && jint)
-                             ? (FxD_m3 > 0.0 ? valmin(Rp_m3, Rm_m4) : valmin(Rp_m4, Rm_m3)) : 1.0;
+                             ? (FxD_m3 > 0.0 ? fmin(Rp_m3, Rm_m4) : fmin(Rp_m4, Rm_m3)) : 1.0;
             double cxr = (i + 1 >= 1 && i + 1 < nx && jint)
-                             ? (FxD_m2 > 0.0 ? valmin(Rp_m2, Rm_m3) : valmin(Rp_m3, Rm_m2)) : 1.0;
+                             ? (FxD_m2 > 0.0 ? fmin(Rp_m2, Rm_m3) : fmin(Rp_m3, Rm_m2)) : 1.0;
             double cpl = (i >= 1 && i < nx && jint)
-                             ? (FpD_m3 > 0.0 ? valmin(Rp_m3, Rmd) : valmin(Rpd, Rm_m3)) : 1.0;
+                             ? (FpD_m3 > 0.0 ? fmin(Rp_m3, Rmd) : fmin(Rpd, Rm_m3)) : 1.0;
             double cpu = shdn1(cpl);      // C at (i, j+1), computed by the next lane
             double FpDu = shdn1(FpD_m3);  // FpDS at (i, j+1)
             if (out_lane && i >= i0 && i < i1) {
