@@ -136,19 +136,30 @@ __device__ __forceinline__ void add_two(double &acc, double &cmp, double v) {
 // launches of at most one wavefront per SIMD.  Otherwise the summands and the
 // ordered sums of a chunk run one after the other through one buffer, in
 // fewer registers and less LDS, so that more wavefronts share each SIMD.
-template <int RTB, int C, bool COMP, bool FUSE>
+//
+// PAIR (plain form only): two wavefronts per column group, a producer that
+// stages chunk k and writes its summands while the consumer adds chunk k - 1
+// in order, one workgroup barrier per chunk.  The SIMD interleaves the two
+// instruction streams, which hides the latency of each (for launches of at
+// most one group per SIMD).
+template <int RTB, int C, bool COMP, bool FUSE, bool PAIR = false>
 __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     constexpr int NCH = C * RTB;
     static_assert(NCH <= 32, "chains must fit in half a wavefront");
     const vx_patch &P = A.p;
+    static_assert(!PAIR || (!COMP && !FUSE), "the pair form is a plain form of its own");
+    constexpr int NW = PAIR ? 1 : 4;  // column groups per workgroup
     const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int i0 = (blockIdx.x * 4 + wave) * C;
-    __shared__ double S[4][NCH][68];   // sub-cell values, p index jb-1+e
-    __shared__ double H[4][68];        // Gamma(p)+c1 p for the chain being prepared
-    __shared__ double Lb[4][NCH][68];  // c2 log(h(j+1)/h(j)), p index jb-1+e
-    __shared__ double T[FUSE ? 2 : 1][4][2][NCH][65];  // summands: buffer, wave, charge/current, chain, p
-    if (i0 >= P.n_x) return;  // uniform per wavefront
+    // LDS slot of this wavefront's column group; in the pair form wavefront 0
+    // produces and wavefront 1 consumes the group's summands
+    const int wave = PAIR ? 0 : (int)(threadIdx.x >> 6);
+    const bool producer = !PAIR || threadIdx.x < 64, consumer = !PAIR || threadIdx.x >= 64;
+    const int i0 = (PAIR ? (int)blockIdx.x : (int)blockIdx.x * 4 + wave) * C;
+    __shared__ double S[NW][NCH][68];   // sub-cell values, p index jb-1+e
+    __shared__ double H[NW][68];        // Gamma(p)+c1 p for the chain being prepared
+    __shared__ double Lb[NW][NCH][68];  // c2 log(h(j+1)/h(j)), p index jb-1+e
+    __shared__ double T[FUSE || PAIR ? 2 : 1][NW][2][NCH][65];  // summands: buffer, group, charge/current, chain, p
+    if (i0 >= P.n_x) return;  // uniform per wavefront (per workgroup in the pair form)
     const int pitch = P.pitch;
     const double *g = P.f[1];
     double asq[NCH];
@@ -346,18 +357,57 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         wave_sync();
     };
     Pre pa, pb;
-    prefetch(-1, pa);
-    if (64 < P.n_p) prefetch(63, pb);
+    if (producer) {
+        prefetch(-1, pa);
+        if (64 < P.n_p) prefetch(63, pb);
+    }
     int last = 0;  // summand buffer of the last chunk
+    // pair form: the producer fills buffer bb with chunk jb while the consumer
+    // adds the other buffer (chunk jb - 64); the barrier hands them over
+    auto pair_chunk = [&](int jb, int bb, Pre &pr) {
+        if (producer) {
+            for (int ch = 0; ch < NCH; ch++) {
+                S[0][ch][lane] = pr.v[ch][0];
+                if (lane < 2) S[0][ch][64 + lane] = pr.v[ch][1];
+            }
+            if (use_tab) {
+                Lb[0][0][lane] = pr.t[0];
+                if (lane < 2) Lb[0][0][64 + lane] = pr.t[1];
+            }
+            int nflag[C];
+            for (int c = 0; c < C; c++) nflag[c] = pr.n[c];
+            if (jb + 128 < P.n_p) prefetch(jb + 127, pr);
+            for (int ch = 0; ch < NCH && !use_tab; ch++) {
+                if (ch > 0 && asq[ch] == asq[ch - 1]) {
+                    for (int e = lane; e < 66; e += 64) Lb[0][ch][e] = Lb[0][ch - 1][e];
+                    continue;
+                }
+                for (int e = lane; e < 67; e += 64) {
+                    double pm = momentum(P.pmin, P.dp, jb - 1 + e, P.p_pos);
+                    H[0][e] = gamma_(pm, asq[ch], P.mc2inv) + A.c1 * pm;
+                }
+                wave_sync();
+                for (int e = lane; e < 66; e += 64) Lb[0][ch][e] = A.c2 * vx_log(H[0][e + 1] / H[0][e]);
+                wave_sync();
+            }
+            wave_sync();
+            summands(jb, bb, nflag);
+        } else if (jb > 0) {
+            ordered(bb ^ 1);
+        }
+        __syncthreads();
+    };
     for (int jb = 0; jb < P.n_p; jb += 128) {
-        chunk(jb, 0, pa);
+        if (PAIR) pair_chunk(jb, 0, pa);
+        else chunk(jb, 0, pa);
         last = 0;
         if (jb + 64 < P.n_p) {
-            chunk(jb + 64, 1, pb);
+            if (PAIR) pair_chunk(jb + 64, 1, pb);
+            else chunk(jb + 64, 1, pb);
             last = 1;
         }
     }
-    if (!COMP && FUSE) ordered(last);
+    if (!COMP && (FUSE || PAIR) && consumer) ordered(last);
     if (COMP) {
         // one compensated reduction across the lanes per chain
 #pragma unroll
@@ -371,7 +421,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             cmpJ[ch] = e;
         }
     }
-    if ((lane & 31) < NCH) {
+    if (consumer && (lane & 31) < NCH) {
         int ch = lane & 31;
         int c = ch / RTB, k = ch % RTB;
         int i = i0 + c;
@@ -530,15 +580,18 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
                    g4 = (unsigned)((p->n_x + 3) / 4);
     // wavefronts of the launch (4 sub-columns each) against the device's SIMDs
     const bool fuse = (size_t)(p->n_x * p->rtb + 3) / 4 <= (size_t)4 * ctx->n_cu;
+    // the pair form: one workgroup of two wavefronts per group of 4 sub-columns
+    const unsigned gp1 = (unsigned)((p->n_x + 3) / 4), gp2 = (unsigned)((p->n_x + 1) / 2),
+                   gp4 = (unsigned)p->n_x;
     switch (p->rtb * 4 + (comp ? 2 : 0) + (fuse ? 1 : 0)) {
         case 4: k_moments<1, 4, false, false><<<g1, 256, 0, ctx->stream>>>(A); break;
-        case 5: k_moments<1, 4, false, true><<<g1, 256, 0, ctx->stream>>>(A); break;
+        case 5: k_moments<1, 4, false, false, true><<<gp1, 128, 0, ctx->stream>>>(A); break;
         case 6: case 7: k_moments<1, 4, true, false><<<g1, 256, 0, ctx->stream>>>(A); break;
         case 8: k_moments<2, 2, false, false><<<g2, 256, 0, ctx->stream>>>(A); break;
-        case 9: k_moments<2, 2, false, true><<<g2, 256, 0, ctx->stream>>>(A); break;
+        case 9: k_moments<2, 2, false, false, true><<<gp2, 128, 0, ctx->stream>>>(A); break;
         case 10: case 11: k_moments<2, 2, true, false><<<g2, 256, 0, ctx->stream>>>(A); break;
         case 16: k_moments<4, 1, false, false><<<g4, 256, 0, ctx->stream>>>(A); break;
-        case 17: k_moments<4, 1, false, true><<<g4, 256, 0, ctx->stream>>>(A); break;
+        case 17: k_moments<4, 1, false, false, true><<<gp4, 128, 0, ctx->stream>>>(A); break;
         case 18: case 19: k_moments<4, 1, true, false><<<g4, 256, 0, ctx->stream>>>(A); break;
         default: return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d not in {1,2,4}", p->rtb);
     }
