@@ -777,19 +777,18 @@ void boxes_up(Settings &s, int sp, level &boxes, int lvl) {
     }
 }
 
+}  // namespace
+
 // Mesh.cpp:315-339: level-lvl cells under a level-(lvl+2) box grown by 2r, sorted
-// and unique.  The reference visits every level-(lvl+2) cell and maps it with
-// trunc(i * fx); fx = n(lvl) / n(lvl+2) < 1 and the map is monotone with steps
-// of at most 1, so its image over [a, b] is every integer from trunc(a * fx) to
+// and unique, for the box [i0, i1] x [j0, j1] of the finer level (i1, j1
+// inclusive, already grown) on a level of nmax x pmax cells, fx = nmax /
+// n(lvl+2), fp likewise.  The reference visits every level-(lvl+2) cell and maps
+// it with trunc(i * fx); fx < 1 and the map is monotone with steps of at most
+// 1, so its image over [a, b] is every integer from trunc(a * fx) to
 // trunc(b * fx): the same list is the rectangle of those ranges (clipped to the
 // level), written in sorted order without visiting the fine cells.
-void cells_under(Settings &s, int sp, int lvl, const rect &b, std::vector<coords> &out) {
-    const int nmax = s.GetXSize(s.maxDepth - lvl), pmax = s.GetPSize(s.maxDepth - lvl, sp);
-    const double fx = nmax / double(s.GetXSize(s.maxDepth - (lvl + 2)));
-    const double fp = pmax / double(s.GetPSize(s.maxDepth - (lvl + 2), sp));
-    const int R = (int)s.refinementRatio;
-    const int i0 = b.first.first - 2 * R, i1 = b.second.first + 2 * R + 1;
-    const int j0 = b.first.second - 2 * R, j1 = b.second.second + 2 * R;
+void cells_under_box(int nmax, int pmax, double fx, double fp, int i0, int i1, int j0, int j1,
+                     std::vector<coords> &out) {
     if (fx > 1.0 || fp > 1.0 || i1 < i0 || j1 < j0) {  // not a coarser level: the reference's loop
         for (int i = i0; i <= i1; i++)
             for (int j = j0; j <= j1; j++) {
@@ -804,6 +803,16 @@ void cells_under(Settings &s, int sp, int lvl, const rect &b, std::vector<coords
     const int jc0 = std::max(0, (int)(j0 * fp)), jc1 = std::min(pmax - 1, (int)(j1 * fp));
     for (int ic = ic0; ic <= ic1; ic++)
         for (int jc = jc0; jc <= jc1; jc++) out.push_back({ic, jc});
+}
+
+namespace {
+void cells_under(Settings &s, int sp, int lvl, const rect &b, std::vector<coords> &out) {
+    const int nmax = s.GetXSize(s.maxDepth - lvl), pmax = s.GetPSize(s.maxDepth - lvl, sp);
+    const double fx = nmax / double(s.GetXSize(s.maxDepth - (lvl + 2)));
+    const double fp = pmax / double(s.GetPSize(s.maxDepth - (lvl + 2), sp));
+    const int R = (int)s.refinementRatio;
+    cells_under_box(nmax, pmax, fx, fp, b.first.first - 2 * R, b.second.first + 2 * R + 1, b.first.second - 2 * R,
+                    b.second.second + 2 * R, out);
 }
 }  // namespace
 

@@ -225,6 +225,21 @@ int vxh_signatures(void *h, int sp, int lvl, int idx, int *out, int n) {
 
 int vxh_regrid(void *h) { return guarded(h, [&](Run *r) { r->sm->reGrid(); }); }
 
+// The nesting list of Mesh::updateHierarchy (host logic, for the CPU tests):
+// level cells under the finer-level box [i0, i1] x [j0, j1]; writes up to n
+// (ic, jc) pairs, returns the count
+int vxh_cells_under_box(int nmax, int pmax, double fx, double fp, int i0, int i1, int j0, int j1, int *out, int n) {
+    std::vector<vxh::coords> v;
+    vxh::cells_under_box(nmax, pmax, fx, fp, i0, i1, j0, j1, v);
+    const int m = (int)v.size();
+    if (out && n >= m)
+        for (int k = 0; k < m; k++) {
+            out[2 * k] = v[k].first;
+            out[2 * k + 1] = v[k].second;
+        }
+    return m;
+}
+
 // regrids of species sp's mesh since set-up, and how many of them rebuilt the patches
 int vxh_regrid_stats(void *h, int sp, int *out) {
     return guarded(h, [&](Run *r) {

@@ -199,6 +199,9 @@ private:
 typedef std::pair<int, int> coords;  // veritas.hpp:45-48
 typedef std::pair<coords, coords> rect;
 typedef std::vector<rect> level;
+// Mesh.cpp:315-339 for one box (host/amr.cpp)
+void cells_under_box(int nmax, int pmax, double fx, double fp, int i0, int i1, int j0, int j1,
+                     std::vector<coords> &out);
 
 // Mesh.hpp:4-41
 class Mesh {
