@@ -155,6 +155,34 @@ def test_regrid_to_the_same_hierarchy_keeps_the_patches(amr_pair):
     compare_fields(g, o, "after an unchanged regrid + 3 steps")
 
 
+def test_moment_forms_agree_on_every_level():
+    """The p-moments in both forms (the split form of large launches and the
+    producer/consumer pairs of small ones, pinned with VX_MOMENTS_FORM) on the
+    three-level case, whose patches have relativeToBottom 1, 2 and 4: the same
+    bits, and the oracle's."""
+    import os
+    p = two_slab_laser()
+    o = OracleRun(p)
+    dt = o.calc_dt(0.4)
+    runs = {}
+    for form in ("split", "pair"):
+        say(f"case: moments form {form}, 3 levels, 3 steps")
+        os.environ["VX_MOMENTS_FORM"] = form
+        try:
+            g = va.Run(p, fused=False)
+            g.advance(dt, 3)
+            runs[form] = g
+        finally:
+            del os.environ["VX_MOMENTS_FORM"]
+    o.advance(dt, 3)
+    compare_mesh(runs["split"], o, "split form")
+    compare_mesh(runs["pair"], o, "pair form")
+    compare_fields(runs["split"], o, "split form")
+    for g in runs.values():
+        g.close()
+    o.close()
+
+
 class g_as_oracle_layout:
     """GPU f1 with the oracle's nested masks (the layouts are equal)."""
 

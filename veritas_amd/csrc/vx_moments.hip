@@ -8,6 +8,7 @@
 // in parallel (interpolants, gamma factors, logarithms) and stage them in LDS;
 // lane 0 then adds them in order.  4096 columns give 4096 independent chains,
 // which keeps every CU busy while each chain stays in reference order.
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -578,8 +579,13 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
     LaunchTimer lt(ctx, "k_moments");
     const unsigned g1 = (unsigned)((p->n_x + 15) / 16), g2 = (unsigned)((p->n_x + 7) / 8),
                    g4 = (unsigned)((p->n_x + 3) / 4);
-    // wavefronts of the launch (4 sub-columns each) against the device's SIMDs
-    const bool fuse = (size_t)(p->n_x * p->rtb + 3) / 4 <= (size_t)4 * ctx->n_cu;
+    // wavefronts of the launch (4 sub-columns each) against the device's SIMDs;
+    // VX_MOMENTS_FORM=split or =pair pins the form (both give the same bits)
+    bool fuse = (size_t)(p->n_x * p->rtb + 3) / 4 <= (size_t)4 * ctx->n_cu;
+    if (const char *form = std::getenv("VX_MOMENTS_FORM")) {
+        if (std::strcmp(form, "split") == 0) fuse = false;
+        if (std::strcmp(form, "pair") == 0) fuse = true;
+    }
     // the pair form: one workgroup of two wavefronts per group of 4 sub-columns
     const unsigned gp1 = (unsigned)((p->n_x + 3) / 4), gp2 = (unsigned)((p->n_x + 1) / 2),
                    gp4 = (unsigned)p->n_x;
