@@ -117,6 +117,36 @@ def mixed_error(va, params, steps):
             "dt": "0.4 x CFL of the FP64 run, every step", "at_step": out}
 
 
+def amr_mixed_error(va, params, steps):
+    """mixed_error for a hierarchy: f over every patch of every level (the
+    hierarchy is fixed, so both runs hold the same patches)."""
+    import numpy as np
+    g, r = va.Run(params, mixed=True), va.Run(params)
+    lay = r.layout()
+
+    def rel(x, y):
+        n = np.linalg.norm(y)
+        return float(np.linalg.norm(x - y) / n) if n > 0 else float(np.linalg.norm(x))
+
+    def all_f(run):
+        return np.concatenate([run.patch(int(l), k, "f1").ravel() for l in sorted(set(lay[:, 0]))
+                               for k in range(int((lay[:, 0] == l).sum()))])
+
+    out, step = {}, 0
+    for m in (steps // 4, steps // 2, steps):
+        while step < m:
+            dt = r.calc_dt(0.4)
+            g.advance(dt)
+            r.advance(dt)
+            step += 1
+        out[str(m)] = {"f": rel(all_f(g), all_f(r)), "Ex": rel(g.get("E"), r.get("E")),
+                       "Ey": rel(g.get("Ey"), r.get("Ey")), "Bz": rel(g.get("Bz"), r.get("Bz"))}
+    g.close()
+    r.close()
+    return {"steps": steps, "reference": "FP64 GPU path (bit-identical to the CPU oracle)",
+            "dt": "0.4 x CFL of the FP64 run, every step", "at_step": out}
+
+
 def graph_step_ms(run, dt, steps):
     """ms per step with the steps replayed as recorded graphs (the default
     mode of SolverManager::Advance); the kernel-timed run above launches
@@ -130,14 +160,16 @@ def graph_step_ms(run, dt, steps):
 
 def bench_amr(a, va):
     """Laser pulse into underdense plasma: base grid 8192 x 512, three levels at
-    ratio 2, p in [-2, 2] m_e c, FP64, dt = 0.4 x CFL, sub-step kernels with the
-    device task programs.  A cell-update is one cell of one patch of any level
+    ratio 2, p in [-2, 2] m_e c, FP64 (or the mixed-precision form with
+    --precision mixed), dt = 0.4 x CFL, sub-step kernels with the device task
+    programs.  A cell-update is one cell of one patch of any level
     advanced one full step (nested coarse cells included: they are computed,
     then overwritten by restriction).  The hierarchy is fixed during the timed
     steps (the reference regrids every 20 or more steps)."""
     params = va.laser()
     t0 = time.perf_counter()
-    run = va.Run(params, fused=False, graphs=False)  # eager launches: per-launch events
+    mixed = a.precision == "mixed"
+    run = va.Run(params, fused=False, graphs=False, mixed=mixed)  # eager launches: per-launch events
     setup = time.perf_counter() - t0
     lay = run.layout()
     cells = int((lay[:, 3] * lay[:, 4]).sum())
@@ -172,10 +204,10 @@ def bench_amr(a, va):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64+f32" if mixed else "f64",
         "data": "synthetic (laser pulse into a ramped plasma slab)",
-        "config": {"workload": "relativistic laser-plasma, base 8192x512, 3 levels at ratio 2, FP64, "
-                               "dt=0.4xCFL, fixed hierarchy",
+        "config": {"workload": "relativistic laser-plasma, base 8192x512, 3 levels at ratio 2, "
+                               + ("mixed precision" if mixed else "FP64") + ", dt=0.4xCFL, fixed hierarchy",
                    "patches_per_level": [int((lay[:, 0] == l).sum()) for l in range(int(lay[:, 0].max()) + 1)],
                    "cells_all_levels": cells, "setup_s": setup, "graph_ms_per_step": graph_ms,
                    "regrid_ms": sorted(regrid_ms)[1], "regrids_that_rebuilt": f"{rebuilds} of {regrids}",
@@ -186,6 +218,10 @@ def bench_amr(a, va):
                      "kernel_share": kms / ms if ms else None},
     }
     run.close()
+    if mixed:
+        out["config"]["precision"] = ("FP32 WENO weights and Zalesak ratios in the batched flux and limiter "
+                                      "launches; FP64 fluxes, updates, task programs, compensated moments, fields")
+        out["error_vs_fp64"] = amr_mixed_error(va, params, a.error_steps) if a.error_steps > 0 else None
     if not a.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from _oracle import OracleRun

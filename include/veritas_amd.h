@@ -335,6 +335,16 @@ int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double
 int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
 int vx_fct_sub2_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
 int vx_fct_sub3_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
+/* Mixed-precision forms of vx_fct_sub0_fluxes_table and vx_fct_sub1_table
+ * (no counterpart in the reference): the WENO weights of both face
+ * reconstructions and the Zalesak ratios R+- in FP32 on f scaled by the power
+ * of two nearest below f_scale (the mesh's max |f|), as vx_fct_stage_fused_mp
+ * does for one walled patch; candidate face values, fluxes, the RK
+ * combination, the limiter C and every update of f stay FP64.  The other
+ * sub-steps and the task programs are shared with the FP64 path. */
+int vx_fct_sub0_fluxes_table_mp(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const vx_fields *fl,
+                                double f_scale);
+int vx_fct_sub1_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_scale);
 
 /* Split form of vx_fct_sub0 for hierarchies: the fluxes and RK combination
  * (Rectangle.cpp:1269-1517), then the low-order predictor (1518-1534). */

@@ -21,7 +21,7 @@ import numpy as np
 import pytest
 
 import veritas_amd as va
-from _amr_cases import two_slab_laser
+from _amr_cases import particle_number, two_slab_laser
 from _oracle import OracleRun
 from veritas_amd.params import landau, two_stream
 
@@ -106,3 +106,69 @@ def test_compensated_moments_agree_with_ordered_sums():
     assert np.abs(q1 - q0).max() <= 1e-13 * np.abs(q0).max()
     assert np.abs(j1 - j0).max() <= 1e-13 * np.abs(j0).max()
     a.close()
+
+
+def test_sub_step_path_equals_fused_stage_in_mixed_precision():
+    """The batched sub-step launches in mixed precision (FP32 weights in the
+    flux launch, FP32 ratios in the limiter launch) evaluate the same
+    expressions on the same scaled values as the fused stage: bit for bit the
+    same on a walled single patch."""
+    p = two_slab_laser(levels=1)
+    say("case: mixed precision, laser single level, sub-step path vs fused stage, 30 steps")
+    a, b = va.Run(p, fused=True, mixed=True), va.Run(p, fused=False, mixed=True)
+    for _ in range(30):
+        dt = a.calc_dt(0.4)
+        a.advance(dt)
+        b.advance(dt)
+    for w in ("f1", "f0", "E", "Ey", "Bz"):
+        assert np.array_equal(a.get(w), b.get(w)), w
+    a.close()
+    b.close()
+
+
+class _GpuNesting:
+    """GPU f1 with the oracle's nested-cell masks (the layouts are equal)."""
+
+    def __init__(self, g, o):
+        self.g, self.o = g, o
+
+    def layout(self):
+        return self.g.layout()
+
+    def patch(self, lvl, k, w):
+        return self.o.patch(lvl, k, w) if w == "nested" else self.g.patch(lvl, k, w)
+
+
+def test_laser_three_levels_against_oracle():
+    """A three-level hierarchy (many patches per level) in mixed precision
+    against the FP64 oracle: errors over every patch of every level, particle
+    number over the uncovered cells of all levels."""
+    p = two_slab_laser()
+    say("case: mixed precision, three-level two-slab laser, 80 steps vs oracle")
+    g, o = va.Run(p, mixed=True), OracleRun(p)
+    lay = o.layout()
+    assert np.array_equal(g.layout(), lay)
+    s = o.scalars()
+    n0 = particle_number(o, s["dx"], s["dp"])
+    step = 0
+    for m in (20, 40, 80):
+        while step < m:
+            dt = o.calc_dt(0.4)
+            g.advance(dt)
+            o.advance(dt)
+            step += 1
+        fg, fo = [], []
+        for lvl in sorted(set(lay[:, 0])):
+            for k in range(int((lay[:, 0] == lvl).sum())):
+                fg.append(g.patch(lvl, k, "f1").ravel())
+                fo.append(o.patch(lvl, k, "f1").ravel())
+        e = {"f": rel(np.concatenate(fg), np.concatenate(fo)), "E": rel(g.get("E"), o.get("E")),
+             "Ey": rel(g.get("Ey"), o.get("Ey")), "Bz": rel(g.get("Bz"), o.get("Bz")),
+             "N": abs(particle_number(_GpuNesting(g, o), s["dx"], s["dp"]) / n0 - 1)}
+        say(f"  step {m}: {e}")
+        assert e["f"] < 1e-5 and e["E"] < 1e-6, (m, e)
+        assert e["Ey"] < 1e-12 and e["Bz"] < 1e-12, (m, e)
+        assert e["N"] < 1e-13, (m, e)
+    assert e["f"] > 0.0  # the mixed-precision launches ran
+    g.close()
+    o.close()

@@ -364,9 +364,21 @@ void Mesh::Sub0(double timeStep, int step) {
     EnsureTable();
     const vx_fields &fl = EMSolver->fields();
     // every patch's fluxes in one launch (patches write only their own planes)
-    dev->check(vx_fct_sub0_fluxes_table(dev->ctx, table, step, timeStep, &fl), "fct sub0 fluxes");
+    if (mixed)
+        dev->check(vx_fct_sub0_fluxes_table_mp(dev->ctx, table, step, timeStep, &fl, FScale()),
+                   "fct sub0 fluxes (mixed)");
+    else
+        dev->check(vx_fct_sub0_fluxes_table(dev->ctx, table, step, timeStep, &fl), "fct sub0 fluxes");
     for (unsigned i = 1; i < levels.size(); i++) levels[i]->RunFluxProgram(step, timeStep, fl);
     dev->check(vx_fct_sub0_predictor_table(dev->ctx, table, step, timeStep, fl.n_f), "fct sub0 predictor");
+}
+
+double Mesh::FScale() const {
+    double s = 0.0;
+    for (const auto &l : levels)
+        for (const auto &r : l->rectangles)
+            s = std::max(s, r->fScale > 0.0 ? r->fScale : settings.GetfMax(particleType));
+    return s;
 }
 
 void Mesh::EnsureTable() {
@@ -393,7 +405,8 @@ void Mesh::Advance(double timeStep, int step) {
     Sub0(timeStep, step);
     PushData(2);
     EnsureTable();
-    dev->check(vx_fct_sub1_table(dev->ctx, table, n_f), "fct sub1");
+    if (mixed) dev->check(vx_fct_sub1_table_mp(dev->ctx, table, n_f, FScale()), "fct sub1 (mixed)");
+    else dev->check(vx_fct_sub1_table(dev->ctx, table, n_f), "fct sub1");
     PushBoundaryC();
     dev->check(vx_fct_sub2_table(dev->ctx, table, n_f), "fct sub2");
     PushData(1);

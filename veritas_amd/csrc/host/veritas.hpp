@@ -224,7 +224,10 @@ public:
     void Sub0(double timeStep, int step);  // sub-step 0 of every patch with the coarse-fine fluxes
     int regrids = 0, rebuilds = 0;  // updateHierarchy calls after set-up, and those that changed the patches
     bool fused = true;  // single walled patch: one fused call per stage
-    bool mixed = false; // the fused stage in mixed precision (FP32 weights and ratios)
+    // mixed precision (FP32 WENO weights and Zalesak ratios): the fused stage of
+    // a single walled patch, else the batched flux and limiter launches
+    bool mixed = false;
+    double FScale() const;  // scale of the FP32 copies: max over patches of max |f|
 
 private:
     Settings &settings;
@@ -317,8 +320,9 @@ public:
     void CflAsync();                       // vx_cfl_bound_async for the next dt
     double CflResult(double cfl);          // cfl * the bound (CalculateDt)
     bool CflReady();                       // the pending bound has arrived
-    // mixed-precision form (single walled patch): FP32 WENO weights and Zalesak
-    // ratios in the fused stage, compensated FP64 moment sums; FP64 otherwise
+    // mixed-precision form: FP32 WENO weights and Zalesak ratios (the fused
+    // stage of a single walled patch, the batched flux and limiter launches of
+    // a hierarchy), compensated FP64 moment sums; FP64 otherwise
     void SetMixed(bool on);
     // Advance as a graph launch (host/graph.cpp): the first step from a given
     // plane-pointer state and dt runs eagerly, the second records the step's

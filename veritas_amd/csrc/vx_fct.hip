@@ -8,6 +8,8 @@
 // are zero in the reference (vectors are zero-assigned in the constructor,
 // Rectangle.cpp:10-38); the kernels reproduce that by computing each quantity
 // only inside its range and using 0 outside.
+#include <cmath>
+
 #include "vx_dev.h"
 #include "vx_internal.h"
 
@@ -23,6 +25,8 @@ struct Sub0Coef {
     int ns;          // step + 1 terms in the RK combination
     double a[6];     // RK coefficients times dt (Rectangle.cpp:1396-1517)
     double aSum;     // a[0] + ... + a[ns-1], summed left to right
+    double sinv = 1.0;   // mixed precision: 2^-k, the scale of the FP32 copies of f
+    float eps32 = 0.0f;  // mixed precision: the WENO epsilon (1e-10) in the scaled units
 };
 
 // EMFieldSolver::GetASquared(GetFinestIndex(i)) scaled by q*q (Rectangle.cpp:1280)
@@ -48,13 +52,15 @@ __device__ __forceinline__ double patch_E(const vx_patch &p, const vx_fields &fl
 // there, and FxH_s/FpH_s (FxL0/FpL0 at stage 0) where the reference's loops
 // compute them; later stages read the stored FxL0/FpL0 (the coarse-fine flux
 // program may have replaced them at interface faces).
+// MP: the WENO weights from FP32 copies of f scaled by A.sinv (vxd::weno_mp);
+// candidates, fluxes and the RK combination stay FP64.
 constexpr int S0W = 58;  // positions per wavefront
 
 struct Sub0Tiles {
     int TI, nstrip, nbx;
 };
 
-template <int STEP>
+template <int STEP, bool MP>
 __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A, const Sub0Tiles &T,
                                            const int tile) {
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
@@ -79,6 +85,7 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
     auto ld = [&](int i) { return (i >= -2 && i < nx + 2 && in_j) ? f1[pidx(pitch, i, j)] : 0.0; };
 
     double fs_m3 = 0, fs_m2 = 0, fs_m1 = 0, fs_0 = 0;
+    float gs_m3 = 0, gs_m2 = 0, gs_m1 = 0, gs_0 = 0;  // MP: scaled FP32 copies
     double ep_m2 = 0, ep_m1 = 0, ep_0 = 0, fp_m2 = 0, fp_m1 = 0, fp_0 = 0;
     const int r0 = i0 - 2;
     double as_m1 = face_as(P, A.fl, r0 - 1), as_0 = face_as(P, A.fl, r0);
@@ -103,6 +110,7 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
 
     for (int r = r0; r <= i1; r++) {
         fs_0 = pfs_a;
+        if (MP) gs_0 = (float)(fs_0 * A.sinv);
         pfs_a = pfs_b;
         pfs_b = ld(r + 2);
         double chx[NH], chp[NH];
@@ -121,7 +129,11 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
         const double Gd = shdn1(G_m1);
         const double ex = (ix >= -1 && ix < nx + 2 && jr_ex) ? dpcc * (Gd - G_m1) : 0.0;
         const bool vFx = ix >= 0 && ix < nx + 1 && jr_ex;
-        const double fx = vFx ? weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0) : 0.0;
+        double fx = 0.0;
+        if (vFx) {
+            if (MP) fx = weno_mp(fs_m3, fs_m2, fs_m1, fs_0, gs_m3, gs_m2, gs_m1, gs_0, A.eps32, ex > 0.0);
+            else fx = weno(fs_m3, fs_m2, fs_m1, fs_0, ex > 0.0);
+        }
         const double ex_u = shdn1(ex), ex_d = shup1(ex), fx_u = shdn1(fx), fx_d = shup1(fx);
         const double fxh = vFx ? P.dx_inv * (fx * ex + w3 * (fx_u - fx_d) * (ex_u - ex_d)) : 0.0;
         const double fxl = vFx ? P.dx_inv * ((ex > 0.0 ? fs_m2 : fs_m1) * ex) : 0.0;
@@ -131,7 +143,13 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
         const double qE = cr ? P.q * patch_E(P, A.fl, r) : 0.0;
         ep_0 = (cr && jr_ep) ? qE - ccdx * (G_p1 - G_0) : 0.0;
         const double s1 = shup1(fs_0), s2 = shup1(s1), s3 = shdn1(fs_0);
-        fp_0 = (cr && jr_fp) ? weno(s2, s1, fs_0, s3, ep_0 > 0.0) : 0.0;
+        fp_0 = 0.0;
+        if (MP) {
+            const float t1 = shup1f(gs_0), t2 = shup1f(t1), t3 = shdn1f(gs_0);
+            if (cr && jr_fp) fp_0 = weno_mp(s2, s1, fs_0, s3, t2, t1, gs_0, t3, A.eps32, ep_0 > 0.0);
+        } else if (cr && jr_fp) {
+            fp_0 = weno(s2, s1, fs_0, s3, ep_0 > 0.0);
+        }
 
         // column ix = r - 1 (Rectangle.cpp:1353-1394)
         const bool vFp = ix >= -1 && ix < nx + 1 && jr_fp;
@@ -164,6 +182,7 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
         }
 
         fs_m3 = fs_m2; fs_m2 = fs_m1; fs_m1 = fs_0;
+        gs_m3 = gs_m2; gs_m2 = gs_m1; gs_m1 = gs_0;
         as_0 = as_p1; G_m1 = G_0; G_0 = G_p1;
         ep_m2 = ep_m1; ep_m1 = ep_0; fp_m2 = fp_m1; fp_m1 = fp_0;
     }
@@ -171,7 +190,7 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
 
 template <int STEP>
 __global__ __launch_bounds__(256) void k_sub0s(vx_patch P, Sub0Coef A, Sub0Tiles T) {
-    sub0s_tile<STEP>(P, A, T, (int)blockIdx.x);
+    sub0s_tile<STEP, false>(P, A, T, (int)blockIdx.x);
 }
 
 // The first patch k of the work list whose workgroups start at or before w
@@ -187,13 +206,13 @@ __device__ __forceinline__ int batch_patch(const vx_batch_info *bi, int n, int w
     return lo;
 }
 
-template <int STEP>
+template <int STEP, bool MP>
 __global__ __launch_bounds__(256) void k_sub0s_batch(const vx_patch *tab, const vx_batch_info *bi, int n,
                                                      Sub0Coef A) {
     const int w = (int)blockIdx.x;
     const int k = batch_patch<0>(bi, n, w);
     const Sub0Tiles T{bi[k].TI, bi[k].nstrip, bi[k].nbx};
-    sub0s_tile<STEP>(tab[k], A, T, w - bi[k].start[0]);
+    sub0s_tile<STEP, MP>(tab[k], A, T, w - bi[k].start[0]);
 }
 
 // Low-order predictor f2 on interior cells (Rectangle.cpp:1518-1534) in
@@ -231,8 +250,10 @@ __device__ __forceinline__ void lo_predictor(const vx_patch &P, double aSum, int
     P.f[2][pidx(pitch, i, j)] = t;
 }
 
-// Zalesak R+- of cell (i, j) (Rectangle.cpp:1538-1579)
-__device__ __forceinline__ void ratios(const vx_patch &P, int i, int j, double &rp, double &rm) {
+// Zalesak R+- of cell (i, j) (Rectangle.cpp:1538-1579); MP: the ratios
+// Q / P in FP32 on the values scaled by sinv (vxd::zalesak_mp)
+template <bool MP>
+__device__ __forceinline__ void ratios(const vx_patch &P, int i, int j, double &rp, double &rm, double sinv) {
     const int pitch = P.pitch;
     const double *FxDS = P.FxDS, *FpDS = P.FpDS, *f0 = P.f[0], *f2 = P.f[2];
     long idx = pidx(pitch, i, j);
@@ -251,8 +272,13 @@ __device__ __forceinline__ void ratios(const vx_patch &P, int i, int j, double &
                   valmin(valmin(d0, d2), valmin(e0, e2)))));
     double Qm = -wMin + a2;
     double Qp = wMax - a2;
-    rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
-    rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
+    if (MP) {
+        rp = Pp > 0.0 ? zalesak_mp(Qp, Pp, sinv) : 0.0;
+        rm = Pm > 0.0 ? zalesak_mp(Qm, Pm, sinv) : 0.0;
+    } else {
+        rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
+        rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
+    }
 }
 
 // Both limiter passes in one: R+- on [-1, n_x+1) x [-1, n_p+1), then C on the
@@ -269,7 +295,8 @@ constexpr int LR = 16;  // rows of a limiter block (LR x 64 threads)
 __host__ __device__ inline int lim_gx(int n_p) { return (n_p + 4 + 62) / 63; }
 __host__ __device__ inline int lim_gy(int n_x) { return (n_x + 4 + LR - 2) / (LR - 1); }
 
-__device__ __forceinline__ void limiter_rc(const vx_patch &P, int bx, int by) {
+template <bool MP>
+__device__ __forceinline__ void limiter_rc(const vx_patch &P, int bx, int by, double sinv) {
     __shared__ double sRp[LR][64], sRm[LR][64];
     const int i0 = -3 + by * (LR - 1), j0 = -3 + bx * 63;
     const int t = (int)threadIdx.x, r = t >> 6, c = t & 63;
@@ -277,7 +304,7 @@ __device__ __forceinline__ void limiter_rc(const vx_patch &P, int bx, int by) {
     const bool own = r >= 1 && c >= 1;
     if (i >= -1 && i < P.n_x + 1 && j >= -1 && j < P.n_p + 1) {
         double rp, rm;
-        ratios(P, i, j, rp, rm);
+        ratios<MP>(P, i, j, rp, rm, sinv);
         sRp[r][c] = rp;
         sRm[r][c] = rm;
         if (own) {
@@ -327,7 +354,9 @@ __device__ __forceinline__ void corrector(const vx_patch &P, int bx, int by) {
 __global__ __launch_bounds__(256) void k_lo_predictor(vx_patch P, double aSum) {
     lo_predictor(P, aSum, (int)blockIdx.x, (int)blockIdx.y);
 }
-__global__ __launch_bounds__(LR * 64) void k_limiter_rc(vx_patch P) { limiter_rc(P, (int)blockIdx.x, (int)blockIdx.y); }
+__global__ __launch_bounds__(LR * 64) void k_limiter_rc(vx_patch P) {
+    limiter_rc<false>(P, (int)blockIdx.x, (int)blockIdx.y, 1.0);
+}
 __global__ __launch_bounds__(256) void k_corrector(vx_patch P) { corrector(P, (int)blockIdx.x, (int)blockIdx.y); }
 
 // Work-list forms: workgroup w of the launch is workgroup w - start of patch k,
@@ -349,12 +378,14 @@ __global__ __launch_bounds__(256) void k_lo_predictor_batch(const vx_patch *tab,
     batch_cell<1, 0>(bi, n, k, bx, by, tab);
     lo_predictor(tab[k], aSum, bx, by);
 }
-__global__ __launch_bounds__(LR * 64) void k_limiter_rc_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
+template <bool MP>
+__global__ __launch_bounds__(LR * 64) void k_limiter_rc_batch(const vx_patch *tab, const vx_batch_info *bi, int n,
+                                                               double sinv) {
     const int w = (int)blockIdx.x;
     const int k = batch_patch<2>(bi, n, w);
     const int local = w - bi[k].start[2];
     const int gx = lim_gx(tab[k].n_p);
-    limiter_rc(tab[k], local % gx, local / gx);
+    limiter_rc<MP>(tab[k], local % gx, local / gx, sinv);
 }
 __global__ __launch_bounds__(256) void k_corrector_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
     int k, bx, by;
@@ -452,6 +483,46 @@ int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
     return rc;
 }
 
+int mp_scale(vx_ctx *ctx, double f_scale, double &sinv, float &eps32, const char *what) {
+    if (!(f_scale > 0.0) || !std::isfinite(f_scale))
+        return fail(ctx, VX_EINVAL, "%s: f scale must be positive and finite", what);
+    sinv = std::ldexp(1.0, -std::ilogb(f_scale));
+    eps32 = (float)(1.0e-10 * sinv * sinv);
+    return VX_OK;
+}
+
+template <bool MP>
+void launch_sub0s_batch(vx_ctx *ctx, vx_patch_table *t, int step, const Sub0Coef &A) {
+    dim3 grid((unsigned)t->batch[t->n].start[0]);
+    LaunchTimer lt(ctx, "k_sub0");
+    switch (step) {
+        case 0: k_sub0s_batch<0, MP><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+        case 1: k_sub0s_batch<1, MP><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+        case 2: k_sub0s_batch<2, MP><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+        case 3: k_sub0s_batch<3, MP><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+        case 4: k_sub0s_batch<4, MP><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+        default: k_sub0s_batch<5, MP><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
+    }
+}
+
+int sub0_fluxes_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const vx_fields *fl, bool mp,
+                             double f_scale) {
+    if (step < 0 || step > 5 || !fl) return fail(ctx, VX_EINVAL, "sub0 table: bad step %d", step);
+    Sub0Coef A;
+    if (mp) {
+        int rc = mp_scale(ctx, f_scale, A.sinv, A.eps32, "sub0 table (mixed)");
+        if (rc) return rc;
+    }
+    int rc = table_batch(ctx, t, fl->n_f);
+    if (rc) return rc;
+    A.fl = *fl;
+    A.step = step;
+    rk_coefficients(step, dt, A.a, A.ns, A.aSum);
+    if (mp) launch_sub0s_batch<true>(ctx, t, step, A);
+    else launch_sub0s_batch<false>(ctx, t, step, A);
+    return check_launch(ctx, "k_sub0s_batch");
+}
+
 }  // namespace
 
 extern "C" {
@@ -547,26 +618,12 @@ int vx_fct_stage_walls(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
 
 // ---- work-list (batched) forms: one launch per sub-step for every patch of a table
 int vx_fct_sub0_fluxes_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const vx_fields *fl) {
-    if (step < 0 || step > 5 || !fl) return fail(ctx, VX_EINVAL, "sub0 table: bad step %d", step);
-    int rc = table_batch(ctx, t, fl->n_f);
-    if (rc) return rc;
-    Sub0Coef A;
-    A.fl = *fl;
-    A.step = step;
-    rk_coefficients(step, dt, A.a, A.ns, A.aSum);
-    dim3 grid((unsigned)t->batch[t->n].start[0]);
-    {
-        LaunchTimer lt(ctx, "k_sub0");
-        switch (step) {
-            case 0: k_sub0s_batch<0><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
-            case 1: k_sub0s_batch<1><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
-            case 2: k_sub0s_batch<2><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
-            case 3: k_sub0s_batch<3><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
-            case 4: k_sub0s_batch<4><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
-            default: k_sub0s_batch<5><<<grid, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, A); break;
-        }
-    }
-    return check_launch(ctx, "k_sub0s_batch");
+    return sub0_fluxes_table(ctx, t, step, dt, fl, false, 0.0);
+}
+
+int vx_fct_sub0_fluxes_table_mp(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const vx_fields *fl,
+                                double f_scale) {
+    return sub0_fluxes_table(ctx, t, step, dt, fl, true, f_scale);
 }
 
 int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, int32_t n_f) {
@@ -583,7 +640,19 @@ int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double
 int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
     int rc = table_batch(ctx, t, n_f);
     if (rc) return rc;
-    k_limiter_rc_batch<<<(unsigned)t->batch[t->n].start[2], LR * 64, 0, ctx->stream>>>(t->d, t->batch_d, t->n);
+    k_limiter_rc_batch<false>
+        <<<(unsigned)t->batch[t->n].start[2], LR * 64, 0, ctx->stream>>>(t->d, t->batch_d, t->n, 1.0);
+    return check_launch(ctx, "k_limiter_rc_batch");
+}
+
+int vx_fct_sub1_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_scale) {
+    double sinv;
+    float eps32;
+    int rc = mp_scale(ctx, f_scale, sinv, eps32, "sub1 table (mixed)");
+    if (!rc) rc = table_batch(ctx, t, n_f);
+    if (rc) return rc;
+    k_limiter_rc_batch<true>
+        <<<(unsigned)t->batch[t->n].start[2], LR * 64, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
     return check_launch(ctx, "k_limiter_rc_batch");
 }
 

@@ -67,46 +67,6 @@ struct StageArgs {
     float eps32;  // MP: the WENO epsilon (1e-10) in the scaled units
 };
 
-// WENO face value with the weights in FP32 (g: f * sinv in FP32) and the
-// candidates and the blend in FP64; same candidate choice as vxd::weno.
-__device__ __forceinline__ double weno_mp(double f1, double f2, double f3, double f4, float g1, float g2,
-                                          float g3, float g4, float eps, bool right) {
-    const double fL = (1.0 / 6) * (-f1 + 5 * f2 + 2 * f3);
-    const double fR = (1.0 / 6) * (2 * f2 + 5 * f3 - f4);
-    const float AL = g1 - 2 * g2 + g3, BL = g3 - g1;
-    const float AR = g2 - 2 * g3 + g4, BR = g4 - g2;
-    const float bL = eps + (4.0f / 3 * (AL * AL) + 0.5f * AL * BL + 0.25f * (BL * BL));
-    const float bR = eps + (4.0f / 3 * (AR * AR) - 0.5f * AR * BR + 0.25f * (BR * BR));
-    // oL / (oL + oR) with o = 0.5 / b^2 is bR^2 / (bL^2 + bR^2); both are
-    // divided by the larger b first, so the squares stay in [0, 1]
-    const float m = fmaxf(bL, bR);
-    const bool ok = m > 1e-30f;
-    const float rm = __builtin_amdgcn_rcpf(ok ? m : 1.0f);
-    const float x = bL * rm, y = bR * rm, x2 = x * x, y2 = y * y;
-    const float rden = __builtin_amdgcn_rcpf(x2 + y2);
-    const float wL = ok ? y2 * rden : 0.5f, wR = ok ? x2 * rden : 0.5f;
-    const float wL0 = wL * (0.75f + wL * (wL - 0.5f));
-    const float wR0 = wR * (0.75f + wR * (wR - 0.5f));
-    const bool takeL = right ? (wL0 > wR0) : (wL0 < wR0);
-    const float w = (takeL ? wL0 : wR0) * __builtin_amdgcn_rcpf(wL0 + wR0);
-    return fR + (double)w * (fL - fR);
-}
-
-// Zalesak ratio min(1, Q / P) for P > 0 in FP32 on the scaled values
-__device__ __forceinline__ double zalesak_mp(double Q, double P, double sinv) {
-    const float qf = (float)(Q * sinv), pf = (float)(P * sinv);
-    if (pf > 1e-30f) return (double)fminf(1.0f, qf * __builtin_amdgcn_rcpf(pf));
-    return qf > 0.0f ? 1.0 : 0.0;
-}
-
-// FP32 lane shifts for the scaled copies of f
-__device__ __forceinline__ float shdn1f(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x130, 0xf, 0xf, true));
-}
-__device__ __forceinline__ float shup1f(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x138, 0xf, 0xf, true));
-}
-
 __device__ __forceinline__ double ldp(const double *pl, int pitch, int nx, int np, int i, int j) {
     return (i >= -2 && i < nx + 2 && j >= -2 && j < np + 2) ? pl[pidx(pitch, i, j)] : 0.0;
 }
