@@ -250,80 +250,105 @@ __device__ __forceinline__ void lo_predictor(const vx_patch &P, double aSum, int
     P.f[2][pidx(pitch, i, j)] = t;
 }
 
-// Zalesak R+- of cell (i, j) (Rectangle.cpp:1538-1579); MP: the ratios
-// Q / P in FP32 on the values scaled by sinv (vxd::zalesak_mp)
-template <bool MP>
-__device__ __forceinline__ void ratios(const vx_patch &P, int i, int j, double &rp, double &rm, double sinv) {
-    const int pitch = P.pitch;
-    const double *FxDS = P.FxDS, *FpDS = P.FpDS, *f0 = P.f[0], *f2 = P.f[2];
-    long idx = pidx(pitch, i, j);
-    double Pp = valmax(0.0, FxDS[idx]) - valmin(0.0, FxDS[pidx(pitch, i + 1, j)]) +
-                valmax(0.0, FpDS[idx]) - valmin(0.0, FpDS[pidx(pitch, i, j + 1)]);
-    double Pm = valmax(0.0, FxDS[pidx(pitch, i + 1, j)]) - valmin(0.0, FxDS[idx]) +
-                valmax(0.0, FpDS[pidx(pitch, i, j + 1)]) - valmin(0.0, FpDS[idx]);
-    double a0 = f0[idx], a2 = f2[idx];
-    double b0 = f0[pidx(pitch, i + 1, j)], b2 = f2[pidx(pitch, i + 1, j)];
-    double c0 = f0[pidx(pitch, i - 1, j)], c2 = f2[pidx(pitch, i - 1, j)];
-    double d0 = f0[pidx(pitch, i, j + 1)], d2 = f2[pidx(pitch, i, j + 1)];
-    double e0 = f0[pidx(pitch, i, j - 1)], e2 = f2[pidx(pitch, i, j - 1)];
-    double wMax = valmax(valmax(a0, a2), valmax(valmax(b0, b2), valmax(valmax(c0, c2),
-                  valmax(valmax(d0, d2), valmax(e0, e2)))));
-    double wMin = valmin(valmin(a0, a2), valmin(valmin(b0, b2), valmin(valmin(c0, c2),
-                  valmin(valmin(d0, d2), valmin(e0, e2)))));
-    double Qm = -wMin + a2;
-    double Qp = wMax - a2;
-    if (MP) {
-        rp = Pp > 0.0 ? zalesak_mp(Qp, Pp, sinv) : 0.0;
-        rm = Pm > 0.0 ? zalesak_mp(Qm, Pm, sinv) : 0.0;
-    } else {
-        rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
-        rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
-    }
-}
+// Zalesak limiter, both passes in one launch (Rectangle.cpp:1538-1594): R+- on
+// [-1, n_x+1) x [-1, n_p+1), then C on the NS range [-2, n_x+2) x [-2, n_p+2):
+// 1 everywhere, the limiter formula on interior faces i in [1, n_x), j in
+// [0, n_p).  Register-streaming form: a wavefront owns 64 consecutive p
+// positions, lane l <-> j = jw - 2 + l, and produces lanes 2..62 (61
+// positions; p neighbours come from DPP lane shifts).  It marches along x over
+// a strip of C rows [i0, i1): at row i it forms R of row i from rows i-1..i+1
+// of f^n, f^td and the antidiffusive fluxes (held in registers, one new row
+// loaded per step), and C of row i from R of rows i and i-1.  R is stored
+// where it is formed inside the strip (the cross-patch C sync reads it); the
+// first row of a strip (i0 - 1) only warms the pipeline.  MP: the ratios Q / P
+// in FP32 on the values scaled by sinv (vxd::zalesak_mp).
+constexpr int LIMW = 61;  // positions per limiter wavefront
 
-// Both limiter passes in one: R+- on [-1, n_x+1) x [-1, n_p+1), then C on the
-// NS range [-2, n_x+2) x [-2, n_p+2): 1 everywhere, the limiter formula on
-// interior faces i in [1, n_x), j in [0, n_p) (Rectangle.cpp:1538-1594).
-// Workgroup (bx, by) forms R on an LR x 64 block of cells (one per thread,
-// rows i0 .., columns j0 ..) in LDS and owns its last LR - 1 rows and 63
-// columns: it stores R there (the cross-patch C sync reads R) and forms C
-// there from the LDS block, whose first row and column are the halo.  Blocks
-// overlap by one row and one column; an overlapped cell repeats its owner's
-// arithmetic, so both hold the same bits.
-constexpr int LR = 16;  // rows of a limiter block (LR x 64 threads)
-
-__host__ __device__ inline int lim_gx(int n_p) { return (n_p + 4 + 62) / 63; }
-__host__ __device__ inline int lim_gy(int n_x) { return (n_x + 4 + LR - 2) / (LR - 1); }
+struct LimTiles {
+    int TI, nstrip, nbx;
+};
 
 template <bool MP>
-__device__ __forceinline__ void limiter_rc(const vx_patch &P, int bx, int by, double sinv) {
-    __shared__ double sRp[LR][64], sRm[LR][64];
-    const int i0 = -3 + by * (LR - 1), j0 = -3 + bx * 63;
-    const int t = (int)threadIdx.x, r = t >> 6, c = t & 63;
-    const int i = i0 + r, j = j0 + c;
-    const bool own = r >= 1 && c >= 1;
-    if (i >= -1 && i < P.n_x + 1 && j >= -1 && j < P.n_p + 1) {
-        double rp, rm;
-        ratios<MP>(P, i, j, rp, rm, sinv);
-        sRp[r][c] = rp;
-        sRm[r][c] = rm;
-        if (own) {
-            const long idx = pidx(P.pitch, i, j);
-            P.Rp[idx] = rp;
-            P.Rm[idx] = rm;
+__device__ __forceinline__ void limiter_stream(const vx_patch &P, const LimTiles &T, int tile, double sinv) {
+    const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
+    const int lane = threadIdx.x & 63;
+    const int bx = tile % T.nbx, by = tile / T.nbx;
+    const int strip = by * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (strip >= T.nstrip) return;  // uniform per wavefront
+    const int i0 = -2 + strip * T.TI;
+    const int i1 = min(i0 + T.TI, nx + 2);  // C rows [i0, i1)
+    const int j = -2 + bx * LIMW - 2 + lane;
+    const bool own = lane >= 2 && lane < 2 + LIMW && j < np + 2;
+    const bool in_j = j >= -2 && j < np + 2;
+    const bool r_j = j >= -1 && j < np + 1;   // R's p range
+    const bool c_j = j >= 0 && j < np;        // interior faces in p
+    const double *f0 = P.f[0], *f2 = P.f[2], *FxDS = P.FxDS, *FpDS = P.FpDS;
+    auto ld = [&](const double *pl, int i) {
+        return (i >= -2 && i < nx + 2 && in_j) ? pl[pidx(pitch, i, j)] : 0.0;
+    };
+    // at row i: rows i-1, i, i+1 of f^n and f^td, rows i, i+1 of FxDS and
+    // FpDS in registers; rows i+2 (q) and i+3 (issued at the top of the step)
+    // in flight
+    const int r0 = i0 - 1;
+    double a0_m = ld(f0, r0 - 1), a0_c = ld(f0, r0), a2_m = ld(f2, r0 - 1), a2_c = ld(f2, r0);
+    double x_c = ld(FxDS, r0), p_c = ld(FpDS, r0);
+    double a0_p = ld(f0, r0 + 1), a2_p = ld(f2, r0 + 1), x_p = ld(FxDS, r0 + 1), p_p = ld(FpDS, r0 + 1);
+    double a0_q = ld(f0, r0 + 2), a2_q = ld(f2, r0 + 2), x_q = ld(FxDS, r0 + 2), p_q = ld(FpDS, r0 + 2);
+    double Rp_m = 0.0, Rm_m = 0.0;  // R of row i - 1
+    for (int i = r0; i < i1; i++) {
+        const double na0 = ld(f0, i + 3), na2 = ld(f2, i + 3), nx_ = ld(FxDS, i + 3), np_ = ld(FpDS, i + 3);
+        // R+- of (i, j) (Rectangle.cpp:1538-1579)
+        const double p_u = shdn1(p_c);  // FpDS(i, j+1)
+        double rp = 0.0, rm = 0.0;
+        {
+            const double Pp = valmax(0.0, x_c) - valmin(0.0, x_p) + valmax(0.0, p_c) - valmin(0.0, p_u);
+            const double Pm = valmax(0.0, x_p) - valmin(0.0, x_c) + valmax(0.0, p_u) - valmin(0.0, p_c);
+            const double d0 = shdn1(a0_c), d2 = shdn1(a2_c);  // (i, j+1)
+            const double e0 = shup1(a0_c), e2 = shup1(a2_c);  // (i, j-1)
+            const double wMax = valmax(valmax(a0_c, a2_c), valmax(valmax(a0_p, a2_p), valmax(valmax(a0_m, a2_m),
+                                valmax(valmax(d0, d2), valmax(e0, e2)))));
+            const double wMin = valmin(valmin(a0_c, a2_c), valmin(valmin(a0_p, a2_p), valmin(valmin(a0_m, a2_m),
+                                valmin(valmin(d0, d2), valmin(e0, e2)))));
+            const double Qm = -wMin + a2_c;
+            const double Qp = wMax - a2_c;
+            if (i >= -1 && i < nx + 1 && r_j) {
+                if (MP) {
+                    rp = Pp > 0.0 ? zalesak_mp(Qp, Pp, sinv) : 0.0;
+                    rm = Pm > 0.0 ? zalesak_mp(Qm, Pm, sinv) : 0.0;
+                } else {
+                    rp = Pp > 0.0 ? valmin(1.0, Qp / Pp) : 0.0;
+                    rm = Pm > 0.0 ? valmin(1.0, Qm / Pm) : 0.0;
+                }
+            }
         }
+        if (i >= i0) {
+            const long idx = pidx(pitch, i, j);
+            if (own && i >= -1 && i < nx + 1 && r_j) {
+                // only the cross-patch C sync reads R again (at patch edges):
+                // streamed past the caches, which keep C and the fluxes that
+                // the corrector reads next
+                __builtin_nontemporal_store(rp, &P.Rp[idx]);
+                __builtin_nontemporal_store(rm, &P.Rm[idx]);
+            }
+            // C of (i, j): R(i, j), R(i-1, j), R(i, j-1) (Rectangle.cpp:1581-1594)
+            const double Rp_l = shup1(rp), Rm_l = shup1(rm);
+            double cx = 1.0, cp = 1.0;
+            if (i >= 1 && i < nx && c_j) {
+                cx = x_c > 0.0 ? valmin(rp, Rm_m) : valmin(Rp_m, rm);
+                cp = p_c > 0.0 ? valmin(rp, Rm_l) : valmin(Rp_l, rm);
+            }
+            if (own) {
+                P.Cx[idx] = cx;
+                P.Cp[idx] = cp;
+            }
+        }
+        Rp_m = rp;
+        Rm_m = rm;
+        a0_m = a0_c; a0_c = a0_p; a0_p = a0_q; a0_q = na0;
+        a2_m = a2_c; a2_c = a2_p; a2_p = a2_q; a2_q = na2;
+        x_c = x_p; x_p = x_q; x_q = nx_;
+        p_c = p_p; p_p = p_q; p_q = np_;
     }
-    __syncthreads();
-    if (!own || i >= P.n_x + 2 || j >= P.n_p + 2) return;
-    const long idx = pidx(P.pitch, i, j);
-    double cx = 1.0, cp = 1.0;
-    if (i >= 1 && i < P.n_x && j >= 0 && j < P.n_p) {
-        // R(i, j) = s[r][c], R(i-1, j) = s[r-1][c], R(i, j-1) = s[r][c-1]
-        cx = P.FxDS[idx] > 0.0 ? valmin(sRp[r][c], sRm[r - 1][c]) : valmin(sRp[r - 1][c], sRm[r][c]);
-        cp = P.FpDS[idx] > 0.0 ? valmin(sRp[r][c], sRm[r][c - 1]) : valmin(sRp[r][c - 1], sRm[r][c]);
-    }
-    P.Cx[idx] = cx;
-    P.Cp[idx] = cp;
 }
 
 // f1 = f2 + limited antidiffusive fluxes on interior cells
@@ -354,8 +379,8 @@ __device__ __forceinline__ void corrector(const vx_patch &P, int bx, int by) {
 __global__ __launch_bounds__(256) void k_lo_predictor(vx_patch P, double aSum) {
     lo_predictor(P, aSum, (int)blockIdx.x, (int)blockIdx.y);
 }
-__global__ __launch_bounds__(LR * 64) void k_limiter_rc(vx_patch P) {
-    limiter_rc<false>(P, (int)blockIdx.x, (int)blockIdx.y, 1.0);
+__global__ __launch_bounds__(256) void k_limiter(vx_patch P, LimTiles T) {
+    limiter_stream<false>(P, T, (int)blockIdx.x, 1.0);
 }
 __global__ __launch_bounds__(256) void k_corrector(vx_patch P) { corrector(P, (int)blockIdx.x, (int)blockIdx.y); }
 
@@ -379,13 +404,12 @@ __global__ __launch_bounds__(256) void k_lo_predictor_batch(const vx_patch *tab,
     lo_predictor(tab[k], aSum, bx, by);
 }
 template <bool MP>
-__global__ __launch_bounds__(LR * 64) void k_limiter_rc_batch(const vx_patch *tab, const vx_batch_info *bi, int n,
-                                                               double sinv) {
+__global__ __launch_bounds__(256) void k_limiter_batch(const vx_patch *tab, const vx_batch_info *bi, int n,
+                                                       double sinv) {
     const int w = (int)blockIdx.x;
     const int k = batch_patch<2>(bi, n, w);
-    const int local = w - bi[k].start[2];
-    const int gx = lim_gx(tab[k].n_p);
-    limiter_rc<MP>(tab[k], local % gx, local / gx, sinv);
+    const LimTiles T{bi[k].lTI, bi[k].lnstrip, bi[k].lnbx};
+    limiter_stream<MP>(tab[k], T, w - bi[k].start[2], sinv);
 }
 __global__ __launch_bounds__(256) void k_corrector_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
     int k, bx, by;
@@ -450,6 +474,20 @@ Sub0Tiles sub0_tiles(const vx_patch &p) {
     return T;
 }
 
+// strips of the streaming limiter: the tallest that still gives 4096 wavefronts
+LimTiles lim_tiles(const vx_patch &p) {
+    LimTiles T;
+    T.nbx = (p.n_p + 4 + LIMW - 1) / LIMW;
+    T.TI = 4;
+    for (int ti : {32, 16, 8})
+        if ((long)T.nbx * ((p.n_x + 4 + ti - 1) / ti) >= 4096) {
+            T.TI = ti;
+            break;
+        }
+    T.nstrip = (p.n_x + 4 + T.TI - 1) / T.TI;
+    return T;
+}
+
 // the work list of a patch table (built once: a table's shapes never change)
 int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
     if (!t || t->n < 1) return fail(ctx, VX_EINVAL, "batched sub-step: empty patch table");
@@ -470,10 +508,14 @@ int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
         b.TI = T.TI;
         b.nstrip = T.nstrip;
         b.nbx = T.nbx;
+        const LimTiles TL = lim_tiles(p);
+        b.lTI = TL.TI;
+        b.lnstrip = TL.nstrip;
+        b.lnbx = TL.nbx;
         dim3 g0 = grid_cells(p.n_x, p.n_p, 0, 0), g4 = grid_cells(p.n_x, p.n_p, 4, 4);
         run[0] += T.nbx * ((T.nstrip + 3) / 4);
         run[1] += (int)(g0.x * g0.y);
-        run[2] += lim_gx(p.n_p) * lim_gy(p.n_x);
+        run[2] += TL.nbx * ((TL.nstrip + 3) / 4);
         run[3] += (int)(g4.x * g4.y);
         run[4] += (int)(((long)(p.n_x + 4) * p.pitch + 255) / 256);
     }
@@ -572,8 +614,9 @@ int vx_fct_sub0(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fi
 int vx_fct_sub1(vx_ctx *ctx, const vx_patch *p) {
     int rc = validate_patch(ctx, p);
     if (rc) return rc;
-    k_limiter_rc<<<dim3((unsigned)lim_gx(p->n_p), (unsigned)lim_gy(p->n_x)), LR * 64, 0, ctx->stream>>>(*p);
-    return check_launch(ctx, "k_limiter_rc");
+    const LimTiles T = lim_tiles(*p);
+    k_limiter<<<(unsigned)(T.nbx * ((T.nstrip + 3) / 4)), 256, 0, ctx->stream>>>(*p, T);
+    return check_launch(ctx, "k_limiter");
 }
 
 int vx_fct_sub2(vx_ctx *ctx, const vx_patch *p) {
@@ -640,9 +683,8 @@ int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double
 int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
     int rc = table_batch(ctx, t, n_f);
     if (rc) return rc;
-    k_limiter_rc_batch<false>
-        <<<(unsigned)t->batch[t->n].start[2], LR * 64, 0, ctx->stream>>>(t->d, t->batch_d, t->n, 1.0);
-    return check_launch(ctx, "k_limiter_rc_batch");
+    k_limiter_batch<false><<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, 1.0);
+    return check_launch(ctx, "k_limiter_batch");
 }
 
 int vx_fct_sub1_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_scale) {
@@ -651,9 +693,8 @@ int vx_fct_sub1_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_s
     int rc = mp_scale(ctx, f_scale, sinv, eps32, "sub1 table (mixed)");
     if (!rc) rc = table_batch(ctx, t, n_f);
     if (rc) return rc;
-    k_limiter_rc_batch<true>
-        <<<(unsigned)t->batch[t->n].start[2], LR * 64, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
-    return check_launch(ctx, "k_limiter_rc_batch");
+    k_limiter_batch<true><<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
+    return check_launch(ctx, "k_limiter_batch");
 }
 
 int vx_fct_sub2_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {

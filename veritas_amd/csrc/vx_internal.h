@@ -58,8 +58,9 @@ struct vx_graph {
 // batched sub-step launches (vx_fct.hip): batch_d holds, per patch, where its
 // workgroups start in each batched launch and its strip shape.
 struct vx_batch_info {
-    int start[5];        // first workgroup of the patch: sub-step-0 tiles, cells +0, limiter blocks, cells +4, plane copy
+    int start[5];        // first workgroup of the patch: sub-step-0 tiles, cells +0, limiter tiles, cells +4, plane copy
     int TI, nstrip, nbx; // strips of the streaming sub-step-0 kernel
+    int lTI, lnstrip, lnbx;  // strips of the streaming limiter
 };
 struct vx_patch_table {
     vx_ctx *ctx = nullptr;
