@@ -160,12 +160,14 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
         // position (ix, j): stores and the RK combination (Rectangle.cpp:1396-1517)
         if (ix >= i0 && ix < i1 && own_j) {
             const long id = pidx(pitch, ix, j);
+            // FxH_s/FpH_s are read again only by the later stages: streamed
+            // past the caches, which keep what the next kernels read
             if (vFx) {
-                P.FxH[STEP][id] = fxh;
+                __builtin_nontemporal_store(fxh, &P.FxH[STEP][id]);
                 if (STEP == 0) P.FxL0[id] = fxl;
             }
             if (vFp) {
-                P.FpH[STEP][id] = fph;
+                __builtin_nontemporal_store(fph, &P.FpH[STEP][id]);
                 if (STEP == 0) P.FpL0[id] = fpl;
             }
             const double xLS = A.aSum * (STEP == 0 ? fxl : clx);

@@ -235,8 +235,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
         if (store_flux) {
             const bool own_x = ix >= i0 && (ix < i0 + A.TI || last_strip);
             const bool own_c = (ix >= i0 || i0 == 0) && (ix < i0 + A.TI || last_strip);
-            if (vFx && own_x && own_row_x) P.FxH[STEP][pidx(pitch, ix, j)] = FxH;
-            if (vFp && own_c && own_row_p) P.FpH[STEP][pidx(pitch, ix, j)] = FpH;
+            // read again only by the later stages: streamed past the caches
+            if (vFx && own_x && own_row_x) __builtin_nontemporal_store(FxH, &P.FxH[STEP][pidx(pitch, ix, j)]);
+            if (vFp && own_c && own_row_p) __builtin_nontemporal_store(FpH, &P.FpH[STEP][pidx(pitch, ix, j)]);
         }
 
         // ---- low-order fluxes of stage 0 at x-face r and column r
