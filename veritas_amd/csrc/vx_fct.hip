@@ -256,15 +256,16 @@ __device__ __forceinline__ void lo_predictor(const vx_patch &P, double aSum, int
 // [-1, n_x+1) x [-1, n_p+1), then C on the NS range [-2, n_x+2) x [-2, n_p+2):
 // 1 everywhere, the limiter formula on interior faces i in [1, n_x), j in
 // [0, n_p).  Register-streaming form: a wavefront owns 64 consecutive p
-// positions, lane l <-> j = jw - 2 + l, and produces lanes 2..62 (61
-// positions; p neighbours come from DPP lane shifts).  It marches along x over
+// positions, lane l <-> j = jw - 2 + l, and produces lanes 2..49 (48
+// positions, so that each wavefront's row segments start on a 64-byte
+// boundary of the pitched row; p neighbours come from DPP lane shifts).  It marches along x over
 // a strip of C rows [i0, i1): at row i it forms R of row i from rows i-1..i+1
 // of f^n, f^td and the antidiffusive fluxes (held in registers, one new row
 // loaded per step), and C of row i from R of rows i and i-1.  R is stored
 // where it is formed inside the strip (the cross-patch C sync reads it); the
 // first row of a strip (i0 - 1) only warms the pipeline.  MP: the ratios Q / P
 // in FP32 on the values scaled by sinv (vxd::zalesak_mp).
-constexpr int LIMW = 61;  // positions per limiter wavefront
+constexpr int LIMW = 48;  // positions per limiter wavefront (of 62 it could form)
 
 struct LimTiles {
     int TI, nstrip, nbx;
