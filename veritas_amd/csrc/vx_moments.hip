@@ -9,6 +9,7 @@
 // lane 0 then adds them in order.  4096 columns give 4096 independent chains,
 // which keeps every CU busy while each chain stays in reference order.
 #include <cstdlib>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -187,21 +188,26 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     // 183-204, 258-274): GetInterpolantsREL(f1..f5)[k] (Rectangle.cpp:139-155)
     // plus the correction that makes the sub-cell values average to the cell
     // value (Rectangle.cpp:195-204), formed once per cell for all k
+    // (i0 + c, j) is a cell whose value the sums use (others count as 0)
+    auto cell_ok = [&](int c, int j) { return i0 + c < P.n_x && j < P.n_p + 2; };
     auto column = [&](int c, int j, double *out) {
+        // loads at indices clamped into the plane, with no branches, so that
+        // the loads of the chunks in flight are counted one by one and a chunk
+        // waits only for its own
         const int i = i0 + c;
-        if (i >= P.n_x || j >= P.n_p + 2) {
-            for (int k = 0; k < RTB; k++) out[k] = 0.0;
-            return;
-        }
+        const bool ok = cell_ok(c, j);
+        const int ic = i < P.n_x ? i : P.n_x - 1, jc = j < P.n_p + 2 ? j : P.n_p + 1;
         if (RTB == 1) {
             // interpolant coefficients and correction are exactly 0 for
-            // relativeToBottom == 1: the sub-cell value is the cell value
-            out[0] = g[pidx(pitch, i, j)];
+            // relativeToBottom == 1: the sub-cell value is the cell value,
+            // masked by cell_ok where it is staged (not here, which would wait
+            // for the load)
+            out[0] = g[pidx(pitch, ic, jc)];
             return;
         }
-        double f1 = g[pidx(pitch, i - 2, j)], f2 = g[pidx(pitch, i - 1, j)];
-        const double f3 = g[pidx(pitch, i, j)];
-        double f4 = g[pidx(pitch, i + 1, j)], f5 = g[pidx(pitch, i + 2, j)];
+        double f1 = g[pidx(pitch, ic - 2, jc)], f2 = g[pidx(pitch, ic - 1, jc)];
+        const double f3 = g[pidx(pitch, ic, jc)];
+        double f4 = g[pidx(pitch, ic + 1, jc)], f5 = g[pidx(pitch, ic + 2, jc)];
         f5 -= f3; f4 -= f3; f2 -= f3; f1 -= f3;
         const double a1 = kInterp[0] * f1 + kInterp[1] * f2 + kInterp[2] * f4 + kInterp[3] * f5;
         const double a2 = kInterp[4] * f1 + kInterp[5] * f2 + kInterp[6] * f4 + kInterp[7] * f5;
@@ -214,7 +220,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         }
         const double corr = f3 - A.inv_rtb * te;
 #pragma unroll
-        for (int k = 0; k < RTB; k++) out[k] = v[k] + corr;
+        for (int k = 0; k < RTB; k++) out[k] = ok ? v[k] + corr : 0.0;
     };
     // a chunk in registers: values for p in [j0, j0+66), the table entries
     // tab[j0+1 .. j0+67) (clamped to the table) and the nested flag of each
@@ -228,20 +234,22 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             double v[RTB];
             column(c, j0 + lane, v);
             for (int k = 0; k < RTB; k++) pr.v[c * RTB + k][0] = v[k];
-            if (lane < 2) column(c, j0 + 64 + lane, v);
-            for (int k = 0; k < RTB; k++) pr.v[c * RTB + k][1] = lane < 2 ? v[k] : 0.0;
+            column(c, j0 + 64 + min(lane, 1), v);  // read from lanes 0 and 1 only
+            for (int k = 0; k < RTB; k++) pr.v[c * RTB + k][1] = v[k];
         }
-        if (use_tab) {
-            const int t0 = min(j0 + 1 + lane, P.n_p + 2), t1 = min(j0 + 65 + lane, P.n_p + 2);
-            pr.t[0] = tab[t0];
-            pr.t[1] = lane < 2 ? tab[t1] : 0.0;
-        }
-        if (P.nested) {
-            const int jn = min(j0 + 1 + lane, P.n_p - 1);
-            for (int c = 0; c < C; c++) pr.n[c] = P.nested[pidx(pitch, min(i0 + c, P.n_x - 1), jn)];
-        } else {
-            for (int c = 0; c < C; c++) pr.n[c] = 0;
-        }
+        // without a table or nested flags the loads read the plane instead
+        // (and are not used): no branch around them
+        const double *tq = use_tab ? tab : g;
+        const int t0 = min(j0 + 1 + lane, P.n_p + 2), t1 = min(j0 + 65 + min(lane, 1), P.n_p + 2);
+        pr.t[0] = tq[t0];
+        pr.t[1] = tq[t1];  // read from lanes 0 and 1 only
+        const uint8_t *nq = P.nested ? P.nested : (const uint8_t *)g;
+        const int jn = min(j0 + 1 + lane, P.n_p - 1);
+        for (int c = 0; c < C; c++) pr.n[c] = nq[pidx(pitch, min(i0 + c, P.n_x - 1), jn)];
+    };
+    // value set `set` of chain ch held for p-cell j (0 outside the sums' cells)
+    auto staged = [&](const Pre &pr, int ch, int j, int set) {
+        return (RTB > 1 || cell_ok(ch / RTB, j)) ? pr.v[ch][set] : 0.0;
     };
     // summands of p-cell j = jb + lane for every chain (Rectangle.cpp:214-253),
     // into T[b] (or the lanes' compensated sums); no branches and LDS reads only
@@ -264,6 +272,37 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
                 T[b][wave][0][ch][lane] = tr;
                 T[b][wave][1][ch][lane] = tj;
             }
+        }
+    };
+    // entries e + 1 and e + 2 of a 66-entry sequence whose entries 0..63 are v0
+    // of lanes 0..63 and entries 64, 65 are v1 of lanes 0 and 1 (lane e's view)
+    auto next2 = [&](double v0, double v1, double &n1, double &n2) {
+        const double x64 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v1), 0),
+                                            __builtin_amdgcn_readlane(__double2loint(v1), 0));
+        const double x65 = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v1), 1),
+                                            __builtin_amdgcn_readlane(__double2loint(v1), 1));
+        const double s1 = shdn1(v0), s2 = shdn1(s1);
+        n1 = lane == 63 ? x64 : s1;
+        n2 = lane == 63 ? x65 : (lane == 62 ? x64 : s2);
+    };
+    // summands as above from registers (pair form with a logarithm table): the
+    // same operands, so the same bits, without staging them in LDS
+    auto summands_reg = [&](int jb, int b, const int *nflag, const double (*sv)[2], double st0, double st1) {
+        const int j = jb + lane;
+        const bool live = j < P.n_p;
+        double gm, gp1;
+        next2(st0, st1, gm, gp1);
+        const double gm1 = st0;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ch++) {
+            const int c = ch / RTB;
+            const bool skip = !live || i0 + c >= P.n_x || nflag[c] != 0;
+            double t0, tp;
+            next2(sv[ch][0], sv[ch][1], t0, tp);
+            const double tm = sv[ch][0];
+            const double jv = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
+            T[b][wave][0][ch][lane] = skip ? -0.0 : t0;
+            T[b][wave][1][ch][lane] = skip ? -0.0 : jv;
         }
     };
     // FUSE: the ordered sums of buffer b ^ 1 (the previous chunk) with
@@ -321,15 +360,15 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     // chunk jb from pr (summand buffer bb), whose registers then take chunk jb + 128
     auto chunk = [&](int jb, int bb, Pre &pr) {
         for (int ch = 0; ch < NCH; ch++) {
-            S[wave][ch][lane] = pr.v[ch][0];
-            if (lane < 2) S[wave][ch][64 + lane] = pr.v[ch][1];
+            S[wave][ch][lane] = staged(pr, ch, jb - 1 + lane, 0);
+            if (lane < 2) S[wave][ch][64 + lane] = staged(pr, ch, jb + 63 + lane, 1);
         }
         if (use_tab) {
             Lb[wave][0][lane] = pr.t[0];
             if (lane < 2) Lb[wave][0][64 + lane] = pr.t[1];
         }
         int nflag[C];
-        for (int c = 0; c < C; c++) nflag[c] = pr.n[c];
+        for (int c = 0; c < C; c++) nflag[c] = P.nested ? pr.n[c] : 0;
         if (jb + 128 < P.n_p) prefetch(jb + 127, pr);
         // logarithm terms, computed once per distinct A^2 among the chains
         for (int ch = 0; ch < NCH && !use_tab; ch++) {
@@ -360,25 +399,40 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     Pre pa, pb;
     if (producer) {
         prefetch(-1, pa);
-        if (64 < P.n_p) prefetch(63, pb);
+        prefetch(63, pb);  // clamped loads: also when there is no second chunk
     }
     int last = 0;  // summand buffer of the last chunk
     // pair form: the producer fills buffer bb with chunk jb while the consumer
-    // adds the other buffer (chunk jb - 64); the barrier hands them over
-    auto pair_chunk = [&](int jb, int bb, Pre &pr) {
-        if (producer) {
+    // adds the other buffer (chunk jb - 64); the barrier hands them over.  Each
+    // wavefront runs a loop of its own (the same number of barriers), and the
+    // producer's loop is specialised on the table case, so that no branch joins
+    // the registers of the chunks in flight (a join would copy them, and the
+    // copy would wait for the loads).
+    auto prod_chunk = [&](auto tab_c, int jb, int bb, Pre &pr) __attribute__((always_inline)) {
+        constexpr bool TAB = decltype(tab_c)::value;
+        if constexpr (TAB) {
+            // the chunk's cell values and table entries stay in registers: the
+            // neighbours p + 1 and p + 2 come from DPP lane shifts, and the two
+            // entries past the last lane from lanes 0 and 1 of the second set
+            double sv[NCH][2];
             for (int ch = 0; ch < NCH; ch++) {
-                S[0][ch][lane] = pr.v[ch][0];
-                if (lane < 2) S[0][ch][64 + lane] = pr.v[ch][1];
+                sv[ch][0] = staged(pr, ch, jb - 1 + lane, 0);
+                sv[ch][1] = staged(pr, ch, jb + 63 + min(lane, 1), 1);
             }
-            if (use_tab) {
-                Lb[0][0][lane] = pr.t[0];
-                if (lane < 2) Lb[0][0][64 + lane] = pr.t[1];
+            const double st0 = pr.t[0], st1 = pr.t[1];
+            int nflag[C];
+            for (int c = 0; c < C; c++) nflag[c] = P.nested ? pr.n[c] : 0;
+            prefetch(jb + 127, pr);  // clamped loads: also past the last chunk
+            summands_reg(jb, bb, nflag, sv, st0, st1);
+        } else {
+            for (int ch = 0; ch < NCH; ch++) {
+                S[0][ch][lane] = staged(pr, ch, jb - 1 + lane, 0);
+                if (lane < 2) S[0][ch][64 + lane] = staged(pr, ch, jb + 63 + lane, 1);
             }
             int nflag[C];
-            for (int c = 0; c < C; c++) nflag[c] = pr.n[c];
-            if (jb + 128 < P.n_p) prefetch(jb + 127, pr);
-            for (int ch = 0; ch < NCH && !use_tab; ch++) {
+            for (int c = 0; c < C; c++) nflag[c] = P.nested ? pr.n[c] : 0;
+            prefetch(jb + 127, pr);  // clamped loads: also past the last chunk
+            for (int ch = 0; ch < NCH; ch++) {
                 if (ch > 0 && asq[ch] == asq[ch - 1]) {
                     for (int e = lane; e < 66; e += 64) Lb[0][ch][e] = Lb[0][ch - 1][e];
                     continue;
@@ -393,19 +447,37 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
             }
             wave_sync();
             summands(jb, bb, nflag);
-        } else if (jb > 0) {
-            ordered(bb ^ 1);
         }
         __syncthreads();
     };
-    for (int jb = 0; jb < P.n_p; jb += 128) {
-        if (PAIR) pair_chunk(jb, 0, pa);
-        else chunk(jb, 0, pa);
-        last = 0;
-        if (jb + 64 < P.n_p) {
-            if (PAIR) pair_chunk(jb + 64, 1, pb);
-            else chunk(jb + 64, 1, pb);
-            last = 1;
+    auto prod_loop = [&](auto tab_c) __attribute__((always_inline)) {
+        for (int jb = 0; jb < P.n_p; jb += 128) {
+            prod_chunk(tab_c, jb, 0, pa);
+            if (jb + 64 < P.n_p) prod_chunk(tab_c, jb + 64, 1, pb);
+        }
+    };
+    if (PAIR && producer) {
+        if (use_tab) prod_loop(std::true_type{});
+        else prod_loop(std::false_type{});
+    } else if (PAIR) {
+        for (int jb = 0; jb < P.n_p; jb += 128) {
+            if (jb > 0) ordered(1);
+            __syncthreads();
+            last = 0;
+            if (jb + 64 < P.n_p) {
+                ordered(0);
+                __syncthreads();
+                last = 1;
+            }
+        }
+    } else {
+        for (int jb = 0; jb < P.n_p; jb += 128) {
+            chunk(jb, 0, pa);
+            last = 0;
+            if (jb + 64 < P.n_p) {
+                chunk(jb + 64, 1, pb);
+                last = 1;
+            }
         }
     }
     if (!COMP && (FUSE || PAIR) && consumer) ordered(last);
