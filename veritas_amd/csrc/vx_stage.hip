@@ -71,6 +71,15 @@ __device__ __forceinline__ double ldp(const double *pl, int pitch, int nx, int n
     return (i >= -2 && i < nx + 2 && j >= -2 && j < np + 2) ? pl[pidx(pitch, i, j)] : 0.0;
 }
 
+// Row i of a plane at the clamped column jc, with i clamped into the plane:
+// no branch around the load, so that the loads in flight are counted one by
+// one and a row waits only for its own; the caller masks the value (0 outside
+// the plane) where it uses it, as ldp would have.
+__device__ __forceinline__ double ldr(const double *pl, int pitch, int nx, int i, int jc) {
+    const int ic = i < -2 ? -2 : (i > nx + 1 ? nx + 1 : i);
+    return pl[pidx(pitch, ic, jc)];
+}
+
 // q^2 A^2 at x-face i (Rectangle.cpp:1280, 1296-1297; EMSolver.cpp:133-135)
 __device__ __forceinline__ double face_as(const StageArgs &A, const double *a2, int i) {
     int fi = A.p.rtb * (i + A.p.x_pos);
@@ -158,17 +167,21 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
 
     // software prefetch: rows r and r+1 are in flight before iteration r, the
     // stored fluxes of x-face / column r-1 one iteration ahead
-    double pfs_a = ldp(fin, pitch, nx, np, r0, j), pf0_a = ldp(f0p, pitch, nx, np, r0, j);
-    double pfs_b = ldp(fin, pitch, nx, np, r0 + 1, j), pf0_b = ldp(f0p, pitch, nx, np, r0 + 1, j);
+    const bool in_j = j >= -2 && j < np + 2;
+    const int jc = j < -2 ? -2 : (j > np + 1 ? np + 1 : j);
+    auto row_ok = [&](int i) { return i >= -2 && i < nx + 2 && in_j; };
+    double pfs_a = ldr(fin, pitch, nx, r0, jc), pf0_a = ldr(f0p, pitch, nx, r0, jc);
+    double pfs_b = ldr(fin, pitch, nx, r0 + 1, jc), pf0_b = ldr(f0p, pitch, nx, r0 + 1, jc);
     constexpr int NH = STEP > 0 ? STEP : 1;  // stored flux planes read by this stage
     double hx[NH], hp[NH];
+    bool hvx = false, hvp = false;  // the loaded face lies in the FxH / FpH ranges
     auto load_h = [&](int face) {
-        const bool vx_ = face >= 0 && face < nx + 1 && jr_ex;
-        const bool vp_ = face >= -1 && face < nx + 1 && jr_fp;
+        hvx = face >= 0 && face < nx + 1 && jr_ex;
+        hvp = face >= -1 && face < nx + 1 && jr_fp;
 #pragma unroll
         for (int k = 0; k < NH; k++) {
-            hx[k] = (k < STEP && vx_) ? P.FxH[k][pidx(pitch, face, j)] : 0.0;
-            hp[k] = (k < STEP && vp_) ? P.FpH[k][pidx(pitch, face, j)] : 0.0;
+            hx[k] = (k < STEP) ? ldr(P.FxH[k], pitch, nx, face, jc) : 0.0;
+            hp[k] = (k < STEP) ? ldr(P.FpH[k], pitch, nx, face, jc) : 0.0;
         }
     };
     load_h(r0 - 1);
@@ -181,20 +194,20 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
     // One march row; each quantity is masked to its reference loop range.
     auto row = [&](int r) {
         // ---- ingest row r
-        fs_0 = pfs_a;
-        f0_0 = pf0_a;
+        fs_0 = row_ok(r) ? pfs_a : 0.0;
+        f0_0 = row_ok(r) ? pf0_a : 0.0;
         if (MP) gs_0 = (float)(fs_0 * sinv);
         pfs_a = pfs_b;
         pf0_a = pf0_b;
-        pfs_b = ldp(fin, pitch, nx, np, r + 2, j);
-        pf0_b = ldp(f0p, pitch, nx, np, r + 2, j);
+        pfs_b = ldr(fin, pitch, nx, r + 2, jc);
+        pf0_b = ldr(f0p, pitch, nx, r + 2, jc);
         if (STEP > 0) {
-            Sx = A.a[0] * hx[0];
-            Sp = A.a[0] * hp[0];
+            Sx = A.a[0] * (hvx ? hx[0] : 0.0);
+            Sp = A.a[0] * (hvp ? hp[0] : 0.0);
 #pragma unroll
             for (int k = 1; k < NH; k++) {
-                Sx = Sx + A.a[k] * hx[k];
-                Sp = Sp + A.a[k] * hp[k];
+                Sx = Sx + A.a[k] * (hvx ? hx[k] : 0.0);
+                Sp = Sp + A.a[k] * (hvp ? hp[k] : 0.0);
             }
         }
         load_h(r);
