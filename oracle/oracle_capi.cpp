@@ -1,5 +1,7 @@
 // CPU ORACLE C API -- test infrastructure only.  Loaded by tests/ (ctypes),
 // __graft_entry__.smoke() and bench.py's CPU-baseline leg.
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -177,6 +179,39 @@ int vxo_flags(void *h, int sp, int lvl, int *out, int n) {
 
 // SolverManager::reGrid (SolverManager.cpp:48-58)
 void vxo_regrid(void *h) { ((Run *)h)->s->reGrid(); }
+
+// For the pins of the Vlasov advance by exact solutions: set slots 0 and 1
+// of the base patch's interior (n_x x n_p, row-major) of species 0; ghosts
+// are left as they are (zero at the walls).  Returns -1 on a size mismatch.
+int vxo_set_f(void *h, const double *f, int n) {
+    orc::Patch &p = base((Run *)h);
+    if (n != p.n_x * p.n_p) return -1;
+    for (int i = 0; i < p.n_x; i++)
+        for (int j = 0; j < p.n_p; j++) {
+            p.f[0][p.NS(i, j)] = f[i * p.n_p + j];
+            p.f[1][p.NS(i, j)] = f[i * p.n_p + j];
+        }
+    return 0;
+}
+
+// The Vlasov part of SolverManager::Advance (SolverManager.cpp:28-39) with an
+// imposed field: at stage i the stage clock is advanced first (Settings.cpp:
+// 166-179, so it reads t_n + c_i dt), then the potential is zeroed and Ex0 set
+// to E0 cos(omega t), so that every cell sees that uniform field
+// (EMSolver.cpp:137-154).  The transverse fields stay as they are (zero unless
+// a drive ran); the moments, the Poisson solve and the Maxwell stage are
+// skipped.
+void vxo_vlasov_imposed(void *h, double dt, int nsteps, double E0, double omega) {
+    Run *r = (Run *)h;
+    orc::Field &em = *r->s->em;
+    for (int n = 0; n < nsteps; n++)
+        for (int i = 0; i < 6; i++) {
+            r->s->settings.UpdateTime(i, dt);
+            std::fill(em.PHI.begin(), em.PHI.end(), 0.0);
+            em.Ex0 = E0 * std::cos(omega * r->s->settings.time);
+            for (auto &m : r->s->meshes) m->Advance(dt, i);
+        }
+}
 
 // Standalone pieces for unit tests.
 double vxo_weno_nomax(double f1, double f2, double f3, double f4, int right) {

@@ -53,6 +53,9 @@ def lib():
         L.vxo_flags.restype = i
         L.vxo_flags.argtypes = [vp, i, i, vp, i]
         L.vxo_regrid.argtypes = [vp]
+        L.vxo_set_f.restype = i
+        L.vxo_set_f.argtypes = [vp, vp, i]
+        L.vxo_vlasov_imposed.argtypes = [vp, ctypes.c_double, i, ctypes.c_double, ctypes.c_double]
         _lib = L
     return _lib
 
@@ -140,3 +143,15 @@ class OracleRun:
 
     def regrid(self):
         lib().vxo_regrid(self.h)
+
+    # ---- imposed-field runs (pins of the Vlasov advance by exact solutions)
+    def set_f(self, f):
+        """Set slots 0 and 1 of the base patch's interior to `f` (nx x np)."""
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        if lib().vxo_set_f(self.h, f.ctypes.data, f.size) != 0:
+            raise ValueError("f has the wrong size")
+
+    def vlasov_imposed(self, dt, nsteps, E0=0.0, omega=0.0):
+        """Vlasov steps with the uniform field E0 cos(omega t) imposed at every
+        stage (t = the stage clock)."""
+        lib().vxo_vlasov_imposed(self.h, dt, nsteps, E0, omega)
