@@ -10,7 +10,7 @@
 #include <cstdlib>
 #include <stdexcept>
 
-#include "../veritas_amd/csrc/vx_consts.h"
+#include "oracle_consts.h"
 #include "../veritas_amd/csrc/vx_math.h"
 
 namespace orc {
@@ -152,7 +152,7 @@ double Patch::Momentum(double i) const { return settings->pmin[sp] + dp * (i + p
 
 // Rectangle.hpp:181-183
 double Patch::Gamma(double p, double a2) const {
-    return std::sqrt(1.0 + ((p * p) + a2) * ((m_inv * vx_c_inv) * (m_inv * vx_c_inv)));
+    return std::sqrt(1.0 + ((p * p) + a2) * ((m_inv * kCInv) * (m_inv * kCInv)));
 }
 
 // Rectangle.cpp:307-312; the sentinel (BoundaryCondition.cpp:6-8) returns 0
@@ -210,7 +210,7 @@ void Patch::CalculateRhoAndJ() {
     std::fill(currentR.begin(), currentR.end(), 0.0);
     const double q = settings->q.at(sp);
     const double mass = settings->m.at(sp);
-    const double c1 = m_inv * vx_c_inv;
+    const double c1 = m_inv * kCInv;
     const double c2 = 1 / c1;
     const double c3 = 1 / 48.0;
     const int n = (int)rtb;
@@ -349,7 +349,7 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
         double w3 = 1 / 48.0;
         double dx_inv = 1 / dx;
         double dp_inv = 1 / dp;
-        double cc = vx_cs * vx_cs * settings->m.at(sp);
+        double cc = kC * kC * settings->m.at(sp);
 #pragma omp parallel for schedule(static)
         for (int i = -1; i < nx2; i++) {  // 1278-1286
             double as = q * q * em->GetASquared(GetFinestIndex(i));
@@ -409,7 +409,7 @@ void Patch::FCTTimeStep(double timestep, int step, int subStep) {
         // 1396-1517: RK combination (a_k = coefficient * dt, aSum summed in order)
         double a[6] = {0, 0, 0, 0, 0, 0};
         int ns = step + 1;
-        for (int k = 0; k < ns; k++) a[k] = vx::rk_a[step][k] * timestep;
+        for (int k = 0; k < ns; k++) a[k] = kRK[step][k] * timestep;
         double aSum = a[0];
         for (int k = 1; k < ns; k++) aSum = aSum + a[k];
 #pragma omp parallel for schedule(static)
@@ -713,7 +713,7 @@ void Field::PoissonSolve(const std::vector<double> &b, std::vector<double> &x) c
 
 // EMSolver.cpp:156-192
 void Field::UpdatePotential() {
-    double te = vx_eps0_inv;
+    double te = kEps0Inv;
     std::vector<double> b(x_size);
     for (unsigned i = 0; i < x_size; i++) b[i] = te * (charge[i] + neutralizationCharge[i]);
     double w = std::pow(settings.GetDx(0), 2.0);
@@ -735,7 +735,7 @@ void Field::RGKUpdateIntermediateSolution(int step, double timestep) {
     std::vector<double> *F[6] = {&By, &Bz, &Ey, &Ez, &Ay, &Az};
     if (step >= 0 && step <= 4) {
         double a[5];
-        for (int k = 0; k <= step; k++) a[k] = vx::rk_a[step][k] * timestep;
+        for (int k = 0; k <= step; k++) a[k] = kRK[step][k] * timestep;
         for (unsigned i = 0; i < iloop; i++)
             for (auto *v : F) {
                 std::vector<double> &y = *v;
@@ -745,9 +745,9 @@ void Field::RGKUpdateIntermediateSolution(int step, double timestep) {
             }
     } else if (step == 5) {
         double a[6], b[6];
-        for (int k = 0; k < 5; k++) a[k] = vx::rk_a[4][k] * timestep;
+        for (int k = 0; k < 5; k++) a[k] = kRK[4][k] * timestep;
         a[5] = 0;
-        for (int k = 0; k < 6; k++) b[k] = vx::rk_a[5][k] * timestep - a[k];
+        for (int k = 0; k < 6; k++) b[k] = kRK[5][k] * timestep - a[k];
         for (unsigned i = 0; i < iloop; i++)
             for (auto *v : F) {
                 std::vector<double> &y = *v;
@@ -772,8 +772,8 @@ void Field::RGKCalculateRHS(int step) {
         int s = Index(i, step + 2), i1 = Index(i, 1), ip1 = Index(i + 1, 1);
         By[s] = dx_inv * (Ez[ip1] - Ez[i1]);
         Bz[s] = -dx_inv * (Ey[ip1] - Ey[i1]);
-        Ey[s] = -dx_inv * (Bz[i1] - settings.GetBZ(0, settings.time)) * vx_eps0_inv * vx_mu_inv;
-        Ez[s] = dx_inv * (By[i1] - settings.GetBY(0, settings.time)) * vx_eps0_inv * vx_mu_inv;
+        Ey[s] = -dx_inv * (Bz[i1] - settings.GetBZ(0, settings.time)) * kEps0Inv * kMuInv;
+        Ez[s] = dx_inv * (By[i1] - settings.GetBY(0, settings.time)) * kEps0Inv * kMuInv;
         Ay[s] = -Ey[i1];
         Az[s] = -Ez[i1];
     }
@@ -781,8 +781,8 @@ void Field::RGKCalculateRHS(int step) {
         int s = Index(i, step + 2), i1 = Index(i, 1), ip1 = Index(i + 1, 1), in1 = Index(i - 1, 1);
         By[s] = dx_inv * (Ez[ip1] - Ez[i1]);
         Bz[s] = -dx_inv * (Ey[ip1] - Ey[i1]);
-        Ey[s] = -dx_inv * (Bz[i1] - Bz[in1]) * vx_eps0_inv * vx_mu_inv;
-        Ez[s] = dx_inv * (By[i1] - By[in1]) * vx_eps0_inv * vx_mu_inv;
+        Ey[s] = -dx_inv * (Bz[i1] - Bz[in1]) * kEps0Inv * kMuInv;
+        Ez[s] = dx_inv * (By[i1] - By[in1]) * kEps0Inv * kMuInv;
         Ay[s] = -Ey[i1];
         Az[s] = -Ez[i1];
     }
@@ -793,10 +793,10 @@ void Field::RGKCalculateRHS(int step) {
         int ip2 = Index(i + 2, 1), in2 = Index(i - 2, 1);
         By[s] = dx_inv * (c1 * (Ez[ip2] - Ez[in1]) + c2 * (Ez[ip1] - Ez[i1]));
         Bz[s] = -dx_inv * (c1 * (Ey[ip2] - Ey[in1]) + c2 * (Ey[ip1] - Ey[i1]));
-        Ey[s] = -dx_inv * (c1 * (Bz[ip1] - Bz[in2]) + c2 * (Bz[i1] - Bz[in1])) * vx_eps0_inv * vx_mu_inv -
-                vx_eps0_inv * J[i - n_prepad] * Ay[i1];
-        Ez[s] = dx_inv * (c1 * (By[ip1] - By[in2]) + c2 * (By[i1] - By[in1])) * vx_eps0_inv * vx_mu_inv -
-                vx_eps0_inv * J[i - n_prepad] * Az[i1];
+        Ey[s] = -dx_inv * (c1 * (Bz[ip1] - Bz[in2]) + c2 * (Bz[i1] - Bz[in1])) * kEps0Inv * kMuInv -
+                kEps0Inv * J[i - n_prepad] * Ay[i1];
+        Ez[s] = dx_inv * (c1 * (By[ip1] - By[in2]) + c2 * (By[i1] - By[in1])) * kEps0Inv * kMuInv -
+                kEps0Inv * J[i - n_prepad] * Az[i1];
         Ay[s] = -Ey[i1];
         Az[s] = -Ez[i1];
     }
@@ -804,8 +804,8 @@ void Field::RGKCalculateRHS(int step) {
         int s = Index(i, step + 2), i1 = Index(i, 1), ip1 = Index(i + 1, 1), in1 = Index(i - 1, 1);
         By[s] = dx_inv * (Ez[ip1] - Ez[i1]);
         Bz[s] = -dx_inv * (Ey[ip1] - Ey[i1]);
-        Ey[s] = -dx_inv * (Bz[i1] - Bz[in1]) * vx_eps0_inv * vx_mu_inv;
-        Ez[s] = dx_inv * (By[i1] - By[in1]) * vx_eps0_inv * vx_mu_inv;
+        Ey[s] = -dx_inv * (Bz[i1] - Bz[in1]) * kEps0Inv * kMuInv;
+        Ez[s] = dx_inv * (By[i1] - By[in1]) * kEps0Inv * kMuInv;
         Ay[s] = -Ey[i1];
         Az[s] = -Ez[i1];
     }
@@ -867,10 +867,10 @@ double Field::EstimateCFLBound() const {
         double As = Ayv * Ayv + Azv * Azv;
         double temp(0.0);
         for (int j = 0; j < n; j++)
-            temp = std::max(temp, std::fabs(q[j]) / m[j] / std::sqrt(1 + As / ((m[j] * vx_cs) * (m[j] * vx_cs))) * dps[j]);
+            temp = std::max(temp, std::fabs(q[j]) / m[j] / std::sqrt(1 + As / ((m[j] * kC) * (m[j] * kC))) * dps[j]);
         pc = std::max(pc, temp * std::fabs(Ayv * Bz[i] - Azv * By[i]) + dpsMax * std::fabs(GetEfield(i)));
     }
-    return 1.0 / std::max((vx_cs / settings.GetDx(0) + pc), 1e-40);
+    return 1.0 / std::max((kC / settings.GetDx(0) + pc), 1e-40);
 }
 
 // ---------------------------------------------------------------- Level

@@ -13,7 +13,7 @@
 #include <limits>
 #include <stdexcept>
 
-#include "../veritas_amd/csrc/vx_consts.h"
+#include "oracle_consts.h"
 #include "oracle.hpp"
 
 namespace orc {
@@ -471,7 +471,7 @@ double Patch::RGKGetFluxX(int i, int j) const {
         double dx_inv = 1 / dx;
         double dp_inv = 1 / dp;
         double q = settings->q.at(sp);
-        double cc = vx_cs * vx_cs * settings->m.at(sp);
+        double cc = kC * kC * settings->m.at(sp);
         double as = q * q * em->GetASquared(GetFinestIndex(i));
         double am = dp_inv * cc * (Gamma(Momentum(j + 1), as) - Gamma(Momentum(j), as));
         double ap1 = dp_inv * cc * (Gamma(Momentum(j + 2.0), as) - Gamma(Momentum(j + 1.0), as));
@@ -491,7 +491,7 @@ double Patch::RGKGetFluxP(int i, int j) const {
         double w3 = 1 / 48.0;
         double dp_inv = 1 / dp;
         double dx_inv = 1 / dx;
-        double cc = vx_cs * vx_cs * settings->m.at(sp);
+        double cc = kC * kC * settings->m.at(sp);
         double q = settings->q.at(sp);
         double as_0 = q * q * em->GetASquared(GetFinestIndex(i - 1));
         double as_1 = q * q * em->GetASquared(GetFinestIndex(i));
@@ -518,7 +518,7 @@ double Patch::RGKGetFluxPL(int i, int j) const {
     if (!is_ilb_p[NS(i, j)]) {
         double dp_inv = 1 / dp;
         double dx_inv = 1 / dx;
-        double cc = vx_cs * vx_cs * settings->m.at(sp);
+        double cc = kC * kC * settings->m.at(sp);
         double q = settings->q.at(sp);
         double as_1 = q * q * em->GetASquared(GetFinestIndex(i));
         double as_2 = q * q * em->GetASquared(GetFinestIndex(i + 1));
@@ -535,7 +535,7 @@ double Patch::RGKGetFluxXL(int i, int j) const {
     if (!is_ilb_x[NS(i, j)]) {
         double dx_inv = 1 / dx;
         double dp_inv = 1 / dp;
-        double cc = vx_cs * vx_cs * settings->m.at(sp);
+        double cc = kC * kC * settings->m.at(sp);
         double q = settings->q.at(sp);
         double as = q * q * em->GetASquared(GetFinestIndex(i));
         double am = dp_inv * cc * (Gamma(Momentum(j + 1), as) - Gamma(Momentum(j), as));

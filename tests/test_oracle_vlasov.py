@@ -10,10 +10,10 @@ before every stage, A = 0) so that the exact solution is known:
 
 - free streaming, E = 0: f(x, p, t) = f0(x - v(p) t, p) with v = p / (m gamma);
   exact cell averages come from an erf in x and Gauss-Legendre in p;
-- f linear in x: every consistent Runge-Kutta combination integrates it
-  exactly in time (the result must not depend on dt), and the only error left
-  is the p-average of v f in the x-flux, which the 1/48 product term makes
-  fourth order in dp (Rectangle.cpp:1330-1345);
+- f linear in x: the flux differences are the same at every stage, so the
+  result must not depend on dt, and the only error left is the p-average of
+  v f in the x-flux, which the 1/48 product term makes fourth order in dp
+  (Rectangle.cpp:1330-1345);
 - a uniform force on x-uniform f: f(p, t) = f0(p - q E t);
 - a force E0 cos(omega t) on f linear in p: the semi-discrete system is then a
   quadrature of E over the stages, whose error falls at the order of the
@@ -145,11 +145,10 @@ def linear_x_run(n_p, cfl, nx=64, t_end=2.0, ps=0.12):
 
 
 def test_linear_profile_is_exact_in_time_and_fourth_order_in_dp():
-    """On f linear in x the high-order x-flux differences are the same at
-    every stage up to the profile's drift, so every Runge-Kutta combination
-    whose coefficients sum to the stage weights gives the same answer: dt must
-    not matter (a wrong stage index in sum_k a_k FxH_k changes it).  What is
-    left is the p-average of v f through the x-face, approximated by
+    """On f linear in x the high-order x-flux differences between neighbouring
+    faces are the same at every stage (the profile drifts, its slope stays),
+    so the step is exact in time: dt must not matter.  What is left is the
+    p-average of v f through the x-face, approximated by
     f_face * <v> + (1/48) (f_{j+1} - f_{j-1}) (v_{j+1} - v_{j-1}), which is
     fourth order in dp (a flipped 1/48 term leaves second order, a wrong
     speed factor no convergence)."""
