@@ -70,22 +70,82 @@ def measured_valu(kernel, nx, np_, mixed):
     return None
 
 
-def cpu_baseline(params, steps):
-    """Time the CPU oracle (test infrastructure, used here only as the
-    baseline) on `steps` steps of the same workload."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from _oracle import OracleRun
-    o = OracleRun(params)
-    dt = o.calc_dt(0.4)
+def host_cpu():
+    """Model, physical cores and threads per core of this host (lscpu)."""
+    import subprocess
+    info = {"model": None, "physical_cores": os.cpu_count() or 1, "threads_per_core": 1}
+    try:
+        out = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=20).stdout
+        kv = dict(l.split(":", 1) for l in out.splitlines() if ":" in l)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        tpc = int(kv.get("Thread(s) per core", "1"))
+        cps = int(kv.get("Core(s) per socket", "0"))
+        sockets = int(kv.get("Socket(s)", "1"))
+        info["threads_per_core"] = tpc
+        if cps > 0:
+            info["physical_cores"] = cps * sockets
+        # the CPUs this process may use (a container's share) bound the cores
+        avail = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+        info["physical_cores"] = max(1, min(info["physical_cores"], avail // max(tpc, 1) or 1))
+        info["cpus_available"] = avail
+    except Exception:
+        pass
+    return info
+
+
+_CPU_CHILD = r"""
+import json, sys, time
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[1] + "/tests")
+import veritas_amd.params as P
+from _oracle import OracleRun
+kw = json.loads(sys.argv[2]); workload, steps, reps = kw.pop("workload"), kw.pop("steps"), kw.pop("reps")
+params = getattr(P, workload)(**kw)
+o = OracleRun(params)
+dt = o.calc_dt(0.4)
+times = []
+for _ in range(reps):
     t0 = time.perf_counter()
     o.advance(dt, steps)
-    el = time.perf_counter() - t0
-    o.close()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": params.nx * params.np * steps / el, "unit": "cell-updates/s",
-            "cores": cores, "kind": "port",
-            "sample": f"{steps} steps of the {params.nx}x{params.np} two-stream workload "
-                      f"(FP64 CPU restatement, OpenMP, {el:.1f} s; initialisation excluded)"}
+    times.append(time.perf_counter() - t0)
+print(json.dumps(times))
+"""
+
+
+def cpu_baseline(workload, kw, cells, steps, reps=3, label=""):
+    """Time the CPU oracle (test infrastructure, used here only as the
+    baseline) on `steps` steps of the same workload, `reps` times in one child
+    process after one initialisation; the median is reported.  OpenMP runs one
+    thread per physical core, bound close to its core (OMP_PROC_BIND=close,
+    OMP_PLACES=cores), set before the child starts."""
+    import subprocess
+    cpu = host_cpu()
+    env = dict(os.environ, OMP_NUM_THREADS=str(cpu["physical_cores"]), OMP_PROC_BIND="close",
+               OMP_PLACES="cores")
+    arg = json.dumps(dict(kw, workload=workload, steps=steps, reps=reps))
+    r = subprocess.run([sys.executable, "-c", _CPU_CHILD, ROOT, arg], env=env, stdout=subprocess.PIPE,
+                       text=True, timeout=900, check=True)
+    times = sorted(json.loads(r.stdout.strip().splitlines()[-1]))
+    med = times[len(times) // 2]
+    return {"value": cells * steps / med, "unit": "cell-updates/s", "cores": cpu["physical_cores"],
+            "kind": "port", "cpu_model": cpu["model"], "threads_per_core": cpu["threads_per_core"],
+            "omp": "OMP_NUM_THREADS=physical cores, OMP_PROC_BIND=close, OMP_PLACES=cores",
+            "runs_s": times,
+            "sample": f"{steps} steps of {label}, median of {reps} runs after one initialisation "
+                      f"(FP64 CPU restatement, OpenMP; initialisation excluded)"}
+
+
+def driver_loop_ms(run, dt_max, steps, cfl=0.4):
+    """ms per step of the reference driver's loop (veritas.cpp:131-137): dt =
+    min(CalculateDt(cfl), dt_max) before every step (a device-to-host read of
+    the field state and the bound on the host), then Advance(dt)."""
+    run.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.advance(min(run.calc_dt(cfl), dt_max))
+    run.sync()
+    return (time.perf_counter() - t0) * 1e3 / steps
 
 
 def mixed_error(va, params, steps):
@@ -223,19 +283,8 @@ def bench_amr(a, va):
                                       "launches; FP64 fluxes, updates, task programs, compensated moments, fields")
         out["error_vs_fp64"] = amr_mixed_error(va, params, a.error_steps) if a.error_steps > 0 else None
     if not a.no_cpu:
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from _oracle import OracleRun
-        o = OracleRun(params)
-        odt = o.calc_dt(0.4)
-        t0 = time.perf_counter()
-        o.advance(odt, a.cpu_steps)
-        el = time.perf_counter() - t0
-        o.close()
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        out["cpu_baseline"] = {"value": cells * a.cpu_steps / el, "unit": "cell-updates/s", "cores": cores,
-                               "kind": "port",
-                               "sample": f"{a.cpu_steps} steps of the same three-level laser workload "
-                                         f"(FP64 CPU restatement, OpenMP, {el:.1f} s; setup excluded)"}
+        out["cpu_baseline"] = cpu_baseline("laser", {}, cells, a.cpu_steps,
+                                           label="the same three-level laser workload")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)
@@ -317,7 +366,7 @@ def main():
     ap.add_argument("--error-steps", type=int, default=500, help="mixed: run length of the error report (0: none)")
     ap.add_argument("--nx", type=int, default=4096)
     ap.add_argument("--np", type=int, default=4096)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="CPU baseline: steps per timed run (3 runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--requests", type=int, default=256, help="serve: requests in the scan stream")
     ap.add_argument("--req-steps", type=int, default=40, help="serve: steps per request")
@@ -351,13 +400,18 @@ def main():
     kms, nlaunch = run.timing_end()
     graph_ms = graph_step_ms(run, dt, a.steps)
 
+    driver_ms = driver_loop_ms(run, dt, a.steps)  # graphs stay on: the library's default mode
+
     cells = params.nx * params.np
     value = cells * a.steps / (ms / 1e3)
-    # dominant kernel: algorithmic bytes per launch averaged over the six stages
-    per_cell = sum(kernel_bytes_per_cell(kernel, s) for s in range(6)) / 6.0
-    bytes_per_launch = per_cell * cells
+    # dominant kernel, one launch per RK stage: algorithmic bytes are 216 B per
+    # cell-update (stage s reads f_0..f_s and writes f_{s+1}), 36 B per cell and
+    # launch on average; the design's own interface bytes (stored fluxes) and
+    # the counters' HBM bytes are reported beside them
+    algo_bytes_per_launch = BYTES_PER_UPDATE / 6.0 * cells
+    design_bytes_per_launch = sum(kernel_bytes_per_cell(kernel, s) for s in range(6)) / 6.0 * cells
     avg_launch_s = (kms / nlaunch) / 1e3 if nlaunch else float("nan")
-    achieved = bytes_per_launch / avg_launch_s
+    achieved = algo_bytes_per_launch / avg_launch_s
     out = {
         "metric": "phase-space cell-updates/s",
         "value": value,
@@ -376,11 +430,14 @@ def main():
                    "nx": params.nx, "np": params.np, "mode": a.mode,
                    "hbm_frac_216B": value * BYTES_PER_UPDATE / HBM_PEAK,
                    "wall_s": wall, "graph_ms_per_step": graph_ms,
+                   "ms_per_step_driver_loop": driver_ms,
                    "tile_rows": run.tile_rows() if a.mode == "fused" else None},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(kernel, params.nx, params.np),
-                     "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "algorithmic_bytes_per_launch": algo_bytes_per_launch,
+                     "design_bytes_per_launch": design_bytes_per_launch,
+                     "design_frac": design_bytes_per_launch / avg_launch_s / HBM_PEAK,
                      "avg_launch_ms": kms / nlaunch if nlaunch else None,
                      "launches": nlaunch, "kernel_share": kms / ms if ms else None},
     }
@@ -396,7 +453,8 @@ def main():
         out["config"]["precision"] = "FP32 WENO weights and Zalesak ratios; FP64 fluxes, updates, compensated moments, fields"
         out["error_vs_fp64"] = mixed_error(va, params, a.error_steps) if a.error_steps > 0 else None
     if not a.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(params, a.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline("two_stream", {"nx": params.nx, "np": params.np}, cells,
+                                           a.cpu_steps, label=f"the {params.nx}x{params.np} two-stream workload")
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out), flush=True)

@@ -18,24 +18,29 @@ namespace orc {
 // ---------------------------------------------------------------- Settings
 // Settings.cpp:5-58 with the scenario playing initialConditions and
 // settingsOverride (veritas.cpp:7-73).
-Settings::Settings(const vx::Scenario &s) : scn(&s) {
-    dx = s.dx();
-    dp = {s.dp()};
-    maxDepth = s.levels - 1;
-    x_size = s.nx;
-    p_size = {(unsigned)s.np};
-    refinementRatio = s.r;
-    refinementCriteria = s.refinement_criteria;
-    minEfficiency = s.min_efficiency;
+Settings::Settings(const vx_settings_desc &d0, const vx_hooks &h) : hooks(h) {
+    if (!h.initial_distribution) throw std::runtime_error("InitialDistribution hook is required");
+    vx_settings_desc d = d0;
+    // Settings.cpp:31: settingsOverride runs on the copied inputs
+    if (h.settings_override) h.settings_override(h.user, &d);
+    if (d.n_species < 1 || d.n_species > VX_MAX_SPECIES) throw std::runtime_error("1 to 8 species");
+    dx = d.dx;
+    dp.assign(d.dp, d.dp + d.n_species);
+    maxDepth = (int)d.Lfinest - 1;
+    x_size = d.nx;
+    p_size.assign(d.np, d.np + d.n_species);
+    refinementRatio = d.r;
+    refinementCriteria = d.refinementCriteria;
+    minEfficiency = d.minEfficiency;
     time = 0.0;
-    preLength = s.pre_length;
-    postLength = s.post_length;
-    cfl = s.cfl;
-    sizeWeight = s.size_weight;
-    m = {s.mass};
-    q = {s.charge};
-    pmin = {s.pmin_si()};
-    quadratureDepth = s.quadrature_depth;
+    preLength = d.preLength;
+    postLength = d.postLength;
+    cfl = d.cfl;
+    sizeWeight = d.sizeWeight;
+    m.assign(d.mass, d.mass + d.n_species);
+    q.assign(d.charge, d.charge + d.n_species);
+    pmin.assign(d.pmin, d.pmin + d.n_species);
+    quadratureDepth = d.quadratureDepth;
     // Settings.cpp:35-50 hard failures
     if (refinementRatio % 2) throw std::runtime_error("refinementRatio must be even");
     if (x_size % refinementRatio != 0) throw std::runtime_error("x_size not divisible by ratio");
@@ -979,7 +984,7 @@ void Mesh::PushBoundaryC() {
 
 // ---------------------------------------------------------------- Solver
 // SolverManager.cpp:8-26
-Solver::Solver(const vx::Scenario &s) : settings(s) {
+Solver::Solver(const vx_settings_desc &d, const vx_hooks &h) : settings(d, h) {
     int nsp = (int)settings.p_size.size();
     for (int i = 0; i < nsp; i++) meshes.push_back(std::unique_ptr<Mesh>(new Mesh(i, settings)));
     std::vector<Mesh *> mp;

@@ -49,24 +49,31 @@ struct Field;
 struct Mesh;
 
 // Settings.hpp:4-65 / Settings.cpp:5-195
+// The user hooks (Settings.hpp:45-64) come as the C function pointers of
+// vx_hooks; settingsOverride (Settings.cpp:31) edits the C description.
 struct Settings {
     double dx, time, minEfficiency, preLength, postLength, refinementCriteria, cfl, sizeWeight;
     std::vector<double> m, q, dp, fMax, pmin;
     unsigned x_size_finest, x_size, refinementRatio;
     int maxDepth, quadratureDepth;
     std::vector<unsigned> p_size, p_size_finest;
-    const vx::Scenario *scn;
+    vx_hooks hooks;
 
-    explicit Settings(const vx::Scenario &s);
+    Settings(const vx_settings_desc &d, const vx_hooks &h);
     double GetDp(int level, int sp) const;
     double GetDx(int level) const;
     int GetXSize(int level) const;
     int GetPSize(int level, int sp) const;
     void UpdateTime(int step, double dt);
     void DetermineMaximum();
-    double InitialDistribution(double x, double p, int sp) const { return scn->initial(x, p, sp); }
-    double GetBY(double x, double t) const { return scn->by(x, t); }
-    double GetBZ(double x, double t) const { return scn->bz(x, t); }
+    double InitialDistribution(double x, double p, int sp) const {
+        return hooks.initial_distribution(hooks.user, x, p, sp);
+    }
+    double GetBY(double x, double t) const { return hooks.get_by ? hooks.get_by(hooks.user, x, t) : 0.0; }
+    double GetBZ(double x, double t) const { return hooks.get_bz ? hooks.get_bz(hooks.user, x, t) : 0.0; }
+    bool RefinementOverride(double x, double p, int depth, int sp) const {
+        return hooks.refinement_override && hooks.refinement_override(hooks.user, x, p, depth, sp) != 0;
+    }
 };
 
 // Rectangle.hpp:6-80
@@ -216,14 +223,14 @@ struct Mesh {  // Mesh.hpp:4-41
     static int countCells(const rect &span);
     static void isFlaggedInside(const std::vector<coords> &flagged, std::vector<bool> &inside,
                                 const rect &r);
-    std::tuple<std::vector<int>, std::vector<int>, std::vector<int>, std::vector<int>>
+    static std::tuple<std::vector<int>, std::vector<int>, std::vector<int>, std::vector<int>>
     computeSignatures(const rect &r, const std::vector<coords> &flagged);
-    coords identifyInflection(const rect &r,
-                              const std::tuple<std::vector<int>, std::vector<int>,
-                                               std::vector<int>, std::vector<int>> &sig);
+    static coords identifyInflection(const rect &r,
+                                     const std::tuple<std::vector<int>, std::vector<int>,
+                                                      std::vector<int>, std::vector<int>> &sig);
     static std::tuple<bool, int, int> hasHole(const std::vector<int> &sig);
     static void getExtrema(rect &extrema, const std::vector<coords> &flagged);
-    level splitRectangle(const rect &r, const std::vector<coords> &flagged, double minEfficiency);
+    static level splitRectangle(const rect &r, const std::vector<coords> &flagged, double minEfficiency);
     void promoteHierarchyToMesh(bool init);
     void InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &levels_n);
 };
@@ -232,7 +239,7 @@ struct Solver {  // SolverManager.hpp:6-19
     Settings settings;
     std::vector<std::unique_ptr<Mesh>> meshes;
     std::unique_ptr<Field> em;
-    explicit Solver(const vx::Scenario &s);
+    Solver(const vx_settings_desc &d, const vx_hooks &h);
     void Advance(double dt);
     void AdvanceFields(double dt);
     void reGrid();

@@ -401,7 +401,7 @@ void Patch::getError(std::vector<coords> &flagged) const {
     for (int i = lower_x; i < upper_x; i++)
         for (int j = lower_p; j < upper_p; j++)
             if ((ErrorEstimate(i - x_pos, j - p_pos) > settings->refinementCriteria) ||
-                settings->scn->refinement_override((i + 0.5) * dx, Momentum(j - p_pos), depth, sp))
+                settings->RefinementOverride((i + 0.5) * dx, Momentum(j - p_pos), depth, sp))
                 flagged.push_back(std::make_pair(i, j));
 }
 

@@ -39,7 +39,20 @@ extern "C" {
 void *vxo_create(const vx_scenario_params *prm) {
     Run *r = new Run;
     r->scn = vx::Scenario::from_params(*prm);
-    r->s.reset(new orc::Solver(r->scn));
+    r->s.reset(new orc::Solver(r->scn.desc(), r->scn.hooks()));
+    return r;
+}
+
+// A run of the reference's inputs and user hooks (`hooks->user` must stay
+// valid while the run lives); nullptr when the inputs are rejected.
+void *vxo_create_desc(const vx_settings_desc *desc, const vx_hooks *hooks) {
+    Run *r = new Run;
+    try {
+        r->s.reset(new orc::Solver(*desc, *hooks));
+    } catch (std::exception &) {
+        delete r;
+        return nullptr;
+    }
     return r;
 }
 
@@ -112,6 +125,11 @@ int vxo_get(void *h, const char *what, int ghosts, double *out, int n) {
     if (m >= 0) return m;
     if (w == "phi") return copy_out(em.PHI, out, n);
     if (w == "charge") return copy_out(em.charge, out, n);
+    // charge of species i from the last assembly (EMSolver.cpp:104-122)
+    if (w.rfind("charge_sp", 0) == 0) {
+        unsigned sp = (unsigned)std::stoi(w.substr(9));
+        return sp < em.charges.size() ? copy_out(em.charges[sp], out, n) : -1;
+    }
     if (w == "J") return copy_out(em.J, out, n);
     if (w == "neutral") return copy_out(em.neutralizationCharge, out, n);
     if (w == "a2") return copy_out(em.a_squared, out, n);
@@ -211,6 +229,23 @@ void vxo_vlasov_imposed(void *h, double dt, int nsteps, double E0, double omega)
             em.Ex0 = E0 * std::cos(omega * r->s->settings.time);
             for (auto &m : r->s->meshes) m->Advance(dt, i);
         }
+}
+
+// Mesh::splitRectangle (Mesh.cpp:644-792) on box (x0, p0)-(x1, p1) with n
+// flagged cells (x, p pairs); writes 4 ints per resulting box, returns 4 * count.
+int vxo_split(int x0, int p0, int x1, int p1, const int *flags, int n, double minEfficiency, int *out,
+              int nout) {
+    std::vector<orc::coords> fl;
+    for (int k = 0; k < n; k++) fl.push_back({flags[2 * k], flags[2 * k + 1]});
+    orc::level boxes = orc::Mesh::splitRectangle({{x0, p0}, {x1, p1}}, fl, minEfficiency);
+    if (out && nout >= 4 * (int)boxes.size())
+        for (size_t b = 0; b < boxes.size(); b++) {
+            out[4 * b] = boxes[b].first.first;
+            out[4 * b + 1] = boxes[b].first.second;
+            out[4 * b + 2] = boxes[b].second.first;
+            out[4 * b + 3] = boxes[b].second.second;
+        }
+    return 4 * (int)boxes.size();
 }
 
 // Standalone pieces for unit tests.

@@ -20,6 +20,8 @@ def bind(L):
     i = ctypes.c_int
     L.vxo_create.restype = vp
     L.vxo_create.argtypes = [vp]
+    L.vxo_create_desc.restype = vp
+    L.vxo_create_desc.argtypes = [vp, vp]
     L.vxo_destroy.argtypes = [vp]
     L.vxo_calc_dt.restype = d
     L.vxo_calc_dt.argtypes = [vp, d]
@@ -78,13 +80,28 @@ def library(path):
         _lib = saved
 
 
-class OracleRun:
-    """One oracle SolverManager for a scenario (species 0, base patch)."""
+class _Shape:
+    def __init__(self, desc):
+        self.nx, self.np, self.cfl = int(desc.nx), int(desc.np[0]), float(desc.cfl)
 
-    def __init__(self, params):
+
+class OracleRun:
+    """One oracle SolverManager for a built-in scenario, or (from_desc) for the
+    reference's inputs and user hooks; get() reads species 0's base patch."""
+
+    def __init__(self, params, _handle=None):
         self.params = params
         self.L = lib()  # the library this run lives in
-        self.h = self.L.vxo_create(ctypes.byref(params))
+        self.h = _handle if _handle is not None else self.L.vxo_create(ctypes.byref(params))
+
+    @classmethod
+    def from_desc(cls, desc, hooks):
+        h = lib().vxo_create_desc(ctypes.byref(desc), ctypes.byref(hooks.c))
+        if not h:
+            raise ValueError("the oracle rejected the inputs")
+        run = cls(_Shape(desc), _handle=h)
+        run._hooks = hooks
+        return run
 
     def close(self):
         if self.h:

@@ -13,6 +13,7 @@ from concurrent.futures import ThreadPoolExecutor
 import pytest
 
 import _oracle
+import test_clustering as K
 import test_oracle_vlasov as V
 
 ORACLE = os.path.join(_oracle.ROOT, "oracle")
@@ -51,6 +52,17 @@ PLANTS = {
                   [("0.6805652953093346, -0.27524053099500667",
                     "0.6805652953093346 * 1.001, -0.27524053099500667")],
                   [V.test_time_dependent_force_converges_at_the_order_of_the_tableau]),
+    # Mesh.cpp:480-490, 541-551: ties between equal inflections resolved toward
+    # the later one (both axes)
+    "inflection_tie": ("oracle_amr.cpp",
+                       [("if (range > std::abs(inflectIdx.at(i) - center)) {",
+                         "if (range >= std::abs(inflectIdx.at(i) - center)) {")],
+                       [K.check_oracle_cases]),
+    # Mesh.cpp:722-741: the bisection rounded up instead of down
+    "bisect_round": ("oracle_amr.cpp",
+                     [("cutPosition = std::floor((lengthx / 2) + rectangle.first.first);",
+                       "cutPosition = std::ceil((lengthx / 2) + rectangle.first.first);")],
+                     [K.check_oracle_cases]),
     # Settings.cpp:166-179: the third stage time 0.333 instead of 0.332
     "stage_clock": ("oracle.cpp",
                     [("time += (0.332 - 0.5) * dt;", "time += (0.333 - 0.5) * dt;"),
@@ -68,7 +80,7 @@ def _build(name, workdir):
             text = fh.read()
         if src == fname:
             for old, new in subs:
-                assert text.count(old) == 1, (name, old)
+                assert text.count(old) >= 1, (name, old)
                 text = text.replace(old, new)
         with open(os.path.join(d, src), "w") as fh:
             fh.write(text)

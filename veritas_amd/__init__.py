@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 from . import serve  # noqa: F401
-from .params import ScenarioParams, landau, laser, two_stream  # noqa: F401
+from .params import (Hooks, ScenarioParams, SettingsDesc, landau, laser,  # noqa: F401
+                     two_stream)
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(HERE, "lib")
@@ -53,6 +54,12 @@ def host_lib():
         vp, d, i = ctypes.c_void_p, ctypes.c_double, ctypes.c_int
         L.vxh_create.restype = vp
         L.vxh_create.argtypes = [vp, i]
+        L.vxh_create_desc.restype = vp
+        L.vxh_create_desc.argtypes = [vp, vp, i]
+        L.vxh_settings_banner.restype = i
+        L.vxh_settings_banner.argtypes = [vp, vp, ctypes.c_char_p, i]
+        L.vxh_screen.restype = i
+        L.vxh_screen.argtypes = [vp, i, ctypes.c_char_p, i]
         L.vxh_create_error.restype = ctypes.c_char_p
         L.vxh_last_error.restype = ctypes.c_char_p
         L.vxh_last_error.argtypes = [vp]
@@ -99,7 +106,7 @@ def host_lib():
         L.vxh_patch_get.argtypes = [vp, i, i, i, ctypes.c_char_p, i, vp, i]
         L.vxh_flags.restype = i
         L.vxh_flags.argtypes = [vp, i, i, vp, i]
-        L.vxh_regrid.argtypes = [vp]
+        L.vxh_regrid.argtypes = [vp, d]
         L.vxh_regrid_stats.argtypes = [vp, i, vp]
         L.vxh_file_output.argtypes = [vp, d]
         L.vxh_output_rectangles.argtypes = [vp, d]
@@ -110,19 +117,69 @@ def host_lib():
     return _host
 
 
-class Run:
-    """One SolverManager (SolverManager.hpp) for a scenario on one GPU."""
+def settings_banner(desc, hooks):
+    """Settings(desc, hooks) on the host alone (no GPU): its banner
+    (Settings.cpp:60-125), or VeritasError with the reason the inputs are
+    rejected (Settings.cpp:35-50)."""
+    L = host_lib()
+    n = L.vxh_settings_banner(ctypes.byref(desc), ctypes.byref(hooks.c), None, 0)
+    if n < 0:
+        raise VeritasError(L.vxh_create_error().decode())
+    buf = ctypes.create_string_buffer(n + 1)
+    L.vxh_settings_banner(ctypes.byref(desc), ctypes.byref(hooks.c), buf, n + 1)
+    return buf.value.decode()
 
-    def __init__(self, params, device=0, fused=True, graphs=True, mixed=False):
+
+class _Shape:
+    """nx, np (species 0) and cfl of a run made from a SettingsDesc."""
+
+    def __init__(self, desc):
+        self.nx, self.np, self.cfl = int(desc.nx), int(desc.np[0]), float(desc.cfl)
+
+
+class Run:
+    """One SolverManager (SolverManager.hpp) on one GPU, for a built-in
+    scenario (`Run(params)`) or for the reference's inputs and user hooks
+    (`Run.from_desc(desc, hooks)`)."""
+
+    def __init__(self, params, device=0, fused=True, graphs=True, mixed=False, _handle=None):
         L = host_lib()
         self.params = params
-        self.h = L.vxh_create(ctypes.byref(params), device)
+        self.h = _handle if _handle is not None else L.vxh_create(ctypes.byref(params), device)
         if not self.h:
             raise VeritasError(L.vxh_create_error().decode())
         self.set_fused(fused)
         self.set_graphs(graphs)
         if mixed:
             self.set_mixed(True)
+
+    @classmethod
+    def from_desc(cls, desc, hooks, device=0, **kw):
+        """A run of the reference's Input/Particles (`SettingsDesc`, SI) and
+        user hooks (`Hooks`): Settings(grid, particles, output) followed by
+        SolverManager(settings) (veritas.cpp:118-128)."""
+        L = host_lib()
+        h = L.vxh_create_desc(ctypes.byref(desc), ctypes.byref(hooks.c), device)
+        if not h:
+            raise VeritasError(L.vxh_create_error().decode())
+        run = cls(_Shape(desc), device=device, _handle=h, **kw)
+        run._hooks = hooks  # the callbacks must outlive the run
+        return run
+
+    def banner(self):
+        """The run banner of Settings.cpp:60-125."""
+        return self._screen(0)
+
+    def screen(self):
+        """The progress table of SolverManager.cpp:9-100 (set-up and regrids)."""
+        return self._screen(1)
+
+    def _screen(self, which):
+        L = host_lib()
+        n = L.vxh_screen(self.h, which, None, 0)
+        buf = ctypes.create_string_buffer(n + 1)
+        L.vxh_screen(self.h, which, buf, n + 1)
+        return buf.value.decode()
 
     def _ck(self, rc):
         if rc != 0:
@@ -291,9 +348,9 @@ class Run:
         self._ck(host_lib().vxh_regrid_stats(self.h, sp, out))
         return int(out[0]), int(out[1])
 
-    def regrid(self):
-        """SolverManager::reGrid (SolverManager.cpp:48-58)."""
-        self._ck(host_lib().vxh_regrid(self.h))
+    def regrid(self, t=0.0):
+        """SolverManager::reGrid(t) (SolverManager.cpp:48-58)."""
+        self._ck(host_lib().vxh_regrid(self.h, float(t)))
 
     # ---- output files (SolverManager::fileOutput / OutputRectangles)
     def file_output(self, t):

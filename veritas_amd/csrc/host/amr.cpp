@@ -22,6 +22,9 @@
 
 #include "veritas.hpp"
 
+#include <iomanip>
+#include <sstream>
+
 namespace vxh {
 
 namespace {
@@ -525,6 +528,10 @@ level split_box(const rect &b, const std::vector<coords> &fl, double minEfficien
 
 }  // namespace
 
+level SplitBox(const rect &b, const std::vector<coords> &flagged, double minEfficiency) {
+    return split_box(b, flagged, minEfficiency);
+}
+
 // ---------------------------------------------------------------- Rectangle links
 
 // Rectangle.cpp:667-669
@@ -622,7 +629,7 @@ void Rectangle::getError(std::vector<coords> &flagged) {
     const int lp = std::max(p_pos, offset), up_ = std::min(p_pos + n_p, settings.GetPSize(depth, particleType) - offset);
     if (ux <= lx || up_ <= lp) return;
     const size_t n = (size_t)(ux - lx) * (up_ - lp);
-    const bool hook = settings.scn.has_refinement_override();
+    const bool hook = settings.HasRefinementOverride();
     if (!hook) {
         // device compaction: only the flagged cells (and the range's signatures)
         // come back, in the row-major order of the scan below
@@ -651,7 +658,7 @@ void Rectangle::getError(std::vector<coords> &flagged) {
     size_t e = 0;
     for (int i = lx; i < ux; i++)
         for (int j = lp; j < up_; j++, e++)
-            if (h[e] || (hook && settings.scn.refinement_override((i + 0.5) * P.dx, P.pmin + P.dp * (j - p_pos + p_pos),
+            if (h[e] || (hook && settings.RefinementOverride((i + 0.5) * P.dx, P.pmin + P.dp * (j - p_pos + p_pos),
                                                                   depth, particleType)))
                 flagged.push_back(std::make_pair(i, j));
 }
@@ -1095,11 +1102,20 @@ void Mesh::InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &old) {
 }
 
 // SolverManager.cpp:48-58 (without the screen output)
-void SolverManager::reGrid() {
+void SolverManager::reGrid(double t) {
     DropGraphs();  // recorded steps hold the old patch tables and programs
+    // SolverManager.cpp:49-51: the time in periods of the laser (tempEM[0] / c)
+    double T = (!settings.tempEM.empty() && settings.tempEM[0] > 0) ? settings.tempEM[0] / vx_cs : 1.0;
+    {
+        std::ostringstream o;
+        o << std::setfill(' ') << std::setw(4) << ' ' << std::fixed << std::setprecision(5) << t / T
+          << std::setw(5) << ' ';
+        ScreenWrite(o.str());
+    }
     for (auto &m : meshes) {
         m->updateHierarchy();
         m->SetFieldSolver(EMSolver.get());
+        ScreenRow(*m);
     }
     EMSolver->RebuildMomentPlan();
 }

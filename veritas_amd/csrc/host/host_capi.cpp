@@ -60,6 +60,62 @@ void *vxh_create(const vx_scenario_params *prm, int device) {
     }
 }
 
+// A run of the reference's inputs (Input, Particles as vx_settings_desc) and
+// user hooks (vx_hooks); `hooks->user` must stay valid while the run lives.
+void *vxh_create_desc(const vx_settings_desc *desc, const vx_hooks *hooks, int device) {
+    try {
+        Run *r = new Run;
+        r->settings.reset(new vxh::Settings(*desc, *hooks));
+        r->sm.reset(new vxh::SolverManager(*r->settings, device));
+        return r;
+    } catch (std::exception &e) {
+        g_create_error = e.what();
+        return nullptr;
+    }
+}
+
+// Text of the run banner (Settings.cpp:60-125) or of the progress table
+// (SolverManager.cpp:9-100): `which` 0 or 1.  Returns the length; copies when
+// n is large enough (with a terminating 0).
+int vxh_screen(void *h, int which, char *out, int n) {
+    Run *r = (Run *)h;
+    std::string t = which == 0 ? r->settings->Banner() : r->sm->Screen();
+    if (out && n > (int)t.size()) std::memcpy(out, t.c_str(), t.size() + 1);
+    return (int)t.size();
+}
+
+// Settings alone (no device): the banner of Settings(desc, hooks), or -1 with
+// vxh_create_error() when the inputs are rejected (Settings.cpp:35-50).
+int vxh_settings_banner(const vx_settings_desc *desc, const vx_hooks *hooks, char *out, int n) {
+    try {
+        vxh::Settings s(*desc, *hooks);
+        std::string t = s.Banner();
+        if (out && n > (int)t.size()) std::memcpy(out, t.c_str(), t.size() + 1);
+        return (int)t.size();
+    } catch (std::exception &e) {
+        g_create_error = e.what();
+        return -1;
+    }
+}
+
+// The host's clustering (Mesh::splitRectangle, Mesh.cpp:644-792) on box
+// (x0, p0)-(x1, p1) with n flagged cells (x, p pairs), no device: 4 ints per
+// resulting box, returns 4 * count.
+int vxh_split_box(int x0, int p0, int x1, int p1, const int *flags, int n, double minEfficiency, int *out,
+                  int nout) {
+    std::vector<vxh::coords> fl;
+    for (int k = 0; k < n; k++) fl.push_back({flags[2 * k], flags[2 * k + 1]});
+    vxh::level boxes = vxh::SplitBox({{x0, p0}, {x1, p1}}, fl, minEfficiency);
+    if (out && nout >= 4 * (int)boxes.size())
+        for (size_t b = 0; b < boxes.size(); b++) {
+            out[4 * b] = boxes[b].first.first;
+            out[4 * b + 1] = boxes[b].first.second;
+            out[4 * b + 2] = boxes[b].second.first;
+            out[4 * b + 3] = boxes[b].second.second;
+        }
+    return 4 * (int)boxes.size();
+}
+
 const char *vxh_create_error() { return g_create_error.c_str(); }
 const char *vxh_last_error(void *h) { return ((Run *)h)->err.c_str(); }
 void vxh_destroy(void *h) { delete (Run *)h; }
@@ -223,7 +279,8 @@ int vxh_signatures(void *h, int sp, int lvl, int idx, int *out, int n) {
     }
 }
 
-int vxh_regrid(void *h) { return guarded(h, [&](Run *r) { r->sm->reGrid(); }); }
+// SolverManager::reGrid(t) (SolverManager.cpp:48-58)
+int vxh_regrid(void *h, double t) { return guarded(h, [&](Run *r) { r->sm->reGrid(t); }); }
 
 // The nesting list of Mesh::updateHierarchy (host logic, for the CPU tests):
 // level cells under the finer-level box [i0, i1] x [j0, j1]; writes up to n
@@ -282,6 +339,12 @@ int vxh_get(void *h, const char *what, int ghosts, double *out, int n) {
         size_t ne8 = (size_t)8 * fs.n_e;
         if (w == "phi") return copy_out(dl(r, fs.phi, N), out, n);
         if (w == "charge") return copy_out(dl(r, fs.charge, N), out, n);
+        if (w.rfind("charge_sp", 0) == 0) {  // species i of the last assembly
+            const int sp = std::stoi(w.substr(9));
+            const double *cs = r->sm->EMSolver->charges_species;
+            if (!cs || sp < 0 || sp >= (int)r->sm->meshes.size()) return -2;
+            return copy_out(dl(r, cs + (size_t)sp * N, N), out, n);
+        }
         if (w == "J") return copy_out(dl(r, fs.J, N), out, n);
         if (w == "neutral") return copy_out(dl(r, fs.neutral, N), out, n);
         if (w == "a2") return copy_out(dl(r, fs.a2, N + 1), out, n);

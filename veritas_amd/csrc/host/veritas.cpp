@@ -6,39 +6,97 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
+#include <iomanip>
+#include <sstream>
 
 namespace vxh {
 
 // ---------------------------------------------------------------- Settings
-Settings::Settings(const vx::Scenario &s) : scn(s) {
-    // initialConditions + settingsOverride of the scenario (veritas.cpp:7-73)
-    dx = s.dx();
-    dp = {s.dp()};
-    maxDepth = s.levels - 1;
-    x_size = s.nx;
-    p_size = {(unsigned)s.np};
-    refinementRatio = s.r;
-    refinementCriteria = s.refinement_criteria;
-    minEfficiency = s.min_efficiency;
-    preLength = s.pre_length;
-    postLength = s.post_length;
-    cfl = s.cfl;
-    sizeWeight = s.size_weight;
-    m = {s.mass};
-    q = {s.charge};
-    pmin = {s.pmin_si()};
-    quadratureDepth = s.quadrature_depth;
-    finish();
+Hooks Hooks::FromC(const vx_hooks &h) {
+    Hooks k;
+    void *u = h.user;
+    if (h.initial_distribution) {
+        auto f = h.initial_distribution;
+        k.InitialDistribution = [f, u](double x, double p, int sp) { return f(u, x, p, sp); };
+    }
+    if (h.get_by) {
+        auto f = h.get_by;
+        k.GetBY = [f, u](double x, double t) { return f(u, x, t); };
+    }
+    if (h.get_bz) {
+        auto f = h.get_bz;
+        k.GetBZ = [f, u](double x, double t) { return f(u, x, t); };
+    }
+    if (h.refinement_override) {
+        auto f = h.refinement_override;
+        k.RefinementOverride = [f, u](double x, double p, int d, int sp) { return f(u, x, p, d, sp) != 0; };
+    }
+    if (h.settings_override) {
+        auto f = h.settings_override;
+        k.settingsOverride = [f, u](Settings &s) {
+            vx_settings_desc d = s.Describe();
+            f(u, &d);
+            // the values the C description carries go back into the settings
+            s.minEfficiency = d.minEfficiency;
+            s.dx = d.dx;
+            s.refinementCriteria = d.refinementCriteria;
+            s.cfl = d.cfl;
+            s.sizeWeight = d.sizeWeight;
+            s.preLength = d.preLength;
+            s.postLength = d.postLength;
+            s.x_size = d.nx;
+            s.refinementRatio = d.r;
+            s.maxDepth = (int)d.Lfinest - 1;
+            s.quadratureDepth = d.quadratureDepth;
+            if (d.n_species < 1 || d.n_species > VX_MAX_SPECIES)
+                throw Error("settingsOverride: n_species out of range");
+            s.m.assign(d.mass, d.mass + d.n_species);
+            s.q.assign(d.charge, d.charge + d.n_species);
+            s.dp.assign(d.dp, d.dp + d.n_species);
+            s.pmin.assign(d.pmin, d.pmin + d.n_species);
+            s.p_size.assign(d.np, d.np + d.n_species);
+        };
+    }
+    return k;
 }
 
-Settings::Settings(const Input &grid, const Particles &particles, const Output &out,
-                   const vx::Scenario &s)
-    : output(out), scn(s) {
-    // Settings.cpp:5-31
+static Input input_of(const vx_settings_desc &d) {
+    Input g;
+    g.minEfficiency = d.minEfficiency;
+    g.dx = d.dx;
+    g.refinementCriteria = d.refinementCriteria;
+    g.cfl = d.cfl;
+    g.sizeWeight = d.sizeWeight;
+    g.preLength = d.preLength;
+    g.postLength = d.postLength;
+    g.nx = d.nx;
+    g.r = d.r;
+    g.Lfinest = d.Lfinest;
+    g.quadratureDepth = d.quadratureDepth;
+    return g;
+}
+
+static Particles particles_of(const vx_settings_desc &d) {
+    if (d.n_species < 1 || d.n_species > VX_MAX_SPECIES) throw Error("n_species must be 1..8");
+    Particles pt;
+    pt.mass.assign(d.mass, d.mass + d.n_species);
+    pt.charge.assign(d.charge, d.charge + d.n_species);
+    pt.dp.assign(d.dp, d.dp + d.n_species);
+    pt.pmin.assign(d.pmin, d.pmin + d.n_species);
+    pt.np.assign(d.np, d.np + d.n_species);
+    pt.misc.assign(d.n_species, {});
+    return pt;
+}
+
+// Settings.cpp:5-31: copy the inputs, then let the user override them
+Settings::Settings(const Input &grid, const Particles &particles, const Output &out, Hooks h)
+    : output(out), hooks(std::move(h)) {
+    if (!hooks.InitialDistribution) throw Error("Settings: the InitialDistribution hook is required");
     dx = grid.dx;
     dp = particles.dp;
-    maxDepth = grid.Lfinest - 1;
+    maxDepth = (int)grid.Lfinest - 1;
     x_size = grid.nx;
     p_size = particles.np;
     refinementRatio = grid.r;
@@ -51,8 +109,46 @@ Settings::Settings(const Input &grid, const Particles &particles, const Output &
     m = particles.mass;
     q = particles.charge;
     pmin = particles.pmin;
-    quadratureDepth = s.quadrature_depth;
+    temp = particles.misc;
+    tempEM = grid.tempEM;
+    time = 0.0;
+    quadratureDepth = grid.quadratureDepth;
+    if (hooks.settingsOverride) hooks.settingsOverride(*this);
     finish();
+}
+
+Settings::Settings(const vx_settings_desc &d, const vx_hooks &h)
+    : Settings(input_of(d), particles_of(d), Output(), Hooks::FromC(h)) {}
+
+Settings::Settings(const vx::Scenario &s) : Settings(s.desc(), s.hooks()) {
+    // the hooks point at the scenario: keep a copy alive and re-point them
+    auto own = std::make_shared<const vx::Scenario>(s);
+    hooks = Hooks::FromC(own->hooks());
+    scenario_ = own;
+}
+
+vx_settings_desc Settings::Describe() const {
+    vx_settings_desc d{};
+    d.minEfficiency = minEfficiency;
+    d.dx = dx;
+    d.refinementCriteria = refinementCriteria;
+    d.cfl = cfl;
+    d.sizeWeight = sizeWeight;
+    d.preLength = preLength;
+    d.postLength = postLength;
+    d.nx = x_size;
+    d.r = refinementRatio;
+    d.Lfinest = (uint32_t)(maxDepth + 1);
+    d.quadratureDepth = quadratureDepth;
+    d.n_species = (int32_t)std::min<size_t>(q.size(), VX_MAX_SPECIES);
+    for (int i = 0; i < d.n_species; i++) {
+        d.mass[i] = m[i];
+        d.charge[i] = q[i];
+        d.dp[i] = dp[i];
+        d.pmin[i] = pmin[i];
+        d.np[i] = p_size[i];
+    }
+    return d;
 }
 
 // Settings.cpp:35-58
@@ -63,13 +159,70 @@ void Settings::finish() {
         throw Error("'x_size' must be completely divisible by the Mesh refinement ratio 'refinementRatio'.");
     for (unsigned p = 0; p < p_size.size(); p++)
         if (p_size[p] % refinementRatio != 0)
-            throw Error("'p_size' must be completely divisible by the Mesh refinement ratio 'refinementRatio'.");
+            throw Error("'p_size' at index (" + std::to_string(p) +
+                        ") must be completely divisible by the Mesh refinement ratio 'refinementRatio'.");
+    if (m.size() != q.size() || dp.size() != q.size() || pmin.size() != q.size() || p_size.size() != q.size())
+        throw Error("Particles: mass, charge, np, dp and pmin must have one entry per species");
+    if (q.empty() || q.size() > VX_MAX_SPECIES) throw Error("Particles: 1 to 8 species");
+    if (temp.size() < q.size()) temp.resize(q.size());
     x_size_finest = x_size * std::pow(refinementRatio, maxDepth);
     p_size_finest.clear();
     for (unsigned i = 0; i < p_size.size(); i++)
         p_size_finest.push_back(p_size[i] * std::pow(refinementRatio, maxDepth));
     fMax.assign(q.size(), 0.0);
     DetermineMaximum();
+}
+
+// Settings.cpp:60-125 (OUTW = 80, veritas.hpp:30)
+std::string Settings::Banner() const {
+    const int W = 80;
+    std::ostringstream o;
+    auto title = [&](const std::string &t, char sp) {
+        int space = W - (int)t.size();
+        int l = space / 2, r = space - l;
+        o << std::string(std::max(l, 0), sp) << t << std::string(std::max(r, 0), sp) << "\n";
+    };
+    int maxleft = 29, maxright = 25;
+    bool nonneg = false;
+    for (double v : pmin) nonneg = nonneg || (int)v >= 0;
+    if (nonneg) maxleft--;
+    title("VERITAS. Current build SHA1: veritas_amd (MI355X)", ' ');
+    o << "\n";
+    title(" Configuration Parameters ", '=');
+    auto row = [&](const std::string &a, const std::string &av, const std::string &b, const std::string &bv) {
+        o << std::left << std::setw(maxleft) << a << std::setw(W / 2 - maxleft) << av;
+        if (!b.empty()) o << std::setw(maxright) << b << std::setw(W / 2 - maxright) << bv;
+        o << "\n";
+    };
+    auto num = [](double v) {
+        std::ostringstream t;
+        t << std::setprecision(5) << v;
+        return t.str();
+    };
+    row("Minimum Efficiency: ", num(minEfficiency), "Mesh refinement ratio: ", num(refinementRatio));
+    row("Maximum number of levels: ", num(maxDepth + 1), "Error tolerance: ", num(refinementCriteria));
+    row("nx: ", num(x_size), "PreLength: ", num(preLength));
+    row("dx: ", num(dx), "PostLength: ", num(postLength));
+    title(" Particle Properties ", '=');
+    for (size_t i = 0; i < q.size(); i++) {
+        row("np: ", num(p_size[i]), "Mass: ", num(m[i]));
+        row("dp: ", num(dp[i]), "Charge: ", num(q[i]));
+        o << std::left << std::setw(pmin[i] >= 0 ? maxleft : maxleft - 1) << "Momentum Cutoff: "
+          << std::setw(W / 2 - maxleft) << num(pmin[i]) << "\n";
+        o << std::setw(maxleft) << "Misc values: ";
+        for (size_t j = 0; j < temp[i].size(); j++) o << num(temp[i][j]) << (j + 1 < temp[i].size() ? "; " : "");
+        o << "\n";
+        if (q.size() > 1 && i + 1 < q.size()) o << std::string(80, '-') << "\n";
+    }
+    title(" Output Settings ", '=');
+    auto b = [](bool v) { return std::string(v ? "true" : "false"); };
+    row("Rectangle Data: ", b(output.rectangleData), "Charge: ", b(output.charge));
+    row("Longitudinal ElectricField: ", b(output.EFieldLongitudinal), "Energy: ", b(output.energy));
+    row("Transverse ElectricField: ", b(output.EFieldTransverse), "Electrostatic Potential: ", b(output.potential));
+    row("Transverse MagniticField: ", b(output.BFieldTransverse), "Squared Vector Potental: ", b(output.AFieldSquared));
+    row("Output Precision: ", num(output.precision), "", "");
+    o << std::string(80, '=') << "\n\n";
+    return o.str();
 }
 
 double Settings::GetDp(int level, int sp) const {
@@ -553,15 +706,60 @@ double EMFieldSolver::EstimateCFLBound() {
 // SolverManager.cpp:8-26
 SolverManager::SolverManager(Settings &s, int device)
     : dev(std::make_shared<Device>(device)), settings(s) {
+    {
+        std::ostringstream o;  // SolverManager.cpp:9
+        o << std::setfill('-') << std::setw(3) << '-' << " Time (T) " << std::setw(4) << '-' << '|'
+          << std::setw(3) << '-' << " Particle " << std::setw(3) << '-' << '|' << std::setw(4) << '-'
+          << " Levels " << std::setw(4) << '-' << '|' << std::setw(8) << '-' << " Rectangles "
+          << std::setw(9) << '-' << "\n";
+        ScreenWrite(o.str());
+    }
     int numSpecies = (int)s.p_size.size();
     for (int i = 0; i < numSpecies; i++) meshes.push_back(std::make_shared<Mesh>(i, s, dev));
     EMSolver = std::make_shared<EMFieldSolver>(s, meshes, dev);
+    {
+        std::ostringstream o;  // SolverManager.cpp:17 (the time is written as 0)
+        o << std::setfill(' ') << std::setw(4) << ' ' << std::fixed << 0.0 << std::setw(5) << ' ';
+        ScreenWrite(o.str());
+    }
     for (auto &m : meshes) {
         m->SetFieldSolver(EMSolver.get());
         m->PushData();
+        ScreenRow(*m);
     }
     EMSolver->RebuildMomentPlan();
     EMSolver->EnforceChargeNeutralization();
+}
+
+void SolverManager::ScreenWrite(const std::string &s) {
+    screen_ += s;
+    if (echo_screen) {
+        fputs(s.c_str(), stdout);
+        fflush(stdout);
+    }
+}
+
+// SolverManager.cpp:59-100: species, "used (allocated)" levels, patches per used level
+void SolverManager::ScreenRow(const Mesh &mesh) {
+    auto centred = [](const std::string &t, int target) {
+        int pad = target - (int)t.size();
+        return pad > 0 ? std::string(pad / 2, ' ') + t + std::string(pad / 2, ' ') : t;
+    };
+    std::ostringstream o, nums;
+    if (mesh.particleType > 0) o << std::setw(17) << ' ';
+    o << std::setw(16) << centred(std::to_string(mesh.particleType), 15) << std::setw(1) << ' ';
+    int full = 0;
+    for (auto &lvl : mesh.levels)
+        if (!lvl->rectangles.empty()) {
+            full++;
+            nums << lvl->rectangles.size() << ", ";
+        }
+    std::string n = nums.str();
+    if (n.size() >= 2) n.erase(n.size() - 2, 2);
+    std::string lv = std::to_string(full) + " (" + std::to_string(mesh.levels.size()) + ")";
+    o << std::setw(16) << centred(lv, 15) << std::setw(1) << ' ';
+    o << std::setw(30) << centred(n, 29) << "\n";
+    ScreenWrite(o.str());
 }
 
 // SolverManager.cpp:28-39 (SolverManager::Advance, host/graph.cpp, replays

@@ -6,6 +6,7 @@
 #pragma once
 
 #include <cstdint>
+#include <functional>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -35,8 +36,12 @@ struct Input {
     double preLength = 2, postLength = 2;
     unsigned int nx = 76 * 4, r = 2, Lfinest = 5;
     std::vector<double> tempEM = {0.0};
+    // Settings::quadratureDepth before settingsOverride runs (the reference
+    // leaves it to the override, veritas.cpp:51)
+    int quadratureDepth = 1;
 };
 struct Particles {
+    std::vector<std::vector<double>> misc = {{0.0, 0.01}};  // Settings::temp, for the user's hooks
     std::vector<double> mass = {9.10938291e-31};
     std::vector<double> charge = {-1.60217657e-19};
     std::vector<unsigned int> np = {50};
@@ -50,20 +55,39 @@ struct Output {
     int precision = 15;
 };
 
-// Settings.hpp:33-65.  The user hooks (InitialDistribution, GetBY, GetBZ,
-// RefinementOverride) are the scenario's.
+class Settings;
+
+// The user hooks of Settings.hpp:45-64 (written by the user in
+// veritas.cpp:36-115) as callables.  GetBY/GetBZ left empty drive nothing,
+// an empty RefinementOverride never flags, an empty settingsOverride changes
+// nothing; InitialDistribution is required.
+struct Hooks {
+    std::function<double(double x, double p, int particleType)> InitialDistribution;
+    std::function<double(double x, double t)> GetBY, GetBZ;
+    std::function<bool(double x, double p, int depth, int particleType)> RefinementOverride;
+    std::function<void(Settings &)> settingsOverride;
+    // the same hooks from C function pointers (vx_hooks)
+    static Hooks FromC(const vx_hooks &h);
+};
+
+// Settings.hpp:33-65
 class Settings {
 public:
     Output output;
     double dx, time, minEfficiency, preLength, postLength, refinementCriteria, cfl, sizeWeight;
     std::vector<double> m, q, dp, fMax, pmin;
     unsigned int x_size_finest, x_size, refinementRatio;
-    int maxDepth, quadratureDepth;
+    int maxDepth, quadratureDepth = 1;
     std::vector<unsigned int> p_size, p_size_finest;
-    vx::Scenario scn;
+    std::vector<double> tempEM;               // Input::tempEM, for the user's hooks
+    std::vector<std::vector<double>> temp;    // Particles::misc, for the user's hooks
+    Hooks hooks;
 
-    Settings(const Input &grid, const Particles &particles, const Output &out,
-             const vx::Scenario &scenario);
+    // Settings.cpp:5-58: copy the inputs, run settingsOverride, check, derive
+    Settings(const Input &grid, const Particles &particles, const Output &out, Hooks hooks);
+    // the same from the C description (vx_settings_desc) and C hooks
+    Settings(const vx_settings_desc &desc, const vx_hooks &hooks);
+    // a built-in scenario (scenario.h) as the provider of inputs and hooks
     explicit Settings(const vx::Scenario &scenario);
     double GetDp(int level, int particleType) const;
     double GetDx(int level) const;
@@ -75,12 +99,22 @@ public:
     void UpdateTime(int step, double dt);
     void DetermineMaximum();
     double InitialDistribution(double x, double p, int particleType) const {
-        return scn.initial(x, p, particleType);
+        return hooks.InitialDistribution(x, p, particleType);
     }
-    double GetBY(double x, double t) const { return scn.by(x, t); }
-    double GetBZ(double x, double t) const { return scn.bz(x, t); }
+    double GetBY(double x, double t) const { return hooks.GetBY ? hooks.GetBY(x, t) : 0.0; }
+    double GetBZ(double x, double t) const { return hooks.GetBZ ? hooks.GetBZ(x, t) : 0.0; }
+    bool HasRefinementOverride() const { return (bool)hooks.RefinementOverride; }
+    bool RefinementOverride(double x, double p, int depth, int particleType) const {
+        return hooks.RefinementOverride && hooks.RefinementOverride(x, p, depth, particleType);
+    }
+    // the inputs as the C description (what checkpoints record)
+    vx_settings_desc Describe() const;
+    // Settings.cpp:60-125: the run banner (configuration, particles, output
+    // switches), in the reference's layout
+    std::string Banner() const;
 
 private:
+    std::shared_ptr<const vx::Scenario> scenario_;  // keeps a scenario's hooks' target alive
     void finish();
 };
 
@@ -199,6 +233,9 @@ private:
 typedef std::pair<int, int> coords;  // veritas.hpp:45-48
 typedef std::pair<coords, coords> rect;
 typedef std::vector<rect> level;
+// Mesh::splitRectangle (Mesh.cpp:644-792): the boxes that cover the flagged
+// cells of box b (host/amr.cpp)
+level SplitBox(const rect &b, const std::vector<coords> &flagged, double minEfficiency);
 // Mesh.cpp:315-339 for one box (host/amr.cpp)
 void cells_under_box(int nmax, int pmax, double fx, double fp, int i0, int i1, int j0, int j1,
                      std::vector<coords> &out);
@@ -293,7 +330,14 @@ public:
     void Advance(double timeStep);
     void AdvanceFields(double timeStep);
     double CalculateDt(double cfl) { return cfl * EMSolver->EstimateCFLBound(); }
-    void reGrid();  // SolverManager.cpp:48-58
+    // SolverManager.cpp:48-58: regrid every species and add a row (t in
+    // laser periods tempEM[0] / c, as the reference prints) to the screen table
+    void reGrid(double t);
+    // SolverManager.cpp:9-22 and 73-100: the progress table the reference
+    // prints (header, the set-up row, a row per regrid); collected here, and
+    // echoed to stdout when echo_screen is set
+    const std::string &Screen() const { return screen_; }
+    bool echo_screen = false;
     // SolverManager.cpp:103-160 and 64-70: the arrays are copied from the
     // device, then formatted and appended to output/ on a writer thread
     void fileOutput(double t);
@@ -336,6 +380,9 @@ public:
     Settings &settings;
 
 private:
+    std::string screen_;
+    void ScreenWrite(const std::string &s);
+    void ScreenRow(const Mesh &mesh);  // SolverManager.cpp:73-100
     std::shared_ptr<struct OutputFiles> files;
     std::thread writer;
     void StepKernels(double timeStep);  // SolverManager.cpp:28-39, launched eagerly

@@ -1,7 +1,10 @@
-// Scenario definitions: the inputs of a run (grid sizes, species, initial
-// distribution, laser drive).  These play the role of the user hooks that the
-// reference leaves to the application (veritas.cpp:7-115: initialConditions,
-// settingsOverride, InitialDistribution, GetBY, GetBZ, RefinementOverride).
+// Built-in scenarios: the inputs of a run (grid sizes, species, initial
+// distribution, laser drive) for the three configurations benchmarked here.
+// A scenario is one provider of what the reference leaves to the application
+// (veritas.cpp:7-115: initialConditions, settingsOverride,
+// InitialDistribution, GetBY, GetBZ, RefinementOverride): desc() gives the
+// Input/Particles values and hooks() the hook functions, which any other
+// caller supplies itself (vx_settings_desc, vx_hooks in vx_params.h).
 //
 // Everything here is an *input*: the GPU framework and the CPU oracle both read
 // the same scenario so that they start from identical data.  Physical
@@ -145,11 +148,45 @@ struct Scenario {
         double E = a0 * Ec * laser_env(t - x / vx_cs) * std::sin(w0 * t - k0 * x);
         return E / vx_cs;
     }
-    // the scenarios here define no refinement override (the hook returns false)
-    bool has_refinement_override() const { return false; }
-    bool refinement_override(double x, double p, int depth, int species) const {
-        (void)x; (void)p; (void)depth; (void)species;
-        return false;
+
+    // The scenario as the reference's inputs: Input and Particles
+    // (Settings.hpp:4-31) with one species, and the user hooks pointing at
+    // this scenario (which must outlive their use).
+    vx_settings_desc desc() const {
+        vx_settings_desc d{};
+        d.minEfficiency = min_efficiency;
+        d.dx = dx();
+        d.refinementCriteria = refinement_criteria;
+        d.cfl = cfl;
+        d.sizeWeight = size_weight;
+        d.preLength = pre_length;
+        d.postLength = post_length;
+        d.nx = (uint32_t)nx;
+        d.r = (uint32_t)r;
+        d.Lfinest = (uint32_t)levels;
+        d.quadratureDepth = quadrature_depth;
+        d.n_species = 1;
+        d.mass[0] = mass;
+        d.charge[0] = charge;
+        d.dp[0] = this->dp();
+        d.pmin[0] = pmin_si();
+        d.np[0] = (uint32_t)np;
+        return d;
+    }
+    static double hook_initial(void *u, double x, double p, int32_t sp) {
+        return ((const Scenario *)u)->initial(x, p, sp);
+    }
+    static double hook_by(void *u, double x, double t) { return ((const Scenario *)u)->by(x, t); }
+    static double hook_bz(void *u, double x, double t) { return ((const Scenario *)u)->bz(x, t); }
+    vx_hooks hooks() const {
+        vx_hooks h{};
+        h.user = (void *)this;
+        h.initial_distribution = &hook_initial;
+        h.get_by = &hook_by;
+        h.get_bz = &hook_bz;
+        h.refinement_override = nullptr;  // the built-in scenarios define none
+        h.settings_override = nullptr;
+        return h;
     }
 
     static Scenario from_params(const vx_scenario_params &c) {

@@ -34,6 +34,41 @@ typedef struct vx_scenario_params {
     double gap;               /* laser: vacuum gap (wavelengths) splitting the plateau, 0 = none */
 } vx_scenario_params;
 
+/* The user hooks of the reference (Settings.hpp:45-64, written by the user in
+ * veritas.cpp:36-115) as C function pointers; each receives `user` first.
+ * Positions, momenta and times are SI.  A NULL get_by/get_bz drives nothing
+ * (0), a NULL refinement_override never flags, a NULL settings_override
+ * changes nothing; initial_distribution is required. */
+typedef struct vx_settings_desc vx_settings_desc;
+typedef struct vx_hooks {
+    void *user;
+    /* Settings::InitialDistribution(x, p, particleType) */
+    double (*initial_distribution)(void *user, double x, double p, int32_t species);
+    /* Settings::GetBY / GetBZ(x, t): transverse B at the left edge */
+    double (*get_by)(void *user, double x, double t);
+    double (*get_bz)(void *user, double x, double t);
+    /* Settings::RefinementOverride(x, p, depth, particleType): nonzero flags the cell */
+    int32_t (*refinement_override)(void *user, double x, double p, int32_t depth, int32_t species);
+    /* Settings::settingsOverride(): may change any value of the description
+     * after the inputs are copied and before they are checked (Settings.cpp:31) */
+    void (*settings_override)(void *user, vx_settings_desc *settings);
+} vx_hooks;
+
+#define VX_MAX_SPECIES 8
+
+/* Input and Particles of Settings.hpp:4-31 (SI), plus Settings::quadratureDepth
+ * (set by settingsOverride in the reference, veritas.cpp:51). */
+struct vx_settings_desc {
+    double minEfficiency, dx, refinementCriteria, cfl, sizeWeight, preLength, postLength;
+    uint32_t nx, r, Lfinest;        /* Input::nx, r, Lfinest */
+    int32_t quadratureDepth;
+    int32_t n_species;              /* entries used in the per-species arrays */
+    int32_t pad0;
+    double mass[VX_MAX_SPECIES], charge[VX_MAX_SPECIES];  /* Particles::mass, charge */
+    double dp[VX_MAX_SPECIES], pmin[VX_MAX_SPECIES];      /* Particles::dp, pmin */
+    uint32_t np[VX_MAX_SPECIES];                          /* Particles::np */
+};
+
 #ifdef __cplusplus
 }
 #endif

@@ -97,3 +97,90 @@ def laser(**kw):
              lambda0=1.0e-6, refinement_criteria=1e-3, min_efficiency=0.6, size_weight=0.0)
     d.update(kw)
     return _base(**d)
+
+
+# ---------------------------------------------------------------- reference inputs and hooks
+
+MAX_SPECIES = 8
+
+# physical constants of the reference (veritas.hpp:22-29; veritas.cpp:15-16)
+C = 299792458.0
+EPS0 = 8.854187817e-12
+M_E = 9.10938291e-31
+Q_E = 1.60217657e-19
+
+
+class SettingsDesc(ctypes.Structure):
+    """The C struct `vx_settings_desc` (veritas_amd/csrc/vx_params.h): the
+    reference's Input and Particles (Settings.hpp:4-31) in SI units, plus
+    Settings::quadratureDepth.  Per-species arrays hold `n_species` entries."""
+    _fields_ = [
+        ("minEfficiency", ctypes.c_double),
+        ("dx", ctypes.c_double),
+        ("refinementCriteria", ctypes.c_double),
+        ("cfl", ctypes.c_double),
+        ("sizeWeight", ctypes.c_double),
+        ("preLength", ctypes.c_double),
+        ("postLength", ctypes.c_double),
+        ("nx", ctypes.c_uint32),
+        ("r", ctypes.c_uint32),
+        ("Lfinest", ctypes.c_uint32),
+        ("quadratureDepth", ctypes.c_int32),
+        ("n_species", ctypes.c_int32),
+        ("pad0", ctypes.c_int32),
+        ("mass", ctypes.c_double * MAX_SPECIES),
+        ("charge", ctypes.c_double * MAX_SPECIES),
+        ("dp", ctypes.c_double * MAX_SPECIES),
+        ("pmin", ctypes.c_double * MAX_SPECIES),
+        ("np", ctypes.c_uint32 * MAX_SPECIES),
+    ]
+
+    @classmethod
+    def make(cls, nx, dx, species, Lfinest=1, r=2, cfl=0.4, minEfficiency=0.6,
+             refinementCriteria=1e-8, sizeWeight=0.0, preLength=0.0, postLength=0.0,
+             quadratureDepth=1):
+        """`species`: list of dicts with mass, charge, np, dp, pmin (SI)."""
+        d = cls(minEfficiency=minEfficiency, dx=dx, refinementCriteria=refinementCriteria, cfl=cfl,
+                sizeWeight=sizeWeight, preLength=preLength, postLength=postLength, nx=nx, r=r,
+                Lfinest=Lfinest, quadratureDepth=quadratureDepth, n_species=len(species))
+        if not 1 <= len(species) <= MAX_SPECIES:
+            raise ValueError("1 to 8 species")
+        for i, s in enumerate(species):
+            d.mass[i], d.charge[i] = s["mass"], s["charge"]
+            d.np[i], d.dp[i], d.pmin[i] = s["np"], s["dp"], s["pmin"]
+        return d
+
+
+_INIT = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                         ctypes.c_int32)
+_FIELD = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_void_p, ctypes.c_double, ctypes.c_double)
+_REFINE = ctypes.CFUNCTYPE(ctypes.c_int32, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                           ctypes.c_int32, ctypes.c_int32)
+_OVERRIDE = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(SettingsDesc))
+
+
+class HooksC(ctypes.Structure):
+    """The C struct `vx_hooks`."""
+    _fields_ = [("user", ctypes.c_void_p), ("initial_distribution", _INIT), ("get_by", _FIELD),
+                ("get_bz", _FIELD), ("refinement_override", _REFINE),
+                ("settings_override", _OVERRIDE)]
+
+
+class Hooks:
+    """The reference's user hooks (Settings.hpp:45-64) as Python callables:
+    initial_distribution(x, p, species) -> f (SI), get_by(x, t) / get_bz(x, t)
+    -> B at the left edge, refinement_override(x, p, depth, species) -> bool,
+    settings_override(desc) edits a SettingsDesc in place.  Only the first is
+    required.  The object keeps the C callbacks alive; keep it for as long as
+    a run made with it lives."""
+
+    def __init__(self, initial_distribution, get_by=None, get_bz=None, refinement_override=None,
+                 settings_override=None):
+        self._init = _INIT(lambda u, x, p, s: float(initial_distribution(x, p, s)))
+        self._by = _FIELD(lambda u, x, t: float(get_by(x, t))) if get_by else _FIELD()
+        self._bz = _FIELD(lambda u, x, t: float(get_bz(x, t))) if get_bz else _FIELD()
+        self._ref = (_REFINE(lambda u, x, p, d, s: int(bool(refinement_override(x, p, d, s))))
+                     if refinement_override else _REFINE())
+        self._ovr = (_OVERRIDE(lambda u, d: settings_override(d.contents))
+                     if settings_override else _OVERRIDE())
+        self.c = HooksC(None, self._init, self._by, self._bz, self._ref, self._ovr)
