@@ -125,6 +125,8 @@ Settings::Settings(const vx::Scenario &s) : Settings(s.desc(), s.hooks()) {
     auto own = std::make_shared<const vx::Scenario>(s);
     hooks = Hooks::FromC(own->hooks());
     scenario_ = own;
+    identity = {(double)s.kind, s.n_ref, s.length, s.pmin, s.pmax, s.theta, s.amplitude, s.k, s.drift,
+                (double)s.modes, (double)s.seed, s.a0, s.n_over_nc, s.ramp, s.fwhm_cycles, s.lambda0, s.gap};
 }
 
 vx_settings_desc Settings::Describe() const {

@@ -82,6 +82,9 @@ public:
     std::vector<double> tempEM;               // Input::tempEM, for the user's hooks
     std::vector<std::vector<double>> temp;    // Particles::misc, for the user's hooks
     Hooks hooks;
+    // the physics of a built-in scenario behind the hooks (empty for user
+    // hooks), which checkpoints compare
+    std::vector<double> identity;
 
     // Settings.cpp:5-58: copy the inputs, run settingsOverride, check, derive
     Settings(const Input &grid, const Particles &particles, const Output &out, Hooks hooks);
