@@ -210,6 +210,18 @@ size_t vx_field_work_elems(int32_t n_f);
  * Poisson solve of the reference's matrix (EMSolver.cpp:28-88) to near correct
  * rounding, E on the finest grid and the Ex0 update. */
 int vx_poisson(vx_ctx *ctx, const vx_field_state *fs);
+/* vx_poisson that allocates nothing and needs no workspace: the same kernels
+ * and bits, with the multi-workgroup solve (n_f > 4096) working in a fixed
+ * device array of the library instead of fs->work (which may be NULL).  No
+ * counterpart in the reference: for callers that cannot allocate device
+ * memory (code recorded into a graph, tight serving loops, harnesses that
+ * forbid hipMalloc).  n_f above VX_POISSON_NOALLOC_MAX_NF is rejected
+ * (VX_EINVAL).  The array (3.67 MB) is held by each GPU the library runs on
+ * until the process ends and is shared by every call on that GPU: calls that
+ * overlap on one GPU (other contexts, streams or threads) race.  Nothing in it
+ * needs to start at zero: every call writes each entry before reading it. */
+#define VX_POISSON_NOALLOC_MAX_NF 65536
+int vx_poisson_noalloc(vx_ctx *ctx, const vx_field_state *fs);
 /* EMFieldSolver::RGKStep (EMSolver.cpp:194-202): transverse-field RHS, RK
  * update and WENO A^2 at faces.  bz0/by0 are the GetBZ/GetBY hook values at
  * x = 0 for this stage's time (veritas.cpp:80-104). */

@@ -18,9 +18,11 @@
 // Transverse fields (EMSolver.cpp:194-338, 479-619): one thread per cell.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "vx_dev.h"
+
 #include "vx_internal.h"
 
 using namespace vxi;
@@ -88,7 +90,7 @@ constexpr int HALO = 2;
 constexpr int CW = NG - 1 + HALO;  // y window margin: convolution reach + halo
 constexpr int NSPEC = 6;           // entries N-3, N-2, N-1, 0, 1, 2
 
-__host__ __device__ inline int pm_chunks(int n) { return (n + CHUNK - 1) / CHUNK; }
+__host__ __device__ constexpr int pm_chunks(int n) { return (n + CHUNK - 1) / CHUNK; }
 
 // Work layout (doubles): b | xa | xb | ra | rb | S | y | Pa[nb] | Pb[nb] | Ex0 on entry
 struct PmWork {
@@ -670,6 +672,19 @@ __global__ void k_neutralize(vx_field_state F) {
     F.neutral[i] = -F.charge[i];
 }
 
+// ------------------------------------------------ fixed work array of vx_poisson_noalloc
+// The multi-workgroup solve's work vector (pm_work layout) for n_f up to
+// VX_POISSON_NOALLOC_MAX_NF: 3.67 MB that each GPU the library runs on holds
+// until the process ends.  Host code never uses this name as an address: the
+// device address comes from hipGetSymbolAddress, once per device, with that
+// device current (scratch_address, at context creation).
+constexpr size_t NOALLOC_WORK =
+    7 * (size_t)VX_POISSON_NOALLOC_MAX_NF + 2 * (size_t)pm_chunks(VX_POISSON_NOALLOC_MAX_NF) + 1;
+static __device__ __align__(256) double scratch[NOALLOC_WORK];
+constexpr int MAX_DEVICES = 64;
+static std::mutex scratch_mu;
+static double *scratch_ptr[MAX_DEVICES] = {};
+
 int validate_fs(vx_ctx *ctx, const vx_field_state *fs) {
     if (!fs) return fail(ctx, VX_EINVAL, "null field state");
     if (fs->n_f < 8 || fs->n_prepad < 2 || fs->n_postpad < 2 ||
@@ -681,11 +696,29 @@ int validate_fs(vx_ctx *ctx, const vx_field_state *fs) {
 
 }  // namespace
 
+// Called by vx_ctx_create with ctx->device current: the device address of
+// this GPU's fixed work array (looked up once per device), or a recorded error.
+int vx_noalloc_scratch_init(vx_ctx *ctx) {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (ctx->device < 0 || ctx->device >= MAX_DEVICES) return VX_EINVAL;
+    if (!scratch_ptr[ctx->device]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            ctx->err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+            return VX_EHIP;
+        }
+        scratch_ptr[ctx->device] = (double *)p;
+    }
+    ctx->poisson_scratch = scratch_ptr[ctx->device];
+    return VX_OK;
+}
+
 extern "C" {
 
 size_t vx_field_work_elems(int32_t n_f) { return (size_t)7 * n_f + 2 * (size_t)pm_chunks(n_f) + 1; }
 
-int vx_poisson(vx_ctx *ctx, const vx_field_state *fs) {
+static int poisson(vx_ctx *ctx, const vx_field_state *fs, double *work) {
     int rc = validate_fs(ctx, fs);
     if (rc) return rc;
     PoissonArgs P;
@@ -717,7 +750,8 @@ int vx_poisson(vx_ctx *ctx, const vx_field_state *fs) {
     for (int k = 0; k < NG; k++) M.g[k] = P.g[k];
     M.invN = P.invN;
     M.nb = pm_chunks(fs->n_f);
-    M.W = pm_work(fs->work, fs->n_f);
+    if (!work) return fail(ctx, VX_EINVAL, "poisson: no work array for n_f = %d", fs->n_f);
+    M.W = pm_work(work, fs->n_f);
     const PmWork &W = M.W;
     const unsigned nb = (unsigned)M.nb;
     k_pm_rhs<<<nb, MT, 0, ctx->stream>>>(M);
@@ -738,6 +772,19 @@ int vx_poisson(vx_ctx *ctx, const vx_field_state *fs) {
         else k_pm_conv<2><<<nb, MT, 0, ctx->stream>>>(M);
     }
     return check_launch(ctx, "k_poisson");
+}
+
+
+int vx_poisson(vx_ctx *ctx, const vx_field_state *fs) { return poisson(ctx, fs, fs ? fs->work : nullptr); }
+
+int vx_poisson_noalloc(vx_ctx *ctx, const vx_field_state *fs) {
+    int rc = validate_fs(ctx, fs);
+    if (rc) return rc;
+    if (fs->n_f > VX_POISSON_NOALLOC_MAX_NF)
+        return fail(ctx, VX_EINVAL, "vx_poisson_noalloc: n_f = %d exceeds its fixed work array (n_f <= %d)", fs->n_f,
+                    VX_POISSON_NOALLOC_MAX_NF);
+    if (!ctx->poisson_scratch) return fail(ctx, VX_EHIP, "vx_poisson_noalloc: no work array address on device %d", ctx->device);
+    return poisson(ctx, fs, ctx->poisson_scratch);
 }
 
 }  // extern "C"

@@ -42,6 +42,9 @@ struct vx_ctx {
     void (*batch_ring_free)(void *) = nullptr;
     // step-graph recording (vx_graph_begin / vx_graph_end)
     bool capturing = false;
+    // this device's fixed work array of vx_poisson_noalloc (vx_field.hip), from
+    // hipGetSymbolAddress at context creation; null if that failed
+    double *poisson_scratch = nullptr;
     // pinned staging ring of vx_upload_small
     static constexpr int RING = 64, RING_BYTES = 256;
     char *ring = nullptr;
@@ -49,6 +52,9 @@ struct vx_ctx {
     bool ring_pending[RING] = {};
     int ring_next = 0;
 };
+
+// vx_field.hip: looks up this device's fixed work array (ctx->device current)
+int vx_noalloc_scratch_init(vx_ctx *ctx);
 
 struct vx_graph {
     hipGraphExec_t exec = nullptr;

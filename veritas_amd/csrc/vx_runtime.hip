@@ -26,6 +26,9 @@ int vx_ctx_create(int device, vx_ctx **out) {
     int ncu = 0;
     if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && ncu > 0)
         c->n_cu = ncu;
+    // the fixed work array of vx_poisson_noalloc; on failure that entry point
+    // reports the error and every other one works
+    (void)vx_noalloc_scratch_init(c);
     *out = c;
     return VX_OK;
 }
