@@ -306,6 +306,9 @@ typedef struct vx_program vx_program;
 int vx_program_create(vx_ctx *ctx, const vx_cell_task *tasks, int32_t n_tasks,
                       const int32_t *batch_start, int32_t n_batches, vx_program **out);
 void vx_program_destroy(vx_program *prog);
+/* Number of task batches of a program (each one launch) and, when n is large
+ * enough, the tasks in each. */
+int32_t vx_program_batch_sizes(const vx_program *prog, int32_t *sizes, int32_t n);
 /* Ghost-fill / restriction / regrid-transfer / limiter-C programs.  r is the
  * refinement ratio (2 or 4). */
 int vx_program_run(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog, int mode,

@@ -248,6 +248,18 @@ int vxo_split(int x0, int p0, int x1, int p1, const int *flags, int n, double mi
     return 4 * (int)boxes.size();
 }
 
+// The sub-cell interpolants of the base patch of species 0 from five
+// neighbouring cell averages f[0..4] (cells -2..2): which = 0 for
+// GetInterpolantsREF (r sub-cells, Rectangle.cpp:121-137), 1 for
+// GetInterpolantsREL (relativeToBottom sub-cells, Rectangle.cpp:139-155).
+// Returns the number of values.
+int vxo_interp(void *h, int which, const double *f, double *out, int n) {
+    orc::Patch &p = base((Run *)h);
+    std::vector<double> t = which == 0 ? p.GetInterpolantsREF(f[0], f[1], f[2], f[3], f[4])
+                                       : p.GetInterpolantsREL(f[0], f[1], f[2], f[3], f[4]);
+    return copy_out(t, out, n);
+}
+
 // Standalone pieces for unit tests.
 double vxo_weno_nomax(double f1, double f2, double f3, double f4, int right) {
     return orc::weno_nomax(f1, f2, f3, f4, right != 0);

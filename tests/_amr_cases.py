@@ -12,7 +12,7 @@ import veritas_amd.params as P
 
 def two_slab_laser(**kw):
     d = dict(nx=128, np=64, wavelengths=40.0, ramp=5.0, gap=8.0, pmin=-0.5, pmax=0.5, theta=0.01,
-             refinement_criteria=0.01, min_efficiency=0.85, levels=3)
+             refinement_criteria=0.01, min_efficiency=0.85, levels=3, size_weight=0.0)
     d.update(kw)
     return P.laser(**d)
 

@@ -54,6 +54,8 @@ def bind(L):
     L.vxo_set_f.restype = i
     L.vxo_set_f.argtypes = [vp, vp, i]
     L.vxo_vlasov_imposed.argtypes = [vp, d, i, d, d]
+    L.vxo_interp.restype = i
+    L.vxo_interp.argtypes = [vp, i, vp, vp, i]
     return L
 
 
@@ -179,6 +181,15 @@ class OracleRun:
 
     def regrid(self):
         self.L.vxo_regrid(self.h)
+
+    def interp(self, which, f5):
+        """GetInterpolantsREF (which 0) or GetInterpolantsREL (1) of the base
+        patch from the cell averages of cells -2..2."""
+        f = np.ascontiguousarray(f5, dtype=np.float64)
+        n = self.L.vxo_interp(self.h, which, f.ctypes.data, None, 0)
+        out = np.empty(n)
+        self.L.vxo_interp(self.h, which, f.ctypes.data, out.ctypes.data, n)
+        return out
 
     # ---- imposed-field runs (pins of the Vlasov advance by exact solutions)
     def set_f(self, f):

@@ -11,6 +11,8 @@ Measured on MI355X (relative L2 of f / E at 25, 50, 100 % of the run):
       4e-8/9e-9, 1e-7/2e-8;
   laser single level (128x64, 400 steps) vs the CPU oracle: 1.2e-6/2.5e-8,
       2.4e-6/1.7e-7, 1.5e-6/2.1e-7 (Ey, Bz below 1e-13);
+  three-level two-slab laser (128x64 base, 80 steps, every patch) vs the CPU
+      oracle: 3.3e-10/3.3e-9, 3.0e-7/4.0e-9, 2.3e-6/4.3e-8;
   config 2 (two-stream 4096^2, 500 steps) vs the FP64 GPU path (bit-identical
       to the oracle): 1.7e-7/9.9e-7, 4.0e-7/2.5e-6, 4.4e-5/7.1e-4 (the
       two-stream instability amplifies every difference in the late phase)."""
@@ -74,7 +76,9 @@ def test_laser_single_level_against_oracle():
     g, o = va.Run(p, mixed=True), OracleRun(p)
     err = run_pair(g, o, (100, 200, 400))
     for m, e in err.items():
-        assert e["f"] < 1e-5 and e["E"] < 1e-6, (m, e)
+        # the 1e-6 bar of the mixed form holds for E here but not for f on this
+        # coarse grid (measured 1.2e-6, 2.4e-6, 1.5e-6): pinned just above
+        assert e["f"] < 3e-6 and e["E"] < 1e-6, (m, e)
         assert e["Ey"] < 1e-12 and e["Bz"] < 1e-12, (m, e)
         assert e["N"] < 1e-13, (m, e)
     g.close()
@@ -166,7 +170,8 @@ def test_laser_three_levels_against_oracle():
              "Ey": rel(g.get("Ey"), o.get("Ey")), "Bz": rel(g.get("Bz"), o.get("Bz")),
              "N": abs(particle_number(_GpuNesting(g, o), s["dx"], s["dp"]) / n0 - 1)}
         say(f"  step {m}: {e}")
-        assert e["f"] < 1e-5 and e["E"] < 1e-6, (m, e)
+        # measured 3.3e-10, 3.0e-7, 2.3e-6 in f: pinned just above (E stays below 1e-7)
+        assert e["f"] < 3e-6 and e["E"] < 1e-6, (m, e)
         assert e["Ey"] < 1e-12 and e["Bz"] < 1e-12, (m, e)
         assert e["N"] < 1e-13, (m, e)
     assert e["f"] > 0.0  # the mixed-precision launches ran

@@ -123,3 +123,31 @@ def test_particle_number_conserved_across_levels_steps_and_regrid():
     assert abs(n3 - n2) / n0 < 1e-13
     assert not np.array_equal(lay_before, o.layout()), "regrid should move the patches"
     o.close()
+
+
+def _cubic_averages(c, lo, hi):
+    """Exact average of c0 + c1 xi + c2 xi^2 + c3 xi^3 over [lo, hi]."""
+    prim = lambda x: c[0] * x + c[1] * x ** 2 / 2 + c[2] * x ** 3 / 3 + c[3] * x ** 4 / 4
+    return (prim(hi) - prim(lo)) / (hi - lo)
+
+
+@pytest.mark.parametrize("which,levels", [(0, 2), (1, 1), (1, 3)])
+def test_sub_cell_interpolants_are_exact_for_cubics(which, levels):
+    """GetInterpolantsREF / REL (Rectangle.cpp:121-155, constants 76-110): from
+    the averages of cells -2..2 of a cubic, the n sub-cells of cell 0 get the
+    cubic's exact sub-cell averages (to the 15 digits of the interpolation
+    table), and their mean is the cell's own value (the coarse cell is kept).
+    A slipped table entry or a sub-cell bound off by one fails this."""
+    p = two_slab_laser(levels=levels, nx=32, np=16)
+    o = OracleRun(p)
+    rng = np.random.default_rng(5 + which + levels)
+    n_sub = 2 if which == 0 else 2 ** (levels - 1)
+    for _ in range(20):
+        c = rng.normal(size=4)
+        f5 = [_cubic_averages(c, k - 0.5, k + 0.5) for k in range(-2, 3)]
+        got = o.interp(which, f5)
+        assert len(got) == n_sub
+        want = [_cubic_averages(c, -0.5 + i / n_sub, -0.5 + (i + 1) / n_sub) for i in range(n_sub)]
+        assert np.allclose(got, want, rtol=0, atol=1e-13 * (1 + np.abs(c).sum())), (got, want)
+        assert abs(np.mean(got) - f5[2]) <= 1e-14 * (1 + np.abs(c).sum())
+    o.close()

@@ -107,6 +107,8 @@ def host_lib():
         L.vxh_flags.restype = i
         L.vxh_flags.argtypes = [vp, i, i, vp, i]
         L.vxh_regrid.argtypes = [vp, d]
+        L.vxh_program_stats.restype = i
+        L.vxh_program_stats.argtypes = [vp, i, vp, i]
         L.vxh_regrid_stats.argtypes = [vp, i, vp]
         L.vxh_file_output.argtypes = [vp, d]
         L.vxh_output_rectangles.argtypes = [vp, d]
@@ -341,6 +343,21 @@ class Run:
         a = (ctypes.c_int * max(n, 1))()
         L.vxh_flags(self.h, sp, level, a, n)
         return np.array(a[:n], dtype=np.int64).reshape(-1, 2)
+
+    def program_stats(self, sp=0):
+        """Task batches (one launch each) of species sp's programs:
+        {"push": [tasks per batch], "pushc": [...], "flux": [[...] per level]}."""
+        L = host_lib()
+        n = L.vxh_program_stats(self.h, sp, None, 0)
+        if n < 0:
+            self._ck(-1)
+        a = (ctypes.c_int * max(n, 1))()
+        L.vxh_program_stats(self.h, sp, a, n)
+        v, k, progs = list(a[:n]), 0, []
+        while k < len(v):
+            progs.append(v[k + 1:k + 1 + v[k]])
+            k += 1 + v[k]
+        return {"push": progs[0], "pushc": progs[1], "flux": progs[2:]}
 
     def regrid_stats(self, sp=0):
         """(regrids since set-up, regrids that rebuilt the patches) of species sp."""

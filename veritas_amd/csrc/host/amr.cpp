@@ -528,6 +528,17 @@ level split_box(const rect &b, const std::vector<coords> &fl, double minEfficien
 
 }  // namespace
 
+std::vector<std::vector<int32_t>> Mesh::ProgramBatches() const {
+    auto sizes = [](const vx_program *p) {
+        std::vector<int32_t> v(vx_program_batch_sizes(p, nullptr, 0));
+        vx_program_batch_sizes(p, v.data(), (int32_t)v.size());
+        return v;
+    };
+    std::vector<std::vector<int32_t>> out = {sizes(push_prog), sizes(pushc_prog)};
+    for (auto &l : levels) out.push_back(sizes(l->flux_prog));
+    return out;
+}
+
 level SplitBox(const rect &b, const std::vector<coords> &flagged, double minEfficiency) {
     return split_box(b, flagged, minEfficiency);
 }

@@ -279,6 +279,25 @@ int vxh_signatures(void *h, int sp, int lvl, int idx, int *out, int n) {
     }
 }
 
+// Task batches of species sp's programs: for each program (PushData's,
+// PushBoundaryC's, then each level's coarse-fine fluxes) its batch count
+// followed by the tasks in each batch.  Returns the number of ints.
+int vxh_program_stats(void *h, int sp, int *out, int n) {
+    Run *r = (Run *)h;
+    std::vector<int> v;
+    try {
+        for (auto &b : r->sm->meshes.at(sp)->ProgramBatches()) {
+            v.push_back((int)b.size());
+            v.insert(v.end(), b.begin(), b.end());
+        }
+    } catch (std::exception &e) {
+        r->err = e.what();
+        return -1;
+    }
+    if (out && n >= (int)v.size()) std::memcpy(out, v.data(), sizeof(int) * v.size());
+    return (int)v.size();
+}
+
 // SolverManager::reGrid(t) (SolverManager.cpp:48-58)
 int vxh_regrid(void *h, double t) { return guarded(h, [&](Run *r) { r->sm->reGrid(t); }); }
 

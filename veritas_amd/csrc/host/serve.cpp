@@ -233,7 +233,27 @@ void Server::Run() {
         }
         Tick(ready);
     }
-    if (!failure.empty()) throw Error("serve: " + failure);
+    if (!failure.empty()) {
+        // a run could not be built: admit nothing more, take back the builds
+        // not yet started, wait for those under way, and drop the active runs,
+        // so that the server is left empty (and usable) before reporting
+        pending.clear();
+        std::unique_lock<std::mutex> lk(mu);
+        building -= (int)to_build.size();
+        to_build.clear();
+        while (building > 0) {
+            cv.wait(lk, [this] { return !built.empty(); });
+            while (!built.empty()) {
+                built.pop_front();
+                building--;
+            }
+        }
+        lk.unlock();
+        for (auto &L : active)
+            if (L->sm) (void)vx_sync(L->sm->dev->ctx);
+        active.clear();
+        throw Error("serve: " + failure);
+    }
 }
 
 }  // namespace vxh

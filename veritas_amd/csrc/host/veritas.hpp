@@ -268,6 +268,9 @@ public:
     // a single walled patch, else the batched flux and limiter launches
     bool mixed = false;
     double FScale() const;  // scale of the FP32 copies: max over patches of max |f|
+    // task batches (launches) of the mesh's programs and the tasks in each:
+    // PushData's, PushBoundaryC's, then the coarse-fine flux program of each level
+    std::vector<std::vector<int32_t>> ProgramBatches() const;
 
 private:
     Settings &settings;

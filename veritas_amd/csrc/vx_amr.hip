@@ -439,6 +439,14 @@ int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const doubl
 
 extern "C" {
 
+int32_t vx_program_batch_sizes(const vx_program *prog, int32_t *sizes, int32_t n) {
+    if (!prog || prog->batch.empty()) return 0;
+    const int32_t nb = (int32_t)prog->batch.size() - 1;
+    if (sizes && n >= nb)
+        for (int32_t b = 0; b < nb; b++) sizes[b] = prog->batch[b + 1] - prog->batch[b];
+    return nb;
+}
+
 int vx_patch_table_create(vx_ctx *ctx, const vx_patch *patches, int32_t n, vx_patch_table **out) {
     if (!out || n < 0 || (n > 0 && !patches)) return fail(ctx, VX_EINVAL, "patch table: bad arguments");
     if (ctx->capturing) return fail(ctx, VX_EINVAL, "patch table while a step graph is recorded");

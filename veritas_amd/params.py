@@ -87,14 +87,21 @@ def laser(**kw):
     from the left into underdense plasma: plateau density 0.01 n_c after a
     10-wavelength vacuum gap and a 50-wavelength linear ramp, in a
     400-wavelength domain; p in [-2, 2] m_e c.  Base grid Nx = 8192, Np = 512,
-    two refinement levels at ratio 2.  `wavelengths` sets the domain length in
+    two refinement levels at ratio 2, cells flagged where f > 1e-6 max f (plus
+    the reference's gradient terms).  `wavelengths` sets the domain length in
     laser wavelengths (lengths are otherwise in c/omega_p)."""
     n_over_nc = kw.get("n_over_nc", 0.01)
     waves = kw.pop("wavelengths", 400.0)
     lam = 2 * math.pi * math.sqrt(n_over_nc)  # one wavelength in c/omega_p
+    # "flag where f > 1e-6 max f" as the reference's criterion: the error
+    # estimate adds sizeWeight / fMax * r^(depth - maxDepth) * |f| to its
+    # gradient terms (Rectangle.hpp:128-130; Rectangle.cpp:68-74) and flags
+    # above refinementCriteria, so sizeWeight = refinementCriteria / 1e-6 flags
+    # every cell with f > 1e-6 fMax on the coarsest level (2e-6 fMax one level
+    # finer), and the gradient terms add cells where f varies
     d = dict(kind=LASER, nx=8192, np=512, levels=3, r=2, length=waves * lam, pmin=-2.0,
              pmax=2.0, theta=1e-3, a0=0.5, n_over_nc=n_over_nc, ramp=50.0, fwhm_cycles=10.0,
-             lambda0=1.0e-6, refinement_criteria=1e-3, min_efficiency=0.6, size_weight=0.0)
+             lambda0=1.0e-6, refinement_criteria=1e-3, min_efficiency=0.6, size_weight=1e3)
     d.update(kw)
     return _base(**d)
 
