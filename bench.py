@@ -70,6 +70,11 @@ def measured_valu(kernel, nx, np_, mixed):
     return None
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON result is the only stdout line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def host_cpu():
     """Model, physical cores and threads per core of this host (lscpu)."""
     import subprocess
@@ -109,6 +114,7 @@ for _ in range(reps):
     t0 = time.perf_counter()
     o.advance(dt, steps)
     times.append(time.perf_counter() - t0)
+    print(f"bench: cpu baseline run {len(times)}: {times[-1]:.1f} s", file=sys.stderr, flush=True)
 print(json.dumps(times))
 """
 
@@ -124,6 +130,7 @@ def cpu_baseline(workload, kw, cells, steps, reps=3, label=""):
     env = dict(os.environ, OMP_NUM_THREADS=str(cpu["physical_cores"]), OMP_PROC_BIND="close",
                OMP_PLACES="cores")
     arg = json.dumps(dict(kw, workload=workload, steps=steps, reps=reps))
+    progress(f"cpu baseline: {reps} x {steps} steps on {cpu['physical_cores']} cores")
     r = subprocess.run([sys.executable, "-c", _CPU_CHILD, ROOT, arg], env=env, stdout=subprocess.PIPE,
                        text=True, timeout=900, check=True)
     times = sorted(json.loads(r.stdout.strip().splitlines()[-1]))
@@ -231,6 +238,7 @@ def bench_amr(a, va):
     mixed = a.precision == "mixed"
     run = va.Run(params, fused=False, graphs=False, mixed=mixed)  # eager launches: per-launch events
     setup = time.perf_counter() - t0
+    progress(f"laser AMR set up in {setup:.1f} s")
     lay = run.layout()
     cells = int((lay[:, 3] * lay[:, 4]).sum())
     dt = run.calc_dt(0.4)
@@ -240,6 +248,7 @@ def bench_amr(a, va):
     ms = run.time_advance(dt, 0, a.steps)
     kms, nlaunch = run.timing_end()
     graph_ms = graph_step_ms(run, dt, a.steps)
+    progress(f"timed steps: {ms / a.steps:.2f} ms/step")
     # the reference's driver regrids every 22 steps (veritas.cpp:146-151): time
     # three regrids (flags, clustering and, when the boxes change, the rebuild)
     regrid_ms = []

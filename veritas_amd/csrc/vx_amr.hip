@@ -23,6 +23,7 @@ using namespace vxd;
 struct vx_program {
     vx_ctx *ctx = nullptr;
     vx_cell_task *d = nullptr;
+    int32_t *d_batch = nullptr;   // the batch starts on the device (k_program_seq)
     std::vector<int32_t> batch;   // host copy of the batch starts
     std::vector<vx_cell_task> host;
     uint64_t checked_serial = 0;  // table this program was bounds-checked against
@@ -227,9 +228,7 @@ __device__ void flux_task(const ProgArgs &A, const double *E, const vx_cell_task
     at(xdir ? K.FxDS : K.FpDS, kp, T.di, T.dj) = s - ls;
 }
 
-__global__ __launch_bounds__(256) void k_program(ProgArgs A, const double *E) {
-    const int t = A.begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= A.end) return;
+__device__ __forceinline__ void program_task(const ProgArgs &A, const double *E, const int t) {
     const vx_cell_task T = A.task[t];
     if (T.op >= VX_OP_C_SAME && T.op <= VX_OP_C_UPDATE) {
         c_task(A, T);
@@ -257,6 +256,28 @@ __global__ __launch_bounds__(256) void k_program(ProgArgs A, const double *E) {
         at(D.f[1], D.pitch, T.di, T.dj) = v;
     } else {
         at(D.f[A.slot], D.pitch, T.di, T.dj) = v;
+    }
+}
+
+// One batch of a program: one thread per task.
+__global__ __launch_bounds__(256) void k_program(ProgArgs A, const double *E) {
+    const int t = A.begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= A.end) return;
+    program_task(A, E, t);
+}
+
+// A run of consecutive small batches of a program in one workgroup: batch b
+// (tasks [starts[b], starts[b + 1])) is done by the whole workgroup before
+// batch b + 1 starts, the order the batches' separate launches give.  The
+// workgroup fence of the barrier makes each batch's writes visible to the
+// next batch's reads in this workgroup (one compute unit).
+constexpr int SEQ_THREADS = 1024;
+__global__ __launch_bounds__(SEQ_THREADS) void k_program_seq(ProgArgs A, const double *E, const int *starts,
+                                                             int b0, int b1) {
+    for (int b = b0; b < b1; b++) {
+        const int lo = starts[b], hi = starts[b + 1];
+        for (int t = lo + (int)threadIdx.x; t < hi; t += SEQ_THREADS) program_task(A, E, t);
+        __syncthreads();
     }
 }
 
@@ -421,12 +442,27 @@ int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r
     return VX_OK;
 }
 
+// Batches of at most SEQ_MAX tasks that follow each other run together in one
+// workgroup (k_program_seq): a few passes of one compute unit cost less than
+// a launch each.  Larger batches keep a launch of their own over the device.
+constexpr int SEQ_MAX = 4 * SEQ_THREADS;
+
 int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const double *E) {
-    const size_t nb = prog->batch.empty() ? 0 : prog->batch.size() - 1;
-    for (size_t b = 0; b < nb; b++) {
+    const int nb = prog->batch.empty() ? 0 : (int)prog->batch.size() - 1;
+    for (int b = 0; b < nb;) {
+        int e = b;
+        while (e < nb && prog->batch[e + 1] - prog->batch[e] <= SEQ_MAX) e++;
+        if (e - b >= 2) {  // a run of small batches
+            k_program_seq<<<1, SEQ_THREADS, 0, ctx->stream>>>(A, E, prog->d_batch, b, e);
+            int rc = check_launch(ctx, "k_program_seq");
+            if (rc) return rc;
+            b = e;
+            continue;
+        }
         A.begin = prog->batch[b];
         A.end = prog->batch[b + 1];
         const int n = A.end - A.begin;
+        b++;
         if (n <= 0) continue;
         k_program<<<(n + 255) / 256, 256, 0, ctx->stream>>>(A, E);
         int rc = check_launch(ctx, "k_program");
@@ -514,8 +550,12 @@ int vx_program_create(vx_ctx *ctx, const vx_cell_task *tasks, int32_t n_tasks, c
         hipError_t e = hipMalloc(&p->d, sizeof(vx_cell_task) * n_tasks);
         if (e == hipSuccess)
             e = hipMemcpy(p->d, tasks, sizeof(vx_cell_task) * n_tasks, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc(&p->d_batch, sizeof(int32_t) * (n_batches + 1));
+        if (e == hipSuccess)
+            e = hipMemcpy(p->d_batch, batch_start, sizeof(int32_t) * (n_batches + 1), hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             if (p->d) (void)hipFree(p->d);
+            if (p->d_batch) (void)hipFree(p->d_batch);
             delete p;
             return fail(ctx, VX_ENOMEM, "program upload: %s", hipGetErrorString(e));
         }
@@ -527,6 +567,7 @@ int vx_program_create(vx_ctx *ctx, const vx_cell_task *tasks, int32_t n_tasks, c
 void vx_program_destroy(vx_program *prog) {
     if (!prog) return;
     if (prog->d) (void)hipFree(prog->d);
+    if (prog->d_batch) (void)hipFree(prog->d_batch);
     delete prog;
 }
 

@@ -9,6 +9,12 @@
 // Rectangle.cpp:10-38); the kernels reproduce that by computing each quantity
 // only inside its range and using 0 outside.
 #include <cmath>
+#include <initializer_list>
+#include <algorithm>
+#include <vector>
+#include <tuple>
+#include <mutex>
+#include <map>
 
 #include "vx_dev.h"
 #include "vx_internal.h"
@@ -462,34 +468,41 @@ dim3 grid_cells(int nx, int np, int extra_i, int extra_j) {
     return dim3((unsigned)((np + extra_j + 63) / 64), (unsigned)((nx + extra_i + 3) / 4));
 }
 
-// strips of the streaming sub-step-0 kernel: the tallest that still gives
-// 2048 wavefronts
-Sub0Tiles sub0_tiles(const vx_patch &p) {
-    Sub0Tiles T;
-    T.nbx = (p.n_p + 4 + S0W - 1) / S0W;
-    T.TI = 4;
-    for (int ti : {64, 32, 16, 8})
-        if ((long)T.nbx * ((p.n_x + 4 + ti - 1) / ti) >= 2048) {
-            T.TI = ti;
+// Strip height of a streaming kernel over n positions in x with nbx columns
+// of wavefronts: pol > 0 is the height itself (at most the positions), pol =
+// -W the tallest of `tall` that still gives W wavefronts for the patch.
+int strip_rows(int pol, int n, int nbx, std::initializer_list<int> tall, int floor_ti) {
+    if (pol > 0) return std::min(pol, n);
+    int ti = floor_ti;
+    for (int t : tall)
+        if ((long)nbx * ((n + t - 1) / t) >= -pol) {
+            ti = t;
             break;
         }
+    return ti;
+}
+
+// strips of the streaming sub-step-0 kernel (default: the tallest that still
+// gives 2048 wavefronts)
+Sub0Tiles sub0_tiles(const vx_patch &p, int pol = -2048) {
+    Sub0Tiles T;
+    T.nbx = (p.n_p + 4 + S0W - 1) / S0W;
+    T.TI = strip_rows(pol, p.n_x + 4, T.nbx, {64, 32, 16, 8}, 4);
     T.nstrip = (p.n_x + 4 + T.TI - 1) / T.TI;
     return T;
 }
 
-// strips of the streaming limiter: the tallest that still gives 4096 wavefronts
-LimTiles lim_tiles(const vx_patch &p) {
+// strips of the streaming limiter (default: the tallest that still gives
+// 4096 wavefronts)
+LimTiles lim_tiles(const vx_patch &p, int pol = -4096) {
     LimTiles T;
     T.nbx = (p.n_p + 4 + LIMW - 1) / LIMW;
-    T.TI = 4;
-    for (int ti : {32, 16, 8})
-        if ((long)T.nbx * ((p.n_x + 4 + ti - 1) / ti) >= 4096) {
-            T.TI = ti;
-            break;
-        }
+    T.TI = strip_rows(pol, p.n_x + 4, T.nbx, {32, 16, 8}, 4);
     T.nstrip = (p.n_x + 4 + T.TI - 1) / T.TI;
     return T;
 }
+
+int table_rebuild(vx_ctx *ctx, vx_patch_table *t);
 
 // the work list of a patch table (built once: a table's shapes never change)
 int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
@@ -500,6 +513,12 @@ int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
     }
     if (t->batch_d) return VX_OK;
     if (ctx->capturing) return fail(ctx, VX_EINVAL, "batched sub-step: work list built while recording");
+    return table_rebuild(ctx, t);
+}
+
+// (Re)build the work list of a table for its current strip policies; the
+// device copy is rewritten in place once the stream has drained.
+int table_rebuild(vx_ctx *ctx, vx_patch_table *t) {
     t->batch.assign(t->n + 1, vx_batch_info{});
     int run[5] = {0, 0, 0, 0, 0};
     for (int k = 0; k <= t->n; k++) {
@@ -507,11 +526,11 @@ int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
         for (int q = 0; q < 5; q++) b.start[q] = run[q];
         if (k == t->n) break;
         const vx_patch &p = t->host[k];
-        const Sub0Tiles T = sub0_tiles(p);
+        const Sub0Tiles T = sub0_tiles(p, t->sub0_pol);
         b.TI = T.TI;
         b.nstrip = T.nstrip;
         b.nbx = T.nbx;
-        const LimTiles TL = lim_tiles(p);
+        const LimTiles TL = lim_tiles(p, t->lim_pol);
         b.lTI = TL.TI;
         b.lnstrip = TL.nstrip;
         b.lnbx = TL.nbx;
@@ -523,9 +542,77 @@ int table_batch(vx_ctx *ctx, vx_patch_table *t, int n_f) {
         run[4] += (int)(((long)(p.n_x + 4) * p.pitch + 255) / 256);
     }
     const size_t bytes = sizeof(vx_batch_info) * t->batch.size();
-    int rc = check_hip(ctx, hipMalloc((void **)&t->batch_d, bytes), "work list");
+    int rc = VX_OK;
+    if (!t->batch_d) rc = check_hip(ctx, hipMalloc((void **)&t->batch_d, bytes), "work list");
+    if (!rc) rc = check_hip(ctx, hipStreamSynchronize(ctx->stream), "work list");
     if (!rc) rc = check_hip(ctx, hipMemcpy(t->batch_d, t->batch.data(), bytes, hipMemcpyHostToDevice), "work list");
     return rc;
+}
+
+// The strip policies of a table's batched flux (kind 0) and limiter (kind 1)
+// launches, chosen by timing on the table's first launch of each kind: every
+// candidate policy runs the launch three times (the launches are idempotent:
+// they read f, the stored fluxes of earlier stages and f^n, and write only
+// their own outputs, the same values under any policy), the fastest is kept
+// (a later candidate must be 3 % faster), and the choice is kept per device
+// for tables of the same patch shapes.
+struct PolKey {
+    int device, kind, mp;
+    std::vector<int> shapes;
+    bool operator<(const PolKey &o) const {
+        return std::tie(device, kind, mp, shapes) < std::tie(o.device, o.kind, o.mp, o.shapes);
+    }
+};
+std::mutex g_pol_mu;
+std::map<PolKey, int> g_pol;
+
+template <class Launch>
+int tune_policy(vx_ctx *ctx, vx_patch_table *t, int kind, bool mp, const std::vector<int> &cands, Launch launch) {
+    int &pol = kind == 0 ? t->sub0_pol : t->lim_pol;
+    PolKey key{ctx->device, kind, (int)mp, {}};
+    for (const vx_patch &p : t->host) {
+        key.shapes.push_back(p.n_x);
+        key.shapes.push_back(p.n_p);
+    }
+    {
+        std::lock_guard<std::mutex> g(g_pol_mu);
+        auto it = g_pol.find(key);
+        if (it != g_pol.end()) {
+            if (pol == it->second) return VX_OK;
+            pol = it->second;
+            return table_rebuild(ctx, t);
+        }
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int best = pol, rc = VX_OK;
+    float best_ms = 1e30f;
+    if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        for (int c : cands) {
+            pol = c;
+            if ((rc = table_rebuild(ctx, t))) break;
+            launch();  // warm
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) break;
+            launch();
+            launch();
+            if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
+            float ms = 0.0f;
+            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
+            if (ms < 0.97f * best_ms) {
+                best_ms = ms;
+                best = c;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if ((rc = check_launch(ctx, kind == 0 ? "k_sub0s_batch (tuning)" : "k_limiter_batch (tuning)"))) return rc;
+    pol = best;
+    {
+        std::lock_guard<std::mutex> g(g_pol_mu);
+        g_pol[key] = best;
+    }
+    return table_rebuild(ctx, t);
 }
 
 int mp_scale(vx_ctx *ctx, double f_scale, double &sinv, float &eps32, const char *what) {
@@ -563,6 +650,14 @@ int sub0_fluxes_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const
     A.fl = *fl;
     A.step = step;
     rk_coefficients(step, dt, A.a, A.ns, A.aSum);
+    if (!t->sub0_tuned[mp] && !ctx->capturing) {
+        rc = tune_policy(ctx, t, 0, mp, {-2048, 128, 64, 32, 16, 8}, [&] {
+            if (mp) launch_sub0s_batch<true>(ctx, t, step, A);
+            else launch_sub0s_batch<false>(ctx, t, step, A);
+        });
+        if (rc) return rc;
+        t->sub0_tuned[mp] = true;
+    }
     if (mp) launch_sub0s_batch<true>(ctx, t, step, A);
     else launch_sub0s_batch<false>(ctx, t, step, A);
     return check_launch(ctx, "k_sub0s_batch");
@@ -683,21 +778,29 @@ int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double
     return check_launch(ctx, "k_lo_predictor_batch");
 }
 
-int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
+static int sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, bool mp, double sinv) {
     int rc = table_batch(ctx, t, n_f);
     if (rc) return rc;
-    k_limiter_batch<false><<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, 1.0);
+    auto launch = [&] {
+        if (mp) k_limiter_batch<true><<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
+        else k_limiter_batch<false><<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
+    };
+    if (!t->lim_tuned[mp] && !ctx->capturing) {
+        if ((rc = tune_policy(ctx, t, 1, mp, {-4096, 64, 32, 16, 8, 4}, launch))) return rc;
+        t->lim_tuned[mp] = true;
+    }
+    launch();
     return check_launch(ctx, "k_limiter_batch");
 }
+
+int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) { return sub1_table(ctx, t, n_f, false, 1.0); }
 
 int vx_fct_sub1_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_scale) {
     double sinv;
     float eps32;
     int rc = mp_scale(ctx, f_scale, sinv, eps32, "sub1 table (mixed)");
-    if (!rc) rc = table_batch(ctx, t, n_f);
     if (rc) return rc;
-    k_limiter_batch<true><<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
-    return check_launch(ctx, "k_limiter_batch");
+    return sub1_table(ctx, t, n_f, true, sinv);
 }
 
 int vx_fct_sub2_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {

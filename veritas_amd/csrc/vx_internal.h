@@ -76,6 +76,12 @@ struct vx_patch_table {
     std::vector<vx_patch> host;   // shapes, for the bounds check
     vx_batch_info *batch_d = nullptr;  // n + 1 entries (the last one: totals)
     std::vector<vx_batch_info> batch;  // host copy
+    // strip-height policies of the batched sub-step-0 flux and limiter
+    // launches (vx_fct.hip): a positive value is one strip height for every
+    // patch, a negative one -W the tallest strips that still give W wavefronts
+    // per patch; chosen by timing on the table's first launch of each
+    int sub0_pol = -2048, lim_pol = -4096;
+    bool sub0_tuned[2] = {false, false}, lim_tuned[2] = {false, false};  // per precision
 };
 
 struct vx_moment_plan {
