@@ -16,6 +16,23 @@ __device__ __forceinline__ double valmin(double a, double b) { return a < b ? a 
 // std::min / std::max as used by the limiter-C sync (Rectangle.cpp:1646-1829)
 __device__ __forceinline__ double stdmin(double a, double b) { return (b < a) ? b : a; }
 
+// v_max_f64 / v_min_f64 as single instructions.  fmax/fmin compile to the
+// same instruction, but first quiet possible signalling NaNs with a
+// v_max_f64 x, x, x of every operand the compiler cannot prove canonical
+// (loaded values, lane shifts, values carried around the row loop); the
+// operands here are never NaN, so that step changes no bits and only costs
+// issue slots.
+__device__ __forceinline__ double hwmax(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double hwmin(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+
 // Rectangle.hpp:86-88: lower p-face of cell j (pmin + dp * (j + p_pos))
 __device__ __forceinline__ double momentum(double pmin, double dp, int j, int p_pos) {
     return pmin + dp * ((double)j + (double)p_pos);

@@ -32,7 +32,9 @@
 // accumulation into f stay FP64.  A face value is fR + w (fL - fR) with the
 // FP32 weight w, so its error is of order 1e-7 |fL - fR|, not 1e-7 |f|.
 
+#include <algorithm>
 #include <cmath>
+#include <functional>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -102,35 +104,33 @@ __device__ __forceinline__ int xcd_tile() {
     return xcd * q + min(xcd, rem) + (w >> 3);
 }
 
-// One workgroup's tile (4 strips of one 64-lane p-column) of a stage.
-template <int STEP, bool MP>
-__device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
+// A strip of one wavefront: TI output rows [i0, i1) of the 58 p-cells from jw.
+// IN (interior): the strip and its p-column lie so far from the patch edges
+// that every reference loop-range test below holds for every lane and every
+// march row (interior_strip states the margins), so IN only drops tests that
+// are true, and the IN and general forms give the same bits.
+template <int STEP, bool MP, bool IN>
+__device__ __forceinline__ void stage_strip(const StageArgs &A, const int jw, const int i0) {
     const vx_patch &P = A.p;
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
     const int lane = threadIdx.x & 63;
-    // the wavefront index is uniform: row loop, row ranges and field reads stay scalar
-    const int bx = tile % A.nbx, by = tile / A.nbx;
-    const int strip = by * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if (strip >= A.nstrip) return;  // uniform per wavefront
-    const int i0 = strip * A.TI;
     const int i1 = min(i0 + A.TI, nx);       // output rows [i0, i1)
     const bool last_strip = i0 + A.TI >= nx;
-    const int jw = bx * OUTW;
     const int j = jw - HALO + lane;
-    const bool out_lane = lane >= HALO && lane < 64 - HALO && j < np;
+    const bool mid_lane = lane >= HALO && lane < 64 - HALO;
+    const bool out_lane = mid_lane && (IN || j < np);
     // faces this wavefront stores (each valid face has exactly one owner)
-    const bool own_row_x = (lane >= HALO && lane < 64 - HALO && j < np) ||
-                           (j == np && lane >= HALO && lane < 65 - HALO) ||
-                           (jw == 0 && j == -1);
-    const bool own_row_p = (lane >= HALO && lane < 64 - HALO && j < np) ||
-                           (j == np && lane >= HALO && lane < 65 - HALO);
+    const bool own_row_x = IN ? mid_lane
+                              : (mid_lane && j < np) || (j == np && lane >= HALO && lane < 65 - HALO) ||
+                                    (jw == 0 && j == -1);
+    const bool own_row_p = IN ? mid_lane : (mid_lane && j < np) || (j == np && lane >= HALO && lane < 65 - HALO);
     constexpr bool store_flux = STEP < 5;
 
     // reference loop ranges in j (Rectangle.cpp:1279-1394) for this lane
-    const bool jr_ex = j >= -1 && j < np + 1;   // ex, fx, FxH, FxL rows
-    const bool jr_ep = j >= -1 && j < np + 2;   // ep
-    const bool jr_fp = j >= 0 && j < np + 1;    // fp, FpH, FpL p-faces
-    const bool jr_R = j >= -1 && j < np + 1;    // R
+    const bool jr_ex = IN || (j >= -1 && j < np + 1);   // ex, fx, FxH, FxL rows
+    const bool jr_ep = IN || (j >= -1 && j < np + 2);   // ep
+    const bool jr_fp = IN || (j >= 0 && j < np + 1);    // fp, FpH, FpL p-faces
+    const bool jr_R = IN || (j >= -1 && j < np + 1);    // R
     // update loop ranges (walls on all sides: xm = 1, xp = n_x, pm = 1, pp = n_p)
     const int xm = P.left ? 1 : 0, xp = P.right ? nx : nx + 1;
     const int pm = P.down ? 1 : 0, pp = P.up ? np : np + 1;
@@ -167,17 +167,17 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
 
     // software prefetch: rows r and r+1 are in flight before iteration r, the
     // stored fluxes of x-face / column r-1 one iteration ahead
-    const bool in_j = j >= -2 && j < np + 2;
-    const int jc = j < -2 ? -2 : (j > np + 1 ? np + 1 : j);
-    auto row_ok = [&](int i) { return i >= -2 && i < nx + 2 && in_j; };
+    const bool in_j = IN || (j >= -2 && j < np + 2);
+    const int jc = IN ? j : (j < -2 ? -2 : (j > np + 1 ? np + 1 : j));
+    auto row_ok = [&](int i) { return IN || (i >= -2 && i < nx + 2 && in_j); };
     double pfs_a = ldr(fin, pitch, nx, r0, jc), pf0_a = ldr(f0p, pitch, nx, r0, jc);
     double pfs_b = ldr(fin, pitch, nx, r0 + 1, jc), pf0_b = ldr(f0p, pitch, nx, r0 + 1, jc);
     constexpr int NH = STEP > 0 ? STEP : 1;  // stored flux planes read by this stage
     double hx[NH], hp[NH];
     bool hvx = false, hvp = false;  // the loaded face lies in the FxH / FpH ranges
     auto load_h = [&](int face) {
-        hvx = face >= 0 && face < nx + 1 && jr_ex;
-        hvp = face >= -1 && face < nx + 1 && jr_fp;
+        hvx = IN || (face >= 0 && face < nx + 1 && jr_ex);
+        hvp = IN || (face >= -1 && face < nx + 1 && jr_fp);
 #pragma unroll
         for (int k = 0; k < NH; k++) {
             hx[k] = (k < STEP) ? ldr(P.FxH[k], pitch, nx, face, jc) : 0.0;
@@ -218,9 +218,9 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
 
         // ---- x-face r-1: ex, fx (Rectangle.cpp:1278-1293), FxH_s (1313-1334)
         const int ix = r - 1;
-        const bool vFx = ix >= 0 && ix < nx + 1 && jr_ex;
+        const bool vFx = IN || (ix >= 0 && ix < nx + 1 && jr_ex);
         double Gd = shdn1(G_m1);  // Gamma at p-face j+1
-        double ex = (ix >= -1 && ix < nx + 2 && jr_ex) ? dpcc * (Gd - G_m1) : 0.0;
+        double ex = (IN || (ix >= -1 && ix < nx + 2 && jr_ex)) ? dpcc * (Gd - G_m1) : 0.0;
         double fx = 0.0;
         if (vFx) {
             if (MP) fx = weno_mp(fs_m3, fs_m2, fs_m1, fs_0, gs_m3, gs_m2, gs_m1, gs_0, eps32, ex > 0.0);
@@ -231,7 +231,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
         double FxH = vFx ? P.dx_inv * (fx * ex + w3 * (fx_u - fx_d) * (ex_u - ex_d)) : 0.0;
 
         // ---- column r: ep, fp (Rectangle.cpp:1294-1312)
-        const bool cr = r >= -1 && r < nx + 1;
+        const bool cr = IN || (r >= -1 && r < nx + 1);
         double qE = cr ? col_qE(A, A.E, r) : 0.0;
         ep_0 = (cr && jr_ep) ? qE - ccdx * (G_p1 - G_0) : 0.0;
         double s1 = shup1(fs_0), s2 = shup1(s1), s3 = shdn1(fs_0);
@@ -243,7 +243,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
             fp_0 = weno(s2, s1, fs_0, s3, ep_0 > 0.0);
         }
         // ---- column r-1: FpH_s (Rectangle.cpp:1353-1375)
-        const bool vFp = ix >= -1 && ix < nx + 1 && jr_fp;
+        const bool vFp = IN || (ix >= -1 && ix < nx + 1 && jr_fp);
         double FpH = vFp ? P.dp_inv * (fp_m1 * ep_m1 + w3 * (fp_0 - fp_m2) * (ep_0 - ep_m2)) : 0.0;
         if (store_flux) {
             const bool own_x = ix >= i0 && (ix < i0 + A.TI || last_strip);
@@ -257,7 +257,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
         // (Rectangle.cpp:1335-1352, 1376-1394 evaluated with the stage-0 fields)
         double G0d = shdn1(G0_0);
         double ex0 = dpcc * (G0d - G0_0);
-        FxL_0 = (r >= 0 && r < nx + 1 && jr_ex) ? P.dx_inv * ((ex0 > 0.0 ? f0_m1 : f0_0) * ex0) : 0.0;
+        FxL_0 = (IN || (r >= 0 && r < nx + 1 && jr_ex)) ? P.dx_inv * ((ex0 > 0.0 ? f0_m1 : f0_0) * ex0) : 0.0;
         double qE0 = cr ? col_qE(A, A.E0, r) : 0.0;
         double ep0 = qE0 - ccdx * (G0_p1 - G0_0);
         double f0_jm1 = shup1(f0_0);
@@ -268,15 +268,15 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
             const int i = r - 1;
             double FpL_up = shdn1(FpL_m1);  // FpL0 at (i, j+1)
             double t = 0.0;
-            if (i >= 0 && i < nx && j >= 0 && j < np) {
+            if (IN || (i >= 0 && i < nx && j >= 0 && j < np)) {
                 t = f0_m1;
-                const bool in0 = i >= xm && i < xp && j >= pm && j < pp;
+                const bool in0 = IN || (i >= xm && i < xp && j >= pm && j < pp);
                 if (in0) {
                     t += A.aSum * FxL_m1;
                     t += A.aSum * FpL_m1;
                 }
-                if (i >= xm && i < xp && j + 1 >= pm && j + 1 < pp) t -= A.aSum * FpL_up;
-                if (i + 1 >= xm && i + 1 < xp && j >= pm && j < pp) t -= A.aSum * FxL_0;
+                if (IN || (i >= xm && i < xp && j + 1 >= pm && j + 1 < pp)) t -= A.aSum * FpL_up;
+                if (IN || (i + 1 >= xm && i + 1 < xp && j >= pm && j < pp)) t -= A.aSum * FxL_0;
             }
             f2_m1 = t;  // ghost cells: zero fill of the walls (BoundaryCondition.cpp:6-8)
         }
@@ -297,7 +297,7 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
             double f0u = shdn1(f0_m2), f0d = shup1(f0_m2);
             double f2u = shdn1(f2_m2), f2d = shup1(f2_m2);
             double rp = 0.0, rm = 0.0;
-            if (i >= -1 && i < nx + 1 && jr_R) {
+            if (IN || (i >= -1 && i < nx + 1 && jr_R)) {
                 // fmax/fmin (one v_max_f64/v_min_f64 each) give the reference's
                 // a > b ? a : b bits here: f is never -0 (every update adds to a
                 // +0 or positive start), R and Q are never -0, nothing is NaN,
@@ -305,10 +305,10 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
                 // zero, where R is 0 either way
                 double Pp = fmax(0.0, FxD_m2) - fmin(0.0, FxD_m1) + fmax(0.0, FpD_m2) - fmin(0.0, FpDu);
                 double Pm = fmax(0.0, FxD_m1) - fmin(0.0, FxD_m2) + fmax(0.0, FpDu) - fmin(0.0, FpD_m2);
-                double wMax = fmax(fmax(f0_m2, f2_m2), fmax(fmax(f0_m1, f2_m1),
-                              fmax(fmax(f0_m3, f2_m3), fmax(fmax(f0u, f2u), fmax(f0d, f2d)))));
-                double wMin = fmin(fmin(f0_m2, f2_m2), fmin(fmin(f0_m1, f2_m1),
-                              fmin(fmin(f0_m3, f2_m3), fmin(fmin(f0u, f2u), fmin(f0d, f2d)))));
+                double wMax = hwmax(hwmax(f0_m2, f2_m2), hwmax(hwmax(f0_m1, f2_m1),
+                              hwmax(hwmax(f0_m3, f2_m3), hwmax(hwmax(f0u, f2u), hwmax(f0d, f2d)))));
+                double wMin = hwmin(hwmin(f0_m2, f2_m2), hwmin(hwmin(f0_m1, f2_m1),
+                              hwmin(hwmin(f0_m3, f2_m3), hwmin(hwmin(f0u, f2u), hwmin(f0d, f2d)))));
                 double Qm = -wMin + f2_m2;
                 double Qp = wMax - f2_m2;
                 if (MP) {
@@ -328,23 +328,23 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
         {
             const int i = r - 3;
             double Rpd = shup1(Rp_m3), Rmd = shup1(Rm_m3);  // R at (i, j-1)
-            const bool jint = j >= 0 && j < np;
-            double cxl = (i >= 1 && i < nx && jint)
-                             ? (FxD_m3 > 0.0 ? fmin(Rp_m3, Rm_m4) : fmin(Rp_m4, Rm_m3)) : 1.0;
-            double cxr = (i + 1 >= 1 && i + 1 < nx && jint)
-                             ? (FxD_m2 > 0.0 ? fmin(Rp_m2, Rm_m3) : fmin(Rp_m3, Rm_m2)) : 1.0;
-            double cpl = (i >= 1 && i < nx && jint)
-                             ? (FpD_m3 > 0.0 ? fmin(Rp_m3, Rmd) : fmin(Rpd, Rm_m3)) : 1.0;
+            const bool jint = IN || (j >= 0 && j < np);
+            double cxl = (IN || (i >= 1 && i < nx && jint))
+                             ? (FxD_m3 > 0.0 ? hwmin(Rp_m3, Rm_m4) : hwmin(Rp_m4, Rm_m3)) : 1.0;
+            double cxr = (IN || (i + 1 >= 1 && i + 1 < nx && jint))
+                             ? (FxD_m2 > 0.0 ? hwmin(Rp_m2, Rm_m3) : hwmin(Rp_m3, Rm_m2)) : 1.0;
+            double cpl = (IN || (i >= 1 && i < nx && jint))
+                             ? (FpD_m3 > 0.0 ? hwmin(Rp_m3, Rmd) : hwmin(Rpd, Rm_m3)) : 1.0;
             double cpu = shdn1(cpl);      // C at (i, j+1), computed by the next lane
             double FpDu = shdn1(FpD_m3);  // FpDS at (i, j+1)
             if (out_lane && i >= i0 && i < i1) {
                 double t = f2_m3;
-                if (i >= xm && i < xp && j >= pm && j < pp) {
+                if (IN || (i >= xm && i < xp && j >= pm && j < pp)) {
                     t += cxl * FxD_m3;
                     t += cpl * FpD_m3;
                 }
-                if (i >= xm && i < xp && j + 1 >= pm && j + 1 < pp) t -= cpu * FpDu;
-                if (i + 1 >= xm && i + 1 < xp && j >= pm && j < pp) t -= cxr * FxD_m2;
+                if (IN || (i >= xm && i < xp && j + 1 >= pm && j + 1 < pp)) t -= cpu * FpDu;
+                if (IN || (i + 1 >= xm && i + 1 < xp && j >= pm && j < pp)) t -= cxr * FxD_m2;
                 A.fout[pidx(pitch, i, j)] = t;
             }
         }
@@ -363,6 +363,28 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
     };
 
     for (int r = r0; r <= i1 + HALO - 1; r++) row(r);
+}
+
+// Whether a strip takes the IN form.  Lanes hold j in [jw - 3, jw + 60] and
+// the march rows are r in [i0 - 3, i1 + 2] with i1 = i0 + TI; the tightest
+// tests above are the updates of cells (r - 1, j) and (r - 3, j) with their
+// +1 neighbours inside [1, n - 1) (xm, pm <= 1, xp >= n_x, pp >= n_p), which
+// need 1 <= j <= n_p - 2 and 4 <= r <= n_x - 1, and the stored-flux faces
+// r0 - 1 >= 0.  The margins below are wider than that.
+__device__ __forceinline__ bool interior_strip(const StageArgs &A, int jw, int i0) {
+    return jw >= 8 && jw + 64 <= A.p.n_p && i0 >= 8 && i0 + A.TI + 8 <= A.p.n_x;
+}
+
+// One workgroup's tile (4 strips of one 64-lane p-column) of a stage.
+template <int STEP, bool MP>
+__device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
+    // the wavefront index is uniform: row loop, row ranges and field reads stay scalar
+    const int bx = tile % A.nbx, by = tile / A.nbx;
+    const int strip = by * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (strip >= A.nstrip) return;  // uniform per wavefront
+    const int jw = bx * OUTW, i0 = strip * A.TI;
+    if (interior_strip(A, jw, i0)) stage_strip<STEP, MP, true>(A, jw, i0);
+    else stage_strip<STEP, MP, false>(A, jw, i0);
 }
 
 template <int STEP, bool MP>
@@ -439,6 +461,34 @@ void tile_store(vx_ctx *ctx, int nx, int np, int ti, int mp, int step) {
     g_tiles[TileKey{ctx->device, nx, np, mp, step}] = ti;
 }
 
+// Workgroups of k_stage<step, mp> resident per compute unit (0 if unknown).
+int stage_blocks_per_cu(int step, bool mp) {
+    int n = 0;
+    hipError_t e = hipSuccess;
+#define VX_OCC(S, M) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_stage<S, M>, 256, 0)
+    if (mp) {
+        switch (step) {
+            case 0: VX_OCC(0, true); break;
+            case 1: VX_OCC(1, true); break;
+            case 2: VX_OCC(2, true); break;
+            case 3: VX_OCC(3, true); break;
+            case 4: VX_OCC(4, true); break;
+            default: VX_OCC(5, true); break;
+        }
+    } else {
+        switch (step) {
+            case 0: VX_OCC(0, false); break;
+            case 1: VX_OCC(1, false); break;
+            case 2: VX_OCC(2, false); break;
+            case 3: VX_OCC(3, false); break;
+            case 4: VX_OCC(4, false); break;
+            default: VX_OCC(5, false); break;
+        }
+    }
+#undef VX_OCC
+    return e == hipSuccess ? n : 0;
+}
+
 int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
     const int nx = A0.p.n_x, np = A0.p.n_p;
     for (auto &c : ctx->ti_cache)
@@ -470,11 +520,30 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
             break;
         }
     if (ctx->capturing) return guess;
+    // Besides the powers of two, the strip heights at which the launch's
+    // workgroups just fill k rounds of the device's resident slots (k = 1..4):
+    // the tallest strips (least warm-up) without a part-empty last round.
+    std::vector<int> cands(std::begin(cand), std::end(cand));
+    const int slots = ctx->n_cu * stage_blocks_per_cu(A0.step, mp);
+    if (slots > 0)
+        for (int k = 1; k <= 4; k++) {
+            const long wg_max = (long)k * slots;
+            // smallest strip height whose ceil(strips / 4) * nbx workgroups fit in wg_max
+            for (int ti = 1; ti <= 1024 && ti <= nx; ti++) {
+                const long strips = (nx + ti - 1) / ti;
+                if ((long)A0.nbx * ((strips + 3) / 4) <= wg_max) {
+                    if (ti > 2) cands.push_back(ti);
+                    break;
+                }
+            }
+        }
+    std::sort(cands.begin(), cands.end(), std::greater<int>());
+    cands.erase(std::unique(cands.begin(), cands.end()), cands.end());
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int best = guess;
     if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
         float best_ms = 1e30f;
-        for (int ti : cand) {
+        for (int ti : cands) {
             if (ti > 2 && ti >= 2 * nx) continue;  // a single strip either way
             StageArgs A = A0;
             A.TI = ti;
