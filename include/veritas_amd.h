@@ -296,7 +296,13 @@ enum {
      * and its antidiffusive flux is recombined; bit k of aux set: finer face k
      * is itself a coarse-fine face, whose stored low-order flux is used */
     VX_OP_FLUX_X = 7,
-    VX_OP_FLUX_P = 8
+    VX_OP_FLUX_P = 8,
+    /* f slot 1 of dst at interior cell (di, dj) = its corrected value
+     * (Rectangle.cpp:1596-1612) from f slot 2, C and the antidiffusive fluxes
+     * of dst: the correction of a cell next to a face whose C the limiter-C
+     * synchronisation wrote, after vx_fct_sub12_table (no counterpart in the
+     * reference, whose corrector runs after the synchronisation) */
+    VX_OP_CORRECT = 9
 };
 enum { VX_MODE_GHOST = 0, VX_MODE_TRANSFER = 1 };
 
@@ -360,6 +366,17 @@ int vx_fct_sub3_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
 int vx_fct_sub0_fluxes_table_mp(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const vx_fields *fl,
                                 double f_scale);
 int vx_fct_sub1_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_scale);
+/* The limiter and the corrector of every patch in one launch (sub-steps 1
+ * and 2, Rectangle.cpp:1536-1612; no counterpart in the reference, which runs
+ * them as two loops with the limiter-C synchronisation of Mesh::PushBoundaryC,
+ * Mesh.cpp:904-917, between them).  Writes R, C and f slot 1 of every interior
+ * cell from C before the synchronisation; the caller then runs a
+ * synchronisation program whose last batch recomputes, with VX_OP_CORRECT
+ * tasks, f slot 1 of every cell next to a face whose C the synchronisation
+ * writes.  The result equals vx_fct_sub1_table, the synchronisation, then
+ * vx_fct_sub2_table. */
+int vx_fct_sub12_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
+int vx_fct_sub12_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_scale);
 
 /* Split form of vx_fct_sub0 for hierarchies: the fluxes and RK combination
  * (Rectangle.cpp:1269-1517), then the low-order predictor (1518-1534). */

@@ -194,3 +194,23 @@ class g_as_oracle_layout:
 
     def patch(self, lvl, k, w):
         return self.o.patch(lvl, k, w) if w == "nested" else self.g.patch(lvl, k, w)
+
+
+@pytest.mark.parametrize("policy", ["8,4", "128,64", "-512,-1024"])
+def test_batched_strip_policies_change_no_bits(policy, monkeypatch):
+    """The strip policy of the batched sub-step-0 flux and limiter launches
+    (VX_AMR_POLICY pins it; the autotuner picks one per device and set of
+    patch shapes) changes no bit: three steps of the three-level case equal to
+    the oracle with short, tall and wavefront-count strips."""
+    monkeypatch.setenv("VX_AMR_POLICY", policy)
+    p = two_slab_laser()
+    say(f"case: three-level two-slab laser, strip policy {policy}, 3 steps vs oracle")
+    g, o = va.Run(p, fused=False), OracleRun(p)
+    dt = o.calc_dt(0.4)
+    for _ in range(3):
+        g.advance(dt)
+        o.advance(dt)
+    compare_mesh(g, o, f"policy {policy}")
+    compare_fields(g, o, f"policy {policy}")
+    g.close()
+    o.close()

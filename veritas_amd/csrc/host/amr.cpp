@@ -18,6 +18,8 @@
 #include <cstdlib>
 #include <iterator>
 #include <limits>
+#include <set>
+#include <tuple>
 #include <unordered_set>
 
 #include "veritas.hpp"
@@ -933,7 +935,8 @@ void Mesh::FreePrograms() {
     for (auto &l : levels) l->FreePrograms();
     vx_program_destroy(push_prog);
     vx_program_destroy(pushc_prog);
-    push_prog = pushc_prog = nullptr;
+    vx_program_destroy(pushcc_prog);
+    push_prog = pushc_prog = pushcc_prog = nullptr;
     vx_patch_table_destroy(table);
     table = nullptr;
 }
@@ -979,6 +982,34 @@ void Mesh::BuildPrograms() {
         for (int type = 4; type <= 6; type++)
             for (auto &l : levels) emit_level(C, *l, type, r);
         pushc_prog = C.upload(*dev);
+        // the same, then the correction of every interior cell whose
+        // corrected value reads a C that the synchronisation writes: Cx at
+        // (i, j) is read by cells (i, j) and (i - 1, j), Cp at (i, j) by (i, j)
+        // and (i, j - 1) (vxi::corrected_cell); in patch, row and column order
+        std::set<std::tuple<int, int, int>> fix;
+        auto face = [&](int p, bool x, int i, int j) {
+            const Rectangle &R = *order.at(p);
+            for (int k = 0; k < 2; k++) {
+                const int ci = x ? i - k : i, cj = x ? j : j - k;
+                if (ci >= 0 && ci < R.n_x && cj >= 0 && cj < R.n_p) fix.insert({p, ci, cj});
+            }
+        };
+        for (const vx_cell_task &t : C.tasks) {
+            const bool x = t.aux < 2;
+            if (t.op == VX_OP_C_SAME || t.op == VX_OP_C_UPDATE) {
+                face(t.dst, x, t.di, t.dj);
+                face(t.src, x, t.si, t.sj);
+            } else if (t.op == VX_OP_C_DIFF) {
+                face(t.dst, x, t.di, t.dj);
+                for (int k = 0; k < r; k++) face(t.src, x, x ? t.si : t.si + k, x ? t.sj + k : t.sj);
+            }
+        }
+        if (!fix.empty()) {
+            C.cut();
+            for (const auto &c : fix)
+                C.push(mk(VX_OP_CORRECT, std::get<0>(c), std::get<1>(c), std::get<2>(c), 0, 0, 0, 0));
+        }
+        pushcc_prog = C.upload(*dev);
     }
     for (auto &lp : levels) {
         Level &L = *lp;

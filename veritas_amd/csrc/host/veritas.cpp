@@ -560,10 +560,13 @@ void Mesh::Advance(double timeStep, int step) {
     Sub0(timeStep, step);
     PushData(2);
     EnsureTable();
-    if (mixed) dev->check(vx_fct_sub1_table_mp(dev->ctx, table, n_f, FScale()), "fct sub1 (mixed)");
-    else dev->check(vx_fct_sub1_table(dev->ctx, table, n_f), "fct sub1");
-    PushBoundaryC();
-    dev->check(vx_fct_sub2_table(dev->ctx, table, n_f), "fct sub2");
+    // sub-steps 1 and 2 in one launch, then the limiter-C synchronisation
+    // with the cells next to the faces it changes corrected again
+    if (mixed) dev->check(vx_fct_sub12_table_mp(dev->ctx, table, n_f, FScale()), "fct sub1+2 (mixed)");
+    else dev->check(vx_fct_sub12_table(dev->ctx, table, n_f), "fct sub1+2");
+    if (pushcc_prog)
+        dev->check(vx_program_run(dev->ctx, table, pushcc_prog, VX_MODE_GHOST, 1, settings.refinementRatio),
+                   "limiter sync");
     PushData(1);
     if (step == 5) {
         EnsureTable();

@@ -282,6 +282,9 @@ private:
     // PushData's and PushBoundaryC's sequences of level programs, each merged
     // into one program (batches span levels and types where no cell conflicts)
     vx_program *push_prog = nullptr, *pushc_prog = nullptr;
+    // pushc_prog followed by one batch of VX_OP_CORRECT tasks for every cell
+    // next to a face whose C it writes (after vx_fct_sub12_table)
+    vx_program *pushcc_prog = nullptr;
     void promoteHierarchyToMesh(bool init);      // Mesh.cpp:794-875
     void InterMeshDataTransfer(std::vector<std::unique_ptr<Level>> &levels_n);  // Mesh.cpp:116-130
     void BuildPrograms();

@@ -238,6 +238,11 @@ __device__ __forceinline__ void program_task(const ProgArgs &A, const double *E,
         flux_task(A, E, T);
         return;
     }
+    if (T.op == VX_OP_CORRECT) {
+        const vx_patch &D = A.pt[T.dst];
+        at(D.f[1], D.pitch, T.di, T.dj) = corrected_cell(D, T.di, T.dj);
+        return;
+    }
     const vx_patch &D = A.pt[T.dst];
     const int rs = A.mode == VX_MODE_TRANSFER ? 1 : A.slot;
     double v = 0.0;
@@ -399,7 +404,7 @@ int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r
     for (size_t t = 0; t < prog->host.size(); t++) {
         const vx_cell_task &T = prog->host[t];
         bool ok = T.dst >= 0 && T.dst < n;
-        const bool uses_src = T.op != VX_OP_ZERO;
+        const bool uses_src = T.op != VX_OP_ZERO && T.op != VX_OP_CORRECT;
         if (ok && uses_src) ok = T.src >= 0 && T.src < n;
         if (ok) {
             const vx_patch &D = tb->host[T.dst];
@@ -430,6 +435,7 @@ int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r
                 case VX_OP_FLUX_X: case VX_OP_FLUX_P:
                     ok = inr(D, di, dj) && inr(S, si - 1, sj - 1) && inr(S, si + r - 1, sj + r - 1);
                     break;
+                case VX_OP_CORRECT: ok = di >= 0 && di < D.n_x && dj >= 0 && dj < D.n_p; break;
                 default: ok = false;
             }
         }

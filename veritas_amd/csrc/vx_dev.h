@@ -16,6 +16,30 @@ __device__ __forceinline__ double valmin(double a, double b) { return a < b ? a 
 // std::min / std::max as used by the limiter-C sync (Rectangle.cpp:1646-1829)
 __device__ __forceinline__ double stdmin(double a, double b) { return (b < a) ? b : a; }
 
+// XCD-aware workgroup order: the dispatcher hands workgroup w of a launch to
+// XCD w mod 8, so map w to a tile index that gives each XCD one contiguous
+// run of tiles.  Neighbouring tiles (which read each other's halo rows and
+// columns) then run on the same XCD and share those cache lines in its L2.
+// A bijection of [0, nwg): XCD x takes q + (x < rem) workgroups and the run
+// [x q + min(x, rem), ...) of that length.
+__device__ __forceinline__ int xcd_order(int w, int nwg) {
+    const int xcd = w & 7, q = nwg >> 3, rem = nwg & 7;
+    return xcd * q + min(xcd, rem) + (w >> 3);
+}
+
+// The same in chunks of K consecutive tiles, dealt round-robin to the XCDs
+// (chunk c to XCD c mod 8): neighbours within a chunk share one L2, and
+// launches whose tiles differ in cost (several patches of different strip
+// shapes) stay balanced over the XCDs.  Workgroups past the last whole round
+// of 8 K keep their own index.  A bijection of [0, nwg).
+template <int K>
+__device__ __forceinline__ int xcd_chunk_order(int w, int nwg) {
+    const int full = nwg - nwg % (8 * K);
+    if (w >= full) return w;
+    const int x = w & 7, k = w >> 3;
+    return (x + 8 * (k / K)) * K + k % K;
+}
+
 // v_max_f64 / v_min_f64 as single instructions.  fmax/fmin compile to the
 // same instruction, but first quiet possible signalling NaNs with a
 // v_max_f64 x, x, x of every operand the compiler cannot prove canonical

@@ -9,6 +9,7 @@
 // Rectangle.cpp:10-38); the kernels reproduce that by computing each quantity
 // only inside its range and using 0 outside.
 #include <cmath>
+#include <cstdlib>
 #include <initializer_list>
 #include <algorithm>
 #include <vector>
@@ -198,7 +199,7 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
 
 template <int STEP>
 __global__ __launch_bounds__(256) void k_sub0s(vx_patch P, Sub0Coef A, Sub0Tiles T) {
-    sub0s_tile<STEP, false>(P, A, T, (int)blockIdx.x);
+    sub0s_tile<STEP, false>(P, A, T, xcd_order((int)blockIdx.x, (int)gridDim.x));
 }
 
 // The first patch k of the work list whose workgroups start at or before w
@@ -217,7 +218,7 @@ __device__ __forceinline__ int batch_patch(const vx_batch_info *bi, int n, int w
 template <int STEP, bool MP>
 __global__ __launch_bounds__(256) void k_sub0s_batch(const vx_patch *tab, const vx_batch_info *bi, int n,
                                                      Sub0Coef A) {
-    const int w = (int)blockIdx.x;
+    const int w = xcd_chunk_order<8>((int)blockIdx.x, (int)gridDim.x);
     const int k = batch_patch<0>(bi, n, w);
     const Sub0Tiles T{bi[k].TI, bi[k].nstrip, bi[k].nbx};
     sub0s_tile<STEP, MP>(tab[k], A, T, w - bi[k].start[0]);
@@ -271,13 +272,19 @@ __device__ __forceinline__ void lo_predictor(const vx_patch &P, double aSum, int
 // where it is formed inside the strip (the cross-patch C sync reads it); the
 // first row of a strip (i0 - 1) only warms the pipeline.  MP: the ratios Q / P
 // in FP32 on the values scaled by sinv (vxd::zalesak_mp).
+// COR: the corrector too (Rectangle.cpp:1596-1612, vxi::corrected_cell with
+// C, f^td and the fluxes from registers): row i - 1 is corrected at step i,
+// once C of row i is formed, so a strip takes one more step (row i1, whose R
+// and C are formed but not stored).  The values use C before the cross-patch
+// C synchronisation; cells next to a face that the synchronisation changes
+// are corrected again after it (VX_OP_CORRECT).
 constexpr int LIMW = 48;  // positions per limiter wavefront (of 62 it could form)
 
 struct LimTiles {
     int TI, nstrip, nbx;
 };
 
-template <bool MP>
+template <bool MP, bool COR>
 __device__ __forceinline__ void limiter_stream(const vx_patch &P, const LimTiles &T, int tile, double sinv) {
     const int nx = P.n_x, np = P.n_p, pitch = P.pitch;
     const int lane = threadIdx.x & 63;
@@ -304,7 +311,11 @@ __device__ __forceinline__ void limiter_stream(const vx_patch &P, const LimTiles
     double a0_p = ld(f0, r0 + 1), a2_p = ld(f2, r0 + 1), x_p = ld(FxDS, r0 + 1), p_p = ld(FpDS, r0 + 1);
     double a0_q = ld(f0, r0 + 2), a2_q = ld(f2, r0 + 2), x_q = ld(FxDS, r0 + 2), p_q = ld(FpDS, r0 + 2);
     double Rp_m = 0.0, Rm_m = 0.0;  // R of row i - 1
-    for (int i = r0; i < i1; i++) {
+    // COR: C, FxDS and FpDS of row i - 1
+    double cx_m = 1.0, cp_m = 1.0, x_m = 0.0, p_m = 0.0;
+    double *const f1 = P.f[1];
+    const int iend = COR ? i1 + 1 : i1;
+    for (int i = r0; i < iend; i++) {
         const double na0 = ld(f0, i + 3), na2 = ld(f2, i + 3), nx_ = ld(FxDS, i + 3), np_ = ld(FpDS, i + 3);
         // R+- of (i, j) (Rectangle.cpp:1538-1579)
         const double p_u = shdn1(p_c);  // FpDS(i, j+1)
@@ -330,9 +341,10 @@ __device__ __forceinline__ void limiter_stream(const vx_patch &P, const LimTiles
                 }
             }
         }
+        double cx = 1.0, cp = 1.0;
         if (i >= i0) {
             const long idx = pidx(pitch, i, j);
-            if (own && i >= -1 && i < nx + 1 && r_j) {
+            if (own && i < i1 && i >= -1 && i < nx + 1 && r_j) {
                 // only the cross-patch C sync reads R again (at patch edges):
                 // streamed past the caches, which keep C and the fluxes that
                 // the corrector reads next
@@ -341,15 +353,33 @@ __device__ __forceinline__ void limiter_stream(const vx_patch &P, const LimTiles
             }
             // C of (i, j): R(i, j), R(i-1, j), R(i, j-1) (Rectangle.cpp:1581-1594)
             const double Rp_l = shup1(rp), Rm_l = shup1(rm);
-            double cx = 1.0, cp = 1.0;
             if (i >= 1 && i < nx && c_j) {
                 cx = x_c > 0.0 ? valmin(rp, Rm_m) : valmin(Rp_m, rm);
                 cp = p_c > 0.0 ? valmin(rp, Rm_l) : valmin(Rp_l, rm);
             }
-            if (own) {
+            if (own && i < i1) {
                 P.Cx[idx] = cx;
                 P.Cp[idx] = cp;
             }
+        }
+        if (COR) {
+            // cell (c, j) = (i - 1, j): the terms of corrected_cell in its order
+            const int c = i - 1;
+            const double cp_u = shdn1(cp_m), fp_u = shdn1(p_m);  // Cp, FpDS at (c, j + 1)
+            if (own && c >= i0 && c >= 0 && c < nx && j >= 0 && j < np) {
+                double t = a2_m;
+                if (in_update_loop(P, c, j)) {
+                    t += cx_m * x_m;
+                    t += cp_m * p_m;
+                }
+                if (in_update_loop(P, c, j + 1)) t -= cp_u * fp_u;
+                if (in_update_loop(P, c + 1, j)) t -= cx * x_c;
+                f1[pidx(pitch, c, j)] = t;
+            }
+            cx_m = cx;
+            cp_m = cp;
+            x_m = x_c;
+            p_m = p_c;
         }
         Rp_m = rp;
         Rm_m = rm;
@@ -361,35 +391,19 @@ __device__ __forceinline__ void limiter_stream(const vx_patch &P, const LimTiles
 }
 
 // f1 = f2 + limited antidiffusive fluxes on interior cells
-// (Rectangle.cpp:1596-1612), same gather order as the predictor.
+// (Rectangle.cpp:1596-1612, vxi::corrected_cell), one thread per cell.
 __device__ __forceinline__ void corrector(const vx_patch &P, int bx, int by) {
     int j = bx * 64 + (threadIdx.x & 63);
     int i = by * 4 + (threadIdx.x >> 6);
     if (i >= P.n_x || j >= P.n_p) return;
-    LoopRange L = loop_range(P);
-    const int pitch = P.pitch;
-    long idx = pidx(pitch, i, j);
-    double t = P.f[2][idx];
-    if (L.in(i, j)) {
-        t += P.Cx[idx] * P.FxDS[idx];
-        t += P.Cp[idx] * P.FpDS[idx];
-    }
-    if (L.in(i, j + 1)) {
-        long k = pidx(pitch, i, j + 1);
-        t -= P.Cp[k] * P.FpDS[k];
-    }
-    if (L.in(i + 1, j)) {
-        long k = pidx(pitch, i + 1, j);
-        t -= P.Cx[k] * P.FxDS[k];
-    }
-    P.f[1][idx] = t;
+    P.f[1][pidx(P.pitch, i, j)] = corrected_cell(P, i, j);
 }
 
 __global__ __launch_bounds__(256) void k_lo_predictor(vx_patch P, double aSum) {
     lo_predictor(P, aSum, (int)blockIdx.x, (int)blockIdx.y);
 }
 __global__ __launch_bounds__(256) void k_limiter(vx_patch P, LimTiles T) {
-    limiter_stream<false>(P, T, (int)blockIdx.x, 1.0);
+    limiter_stream<false, false>(P, T, xcd_order((int)blockIdx.x, (int)gridDim.x), 1.0);
 }
 __global__ __launch_bounds__(256) void k_corrector(vx_patch P) { corrector(P, (int)blockIdx.x, (int)blockIdx.y); }
 
@@ -398,7 +412,7 @@ __global__ __launch_bounds__(256) void k_corrector(vx_patch P) { corrector(P, (i
 template <int KIND, int EXTRA>
 __device__ __forceinline__ void batch_cell(const vx_batch_info *bi, int n, int &k, int &bx, int &by,
                                            const vx_patch *tab) {
-    const int w = (int)blockIdx.x;
+    const int w = xcd_order((int)blockIdx.x, (int)gridDim.x);
     k = batch_patch<KIND>(bi, n, w);
     const int local = w - bi[k].start[KIND];
     const int gx = (tab[k].n_p + EXTRA + 63) / 64;
@@ -412,13 +426,13 @@ __global__ __launch_bounds__(256) void k_lo_predictor_batch(const vx_patch *tab,
     batch_cell<1, 0>(bi, n, k, bx, by, tab);
     lo_predictor(tab[k], aSum, bx, by);
 }
-template <bool MP>
+template <bool MP, bool COR>
 __global__ __launch_bounds__(256) void k_limiter_batch(const vx_patch *tab, const vx_batch_info *bi, int n,
                                                        double sinv) {
-    const int w = (int)blockIdx.x;
+    const int w = xcd_chunk_order<8>((int)blockIdx.x, (int)gridDim.x);
     const int k = batch_patch<2>(bi, n, w);
     const LimTiles T{bi[k].lTI, bi[k].lnstrip, bi[k].lnbx};
-    limiter_stream<MP>(tab[k], T, w - bi[k].start[2], sinv);
+    limiter_stream<MP, COR>(tab[k], T, w - bi[k].start[2], sinv);
 }
 __global__ __launch_bounds__(256) void k_corrector_batch(const vx_patch *tab, const vx_batch_info *bi, int n) {
     int k, bx, by;
@@ -569,6 +583,22 @@ std::map<PolKey, int> g_pol;
 template <class Launch>
 int tune_policy(vx_ctx *ctx, vx_patch_table *t, int kind, bool mp, const std::vector<int> &cands, Launch launch) {
     int &pol = kind == 0 ? t->sub0_pol : t->lim_pol;
+    // VX_AMR_POLICY="sub0,limiter" pins both policies (a strip height, or
+    // minus the wavefront count to reach), e.g. to profile a run without the
+    // timing launches or to check that the policy changes no result
+    if (const char *env = std::getenv("VX_AMR_POLICY")) {
+        int v[2] = {0, 0}, n = 0;
+        for (const char *q = env; *q && n < 2;) {
+            v[n++] = std::atoi(q);
+            while (*q && *q != ',') q++;
+            if (*q == ',') q++;
+        }
+        if (n == 2 && v[kind] != 0 && v[kind] <= 1024 && v[kind] >= -(1 << 24)) {
+            if (pol == v[kind]) return VX_OK;
+            pol = v[kind];
+            return table_rebuild(ctx, t);
+        }
+    }
     PolKey key{ctx->device, kind, (int)mp, {}};
     for (const vx_patch &p : t->host) {
         key.shapes.push_back(p.n_x);
@@ -778,12 +808,17 @@ int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double
     return check_launch(ctx, "k_lo_predictor_batch");
 }
 
-static int sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, bool mp, double sinv) {
+// cor: the limiter launch also corrects every interior cell (k_limiter_batch<MP, true>)
+static int sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, bool mp, double sinv, bool cor) {
     int rc = table_batch(ctx, t, n_f);
     if (rc) return rc;
     auto launch = [&] {
-        if (mp) k_limiter_batch<true><<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
-        else k_limiter_batch<false><<<(unsigned)t->batch[t->n].start[2], 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
+        // the workgroup count follows the strip policy, which tuning changes
+        const unsigned nwg = (unsigned)t->batch[t->n].start[2];
+        if (mp && cor) k_limiter_batch<true, true><<<nwg, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
+        else if (mp) k_limiter_batch<true, false><<<nwg, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
+        else if (cor) k_limiter_batch<false, true><<<nwg, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
+        else k_limiter_batch<false, false><<<nwg, 256, 0, ctx->stream>>>(t->d, t->batch_d, t->n, sinv);
     };
     if (!t->lim_tuned[mp] && !ctx->capturing) {
         if ((rc = tune_policy(ctx, t, 1, mp, {-4096, 64, 32, 16, 8, 4}, launch))) return rc;
@@ -793,14 +828,28 @@ static int sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, bool mp, doub
     return check_launch(ctx, "k_limiter_batch");
 }
 
-int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) { return sub1_table(ctx, t, n_f, false, 1.0); }
+int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
+    return sub1_table(ctx, t, n_f, false, 1.0, false);
+}
 
 int vx_fct_sub1_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_scale) {
     double sinv;
     float eps32;
     int rc = mp_scale(ctx, f_scale, sinv, eps32, "sub1 table (mixed)");
     if (rc) return rc;
-    return sub1_table(ctx, t, n_f, true, sinv);
+    return sub1_table(ctx, t, n_f, true, sinv, false);
+}
+
+int vx_fct_sub12_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {
+    return sub1_table(ctx, t, n_f, false, 1.0, true);
+}
+
+int vx_fct_sub12_table_mp(vx_ctx *ctx, vx_patch_table *t, int32_t n_f, double f_scale) {
+    double sinv;
+    float eps32;
+    int rc = mp_scale(ctx, f_scale, sinv, eps32, "sub12 table (mixed)");
+    if (rc) return rc;
+    return sub1_table(ctx, t, n_f, true, sinv, true);
 }
 
 int vx_fct_sub2_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f) {

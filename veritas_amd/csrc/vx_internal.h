@@ -139,6 +139,39 @@ __host__ __device__ inline long pidx(int pitch, int i, int j) {
     return (long)(i + 2) * pitch + (j + 2);
 }
 
+// Whether iteration (i, j) of the reference's FCT update loops, which run over
+// [xm, xp) x [pm, pp) with xm = 1 at a left wall (else 0), xp = n_x at a right
+// wall (else n_x + 1), and likewise in p, touches the patch (Rectangle.cpp:
+// 1518-1534, 1596-1612).
+__host__ __device__ inline bool in_update_loop(const vx_patch &P, int i, int j) {
+    return i >= (P.left ? 1 : 0) && i < (P.right ? P.n_x : P.n_x + 1) && j >= (P.down ? 1 : 0) &&
+           j < (P.up ? P.n_p : P.n_p + 1);
+}
+
+// Corrected value of interior cell (i, j) (Rectangle.cpp:1596-1612) in gather
+// form: f^td plus the limited antidiffusive fluxes in the order the serial
+// scatter loop applies them -- iteration (i, j) adds Cx FxDS and Cp FpDS at
+// (i, j), iteration (i, j+1) subtracts Cp FpDS at (i, j+1), iteration (i+1, j)
+// subtracts Cx FxDS at (i+1, j), each only inside the loop ranges.
+__device__ inline double corrected_cell(const vx_patch &P, int i, int j) {
+    const int pitch = P.pitch;
+    const long idx = pidx(pitch, i, j);
+    double t = P.f[2][idx];
+    if (in_update_loop(P, i, j)) {
+        t += P.Cx[idx] * P.FxDS[idx];
+        t += P.Cp[idx] * P.FpDS[idx];
+    }
+    if (in_update_loop(P, i, j + 1)) {
+        const long k = pidx(pitch, i, j + 1);
+        t -= P.Cp[k] * P.FpDS[k];
+    }
+    if (in_update_loop(P, i + 1, j)) {
+        const long k = pidx(pitch, i + 1, j);
+        t -= P.Cx[k] * P.FxDS[k];
+    }
+    return t;
+}
+
 inline int validate_patch(vx_ctx *ctx, const vx_patch *p) {
     if (!p) return fail(ctx, VX_EINVAL, "null patch");
     if (p->n_x < 1 || p->n_p < 1 || p->pitch < p->n_p + 4 || (p->pitch % 8) != 0)

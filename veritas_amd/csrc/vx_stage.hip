@@ -94,15 +94,9 @@ __device__ __forceinline__ double col_qE(const StageArgs &A, const double *E, in
     return A.p.q * patch_E(E, A.n_f, A.p.rtb, A.p.x_pos, i);
 }
 
-// XCD-aware tile order: the dispatcher hands workgroup w to XCD w mod 8, so
-// renumber workgroups so that each XCD receives a contiguous run of tiles;
-// p-neighbouring tiles then march over the same rows on the same XCD and
-// share the cache lines of their overlapping halos in that XCD's L2.
-__device__ __forceinline__ int xcd_tile() {
-    const int nwg = gridDim.x, w = blockIdx.x;
-    const int xcd = w & 7, q = nwg >> 3, rem = nwg & 7;
-    return xcd * q + min(xcd, rem) + (w >> 3);
-}
+// XCD-aware tile order (vxd::xcd_order): p-neighbouring tiles march over the
+// same rows on the same XCD and share their overlapping halos in its L2.
+__device__ __forceinline__ int xcd_tile() { return xcd_order((int)blockIdx.x, (int)gridDim.x); }
 
 // A strip of one wavefront: TI output rows [i0, i1) of the 58 p-cells from jw.
 // IN (interior): the strip and its p-column lie so far from the patch edges
