@@ -236,7 +236,12 @@ def bench_amr(a, va):
     params = va.laser()
     t0 = time.perf_counter()
     mixed = a.precision == "mixed"
-    run = va.Run(params, fused=False, graphs=False, mixed=mixed)  # eager launches: per-launch events
+    # --mode plain: every sub-step launched patch by patch (FP64), the plain
+    # per-patch form the batched launches are measured against
+    batched = a.mode != "plain"
+    if not batched and mixed:
+        raise SystemExit("bench: the per-patch form (--mode plain) is FP64 only")
+    run = va.Run(params, fused=False, graphs=False, mixed=mixed, batched=batched)  # eager: per-launch events
     setup = time.perf_counter() - t0
     progress(f"laser AMR set up in {setup:.1f} s")
     lay = run.layout()
@@ -276,7 +281,8 @@ def bench_amr(a, va):
         "dtype": "f64+f32" if mixed else "f64",
         "data": "synthetic (laser pulse into a ramped plasma slab)",
         "config": {"workload": "relativistic laser-plasma, base 8192x512, 3 levels at ratio 2, "
-                               + ("mixed precision" if mixed else "FP64") + ", dt=0.4xCFL, fixed hierarchy",
+                               + ("mixed precision" if mixed else "FP64") + ", dt=0.4xCFL, fixed hierarchy"
+                               + ("" if batched else ", per-patch launches"),
                    "patches_per_level": [int((lay[:, 0] == l).sum()) for l in range(int(lay[:, 0].max()) + 1)],
                    "cells_all_levels": cells, "setup_s": setup, "graph_ms_per_step": graph_ms,
                    "regrid_ms": sorted(regrid_ms)[1], "regrids_that_rebuilt": f"{rebuilds} of {regrids}",

@@ -214,3 +214,20 @@ def test_batched_strip_policies_change_no_bits(policy, monkeypatch):
     compare_fields(g, o, f"policy {policy}")
     g.close()
     o.close()
+
+
+def test_per_patch_launches_equal_the_batched_launches():
+    """The plain per-patch form (every sub-step launched patch by patch) and
+    the batched form (one launch per sub-step for every patch) give the same
+    bits on the three-level case, three steps from the same state."""
+    p = two_slab_laser()
+    say("case: three-level two-slab laser, per-patch vs batched launches, 3 steps")
+    a, b = va.Run(p, fused=False), va.Run(p, fused=False, batched=False)
+    dt = a.calc_dt(0.4)
+    for _ in range(3):
+        a.advance(dt)
+        b.advance(dt)
+    compare_mesh(a, b, "per-patch vs batched")
+    compare_fields(a, b, "per-patch vs batched")
+    a.close()
+    b.close()

@@ -66,6 +66,7 @@ def host_lib():
         L.vxh_destroy.argtypes = [vp]
         L.vxh_calc_dt.argtypes = [vp, d, ctypes.POINTER(d)]
         L.vxh_set_fused.argtypes = [vp, i]
+        L.vxh_set_batched.argtypes = [vp, i]
         L.vxh_set_graphs.argtypes = [vp, i]
         L.vxh_set_mixed.argtypes = [vp, i]
         L.vxh_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
@@ -144,13 +145,15 @@ class Run:
     scenario (`Run(params)`) or for the reference's inputs and user hooks
     (`Run.from_desc(desc, hooks)`)."""
 
-    def __init__(self, params, device=0, fused=True, graphs=True, mixed=False, _handle=None):
+    def __init__(self, params, device=0, fused=True, graphs=True, mixed=False, batched=True, _handle=None):
         L = host_lib()
         self.params = params
         self.h = _handle if _handle is not None else L.vxh_create(ctypes.byref(params), device)
         if not self.h:
             raise VeritasError(L.vxh_create_error().decode())
         self.set_fused(fused)
+        if not batched:
+            self.set_batched(False)
         self.set_graphs(graphs)
         if mixed:
             self.set_mixed(True)
@@ -197,6 +200,11 @@ class Run:
 
     def set_fused(self, on):
         self._ck(host_lib().vxh_set_fused(self.h, int(bool(on))))
+
+    def set_batched(self, on):
+        """False: hierarchies launch every sub-step patch by patch in FP64 (the
+        plain per-patch form) instead of one launch per patch table."""
+        self._ck(host_lib().vxh_set_batched(self.h, int(bool(on))))
 
     def set_mixed(self, on):
         """Mixed-precision form (SolverManager::SetMixed): FP32 WENO weights and

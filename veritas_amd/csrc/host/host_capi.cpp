@@ -124,6 +124,10 @@ int vxh_calc_dt(void *h, double cfl, double *out) {
     return guarded(h, [&](Run *r) { *out = r->sm->CalculateDt(cfl); });
 }
 
+int vxh_set_batched(void *h, int on) {
+    return guarded(h, [&](Run *r) { r->sm->SetBatched(on != 0); });
+}
+
 int vxh_set_fused(void *h, int on) {
     return guarded(h, [&](Run *r) { r->sm->SetFused(on != 0); });
 }

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <iomanip>
 #include <sstream>
 
@@ -554,9 +555,30 @@ void Mesh::Advance(double timeStep, int step) {
         single->FCTStageFused(timeStep, step, mixed);
         return;
     }
+    const int n_f = EMSolver->fields().n_f;
+    if (!batched) {
+        // the plain per-patch form: FP64 sub-step launches one patch at a time,
+        // the same task programs between them
+        const vx_fields &fl = EMSolver->fields();
+        auto each = [&](const std::function<int(const vx_patch *)> &f, const char *what) {
+            for (auto &l : levels)
+                for (auto &r : l->rectangles) dev->check(f(&r->desc()), what);
+        };
+        for (auto &l : levels)
+            for (auto &r : l->rectangles) r->UnaliasSlots();
+        each([&](const vx_patch *p) { return vx_fct_sub0_fluxes(dev->ctx, p, step, timeStep, &fl); }, "sub0 fluxes");
+        for (unsigned i = 1; i < levels.size(); i++) levels[i]->RunFluxProgram(step, timeStep, fl);
+        each([&](const vx_patch *p) { return vx_fct_sub0_predictor(dev->ctx, p, step, timeStep); }, "predictor");
+        PushData(2);
+        each([&](const vx_patch *p) { return vx_fct_sub1(dev->ctx, p); }, "sub1");
+        PushBoundaryC();
+        each([&](const vx_patch *p) { return vx_fct_sub2(dev->ctx, p); }, "sub2");
+        PushData(1);
+        if (step == 5) each([&](const vx_patch *p) { return vx_fct_sub3(dev->ctx, p); }, "sub3");
+        return;
+    }
     // each sub-step is one launch over the mesh's patch table (Level::FCTTimeStep
     // over every level, Mesh.cpp:64-89; the patches of a sub-step are independent)
-    const int n_f = EMSolver->fields().n_f;
     Sub0(timeStep, step);
     PushData(2);
     EnsureTable();
@@ -854,6 +876,11 @@ void SolverManager::StageFields(double dt, int step) {
 void SolverManager::SetFused(bool on) {
     DropGraphs();
     for (auto &m : meshes) m->fused = on;
+}
+
+void SolverManager::SetBatched(bool on) {
+    DropGraphs();
+    for (auto &m : meshes) m->batched = on;
 }
 
 void SolverManager::SetMixed(bool on) {

@@ -264,6 +264,9 @@ public:
     void Sub0(double timeStep, int step);  // sub-step 0 of every patch with the coarse-fine fluxes
     int regrids = 0, rebuilds = 0;  // updateHierarchy calls after set-up, and those that changed the patches
     bool fused = true;  // single walled patch: one fused call per stage
+    // false: every sub-step launched patch by patch (the plain per-patch form of
+    // Level::FCTTimeStep, Level.cpp:12-17) instead of one launch per table
+    bool batched = true;
     // mixed precision (FP32 WENO weights and Zalesak ratios): the fused stage of
     // a single walled patch, else the batched flux and limiter launches
     bool mixed = false;
@@ -364,6 +367,7 @@ public:
     void StageVlasovSub(double dt, int step, int sub);
     void StageFields(double dt, int step);
     void SetFused(bool on);
+    void SetBatched(bool on);
     // One step of Advance in pieces, for a caller that interleaves several runs
     // (the serving loop, host/serve.cpp): per stage StageBegin, then either the
     // batched item (FusedItem true) or StageVlasov, then StageEnd.
