@@ -114,8 +114,11 @@ int vx_fct_stage_walls(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
  * fluxes from f[0] and the stage-0 fields, stores FxH/FpH of this stage, and
  * writes f_{s+1} into `fout`.  `fin` is f_s (p->f[1]); `fout` must alias
  * neither `fin` nor p->f[0]; ghost rings of all three planes must be zero.
- * Bit-identical to the sub-step kernels.  Call vx_fct_stage_fused_available()
- * to check that the library provides it. */
+ * At step 4 the FxH/FpH planes of step 4 receive step 5's partial RK sum
+ * (its coefficients times the same dt, over the fluxes of steps 0-4) instead
+ * of the step-4 fluxes, and step 5 reads that sum: run steps 4 and 5 of a
+ * time step both fused, with one dt.  Bit-identical to the sub-step kernels.
+ * Call vx_fct_stage_fused_available() to check that the library provides it. */
 int vx_fct_stage_fused(vx_ctx *ctx, const vx_patch *p, int step, double dt, const vx_fields *fl,
                        const double *fin, double *fout);
 /* Mixed-precision form of vx_fct_stage_fused (no counterpart in the

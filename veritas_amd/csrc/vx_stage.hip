@@ -21,7 +21,12 @@
 //        cell r-3   : limiter C on its faces and the corrected value f_{s+1}
 //  * Store-flux design: FxH_s/FpH_s are written once (owned faces) for the
 //    later stages' RK combinations instead of recomputing the WENO fluxes of
-//    every earlier stage from stored stage values.
+//    every earlier stage from stored stage values.  Stage 5 needs the earlier
+//    fluxes only through its own combination, so stage 4 writes, in place of
+//    FxH_4/FpH_4, stage 5's partial sum b_0 F_0 + ... + b_4 F_4 (left to
+//    right, the order stage 5 would use), and stage 5 reads that one plane per
+//    direction instead of five: the same bits, 64 B/cell less to read and
+//    fewer registers held at stage 5.
 // Every quantity is computed only inside the reference's loop range and is 0
 // outside it, as the reference's never-written array entries are.
 //
@@ -62,6 +67,7 @@ struct StageArgs {
     int n_f;
     int step, ns;
     double a[6], aSum;
+    double b5[6];  // stage 5's coefficients times dt (stage 4 forms its partial sum)
     int TI;       // output rows per strip
     int nstrip;   // strips in x
     int nbx;      // tiles along p
@@ -166,7 +172,9 @@ __device__ __forceinline__ void stage_strip(const StageArgs &A, const int jw, co
     auto row_ok = [&](int i) { return IN || (i >= -2 && i < nx + 2 && in_j); };
     double pfs_a = ldr(fin, pitch, nx, r0, jc), pf0_a = ldr(f0p, pitch, nx, r0, jc);
     double pfs_b = ldr(fin, pitch, nx, r0 + 1, jc), pf0_b = ldr(f0p, pitch, nx, r0 + 1, jc);
-    constexpr int NH = STEP > 0 ? STEP : 1;  // stored flux planes read by this stage
+    // stored flux planes read by this stage; stage 5 reads only plane 4, which
+    // holds its partial sum
+    constexpr int NH = STEP == 5 ? 1 : (STEP > 0 ? STEP : 1);
     double hx[NH], hp[NH];
     bool hvx = false, hvp = false;  // the loaded face lies in the FxH / FpH ranges
     auto load_h = [&](int face) {
@@ -174,8 +182,9 @@ __device__ __forceinline__ void stage_strip(const StageArgs &A, const int jw, co
         hvp = IN || (face >= -1 && face < nx + 1 && jr_fp);
 #pragma unroll
         for (int k = 0; k < NH; k++) {
-            hx[k] = (k < STEP) ? ldr(P.FxH[k], pitch, nx, face, jc) : 0.0;
-            hp[k] = (k < STEP) ? ldr(P.FpH[k], pitch, nx, face, jc) : 0.0;
+            const int plane = STEP == 5 ? 4 : k;
+            hx[k] = (k < STEP) ? ldr(P.FxH[plane], pitch, nx, face, jc) : 0.0;
+            hp[k] = (k < STEP) ? ldr(P.FpH[plane], pitch, nx, face, jc) : 0.0;
         }
     };
     load_h(r0 - 1);
@@ -183,7 +192,8 @@ __device__ __forceinline__ void stage_strip(const StageArgs &A, const int jw, co
     // The RK combination sum_k a_k FxH_k is evaluated left to right; the terms
     // of the earlier stages are folded into Sx/Sp as soon as their prefetched
     // values are consumed, which frees their registers a whole iteration early.
-    double Sx = 0.0, Sp = 0.0;
+    // Stage 4 folds the same values into stage 5's partial sum Qx/Qp.
+    double Sx = 0.0, Sp = 0.0, Qx = 0.0, Qp = 0.0;
 
     // One march row; each quantity is masked to its reference loop range.
     auto row = [&](int r) {
@@ -195,13 +205,27 @@ __device__ __forceinline__ void stage_strip(const StageArgs &A, const int jw, co
         pf0_a = pf0_b;
         pfs_b = ldr(fin, pitch, nx, r + 2, jc);
         pf0_b = ldr(f0p, pitch, nx, r + 2, jc);
-        if (STEP > 0) {
+        if (STEP == 5) {
+            // b_0 F_0 + ... + b_4 F_4 as stage 4 formed it; outside the face
+            // ranges the masked sum of the five terms is +0
+            Sx = hvx ? hx[0] : 0.0;
+            Sp = hvp ? hp[0] : 0.0;
+        } else if (STEP > 0) {
             Sx = A.a[0] * (hvx ? hx[0] : 0.0);
             Sp = A.a[0] * (hvp ? hp[0] : 0.0);
 #pragma unroll
             for (int k = 1; k < NH; k++) {
                 Sx = Sx + A.a[k] * (hvx ? hx[k] : 0.0);
                 Sp = Sp + A.a[k] * (hvp ? hp[k] : 0.0);
+            }
+            if (STEP == 4) {
+                Qx = A.b5[0] * (hvx ? hx[0] : 0.0);
+                Qp = A.b5[0] * (hvp ? hp[0] : 0.0);
+#pragma unroll
+                for (int k = 1; k < NH; k++) {
+                    Qx = Qx + A.b5[k] * (hvx ? hx[k] : 0.0);
+                    Qp = Qp + A.b5[k] * (hvp ? hp[k] : 0.0);
+                }
             }
         }
         load_h(r);
@@ -242,9 +266,12 @@ __device__ __forceinline__ void stage_strip(const StageArgs &A, const int jw, co
         if (store_flux) {
             const bool own_x = ix >= i0 && (ix < i0 + A.TI || last_strip);
             const bool own_c = (ix >= i0 || i0 == 0) && (ix < i0 + A.TI || last_strip);
-            // read again only by the later stages: streamed past the caches
-            if (vFx && own_x && own_row_x) __builtin_nontemporal_store(FxH, &P.FxH[STEP][pidx(pitch, ix, j)]);
-            if (vFp && own_c && own_row_p) __builtin_nontemporal_store(FpH, &P.FpH[STEP][pidx(pitch, ix, j)]);
+            // read again only by the later stages: streamed past the caches;
+            // stage 4 stores stage 5's partial sum instead of its fluxes
+            const double hx_out = STEP == 4 ? Qx + A.b5[4] * FxH : FxH;
+            const double hp_out = STEP == 4 ? Qp + A.b5[4] * FpH : FpH;
+            if (vFx && own_x && own_row_x) __builtin_nontemporal_store(hx_out, &P.FxH[STEP][pidx(pitch, ix, j)]);
+            if (vFp && own_c && own_row_p) __builtin_nontemporal_store(hp_out, &P.FpH[STEP][pidx(pitch, ix, j)]);
         }
 
         // ---- low-order fluxes of stage 0 at x-face r and column r
@@ -596,6 +623,7 @@ static int stage_args(vx_ctx *ctx, const vx_patch *p, int step, double dt, const
     A.ns = step + 1;
     for (int k = 0; k < 6; k++) A.a[k] = 0.0;
     for (int k = 0; k < A.ns; k++) A.a[k] = vx::rk_a[step][k] * dt;
+    for (int k = 0; k < 6; k++) A.b5[k] = vx::rk_a[5][k] * dt;  // as stage 5 forms its a[k]
     A.aSum = A.a[0];
     for (int k = 1; k < A.ns; k++) A.aSum = A.aSum + A.a[k];
     A.nbx = (p->n_p + OUTW - 1) / OUTW;
