@@ -36,9 +36,11 @@ def kernel_bytes_per_cell(kernel, step):
         # FxL0/FpL0 written at stage 0 and read afterwards
         return 8 + 16 + 16 + 16 * s + 16
     if kernel == "k_stage":
-        # store-flux fused stage: read f_s and f^n, read FxH/FpH of stages < s,
-        # write FxH_s/FpH_s (except the last stage) and f_{s+1}
-        return 8 + 8 + 16 * s + (16 if s < 5 else 0) + 8
+        # store-flux fused stage: read f_s and f^n, read FxH/FpH of stages < s
+        # (at the last stage one partial-sum plane per direction, which stage 4
+        # writes in place of its own fluxes), write FxH_s/FpH_s (except the
+        # last stage) and f_{s+1}
+        return 8 + 8 + (16 * s if s < 5 else 16) + (16 if s < 5 else 0) + 8
     raise ValueError(kernel)
 
 
@@ -446,7 +448,8 @@ def main():
                    "hbm_frac_216B": value * BYTES_PER_UPDATE / HBM_PEAK,
                    "wall_s": wall, "graph_ms_per_step": graph_ms,
                    "ms_per_step_driver_loop": driver_ms,
-                   "tile_rows": run.tile_rows() if a.mode == "fused" else None},
+                   "tile_rows": run.tile_rows() if a.mode == "fused" else None,
+                   "tile_waves": run.tile_waves() if a.mode == "fused" else None},
         "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved / 1e9,
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s", "frac": achieved / HBM_PEAK,
                      "traffic": measured_traffic(kernel, params.nx, params.np),

@@ -152,6 +152,10 @@ int vx_fct_stage_fused_batch(vx_ctx *ctx, const vx_stage_item *items, int n, int
  * on the first call for that shape, stage and precision, 0 before it.  No
  * counterpart in the reference: it reports the tile shape the library chose. */
 int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed);
+/* Resident waves per SIMD of the kernel form chosen with those strips: 4 (the
+ * form compiled for at most 128 VGPRs) or 3 (the other), 0 before the first
+ * call; both forms give the same bits.  No counterpart in the reference. */
+int vx_stage_tile_waves(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed);
 int vx_fct_stage_fused_available(void);
 
 /* Ghost fill of slot `slot` of a patch whose neighbours are all the domain-edge

@@ -196,3 +196,24 @@ def test_ragged_and_tiny_shapes(nx, np_, fused):
     compare_fields(g, o, f"{nx}x{np_}")
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("rows", ["16", "4112", "9", "4105"])
+def test_fused_stage_kernel_forms_and_strip_heights(rows, monkeypatch):
+    """The fused stage's two kernel forms (three and four resident waves per
+    SIMD; VX_TILE_ROWS pins the strip height, + 4096 for the four-wave form)
+    at two strip heights: 10 steps of a 384 x 512 two-stream case (interior
+    and edge strips) bitwise against the oracle."""
+    monkeypatch.setenv("VX_TILE_ROWS", rows)
+    p = two_stream(nx=384, np=512, length=4 * math.pi, amplitude=0.05)
+    g, o = va.Run(p, fused=True), OracleRun(p)
+    dt = o.calc_dt()
+    g.advance(dt, 10)
+    o.advance(dt, 10)
+    assert g.tile_rows() == [int(rows) % 4096] * 6
+    assert g.tile_waves() == [4 if int(rows) > 4096 else 3] * 6
+    assert_same(f"rows {rows} f1", g.get("f1", 2), o.get("f1", 2))
+    assert_same(f"rows {rows} f0", g.get("f0", 2), o.get("f0", 2))
+    compare_fields(g, o, f"rows {rows}")
+    g.close()
+    o.close()

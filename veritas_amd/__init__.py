@@ -71,6 +71,7 @@ def host_lib():
         L.vxh_set_mixed.argtypes = [vp, i]
         L.vxh_graph_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
         L.vxh_tile_rows.argtypes = [vp, i, i, i]
+        L.vxh_tile_waves.argtypes = [vp, i, i, i]
         L.vxh_signatures.restype = i
         L.vxh_signatures.argtypes = [vp, i, i, i, vp, i]
         L.vxh_server_create.restype = vp
@@ -223,6 +224,13 @@ class Run:
         n_x = self.params.nx if n_x is None else n_x
         n_p = self.params.np if n_p is None else n_p
         return [host_lib().vxh_tile_rows(self.h, n_x, n_p, s) for s in range(6)]
+
+    def tile_waves(self, n_x=None, n_p=None):
+        """Resident waves per SIMD (3 or 4) of the fused stage's kernel form
+        chosen with those strips at each RK stage (0 until it is chosen)."""
+        n_x = self.params.nx if n_x is None else n_x
+        n_p = self.params.np if n_p is None else n_p
+        return [host_lib().vxh_tile_waves(self.h, n_x, n_p, s) for s in range(6)]
 
     def graph_stats(self):
         out = (ctypes.c_int * 2)()

@@ -154,6 +154,12 @@ int vxh_tile_rows(void *h, int n_x, int n_p, int step) {
     return vx_stage_tile_rows(r->sm->dev->ctx, n_x, n_p, step, r->sm->meshes[0]->mixed ? 1 : 0);
 }
 
+// resident waves per SIMD of the fused stage's kernel form chosen for that shape (0: not yet chosen)
+int vxh_tile_waves(void *h, int n_x, int n_p, int step) {
+    Run *r = (Run *)h;
+    return vx_stage_tile_waves(r->sm->dev->ctx, n_x, n_p, step, r->sm->meshes[0]->mixed ? 1 : 0);
+}
+
 int vxh_advance(void *h, double dt, int nsteps) {
     return guarded(h, [&](Run *r) {
         for (int i = 0; i < nsteps; i++) r->sm->Advance(dt);

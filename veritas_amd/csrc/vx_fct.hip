@@ -617,16 +617,24 @@ int tune_policy(vx_ctx *ctx, vx_patch_table *t, int kind, bool mp, const std::ve
     int best = pol, rc = VX_OK;
     float best_ms = 1e30f;
     if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
+        bool failed = false;
         for (int c : cands) {
             pol = c;
             if ((rc = table_rebuild(ctx, t))) break;
             launch();  // warm
-            if (hipEventRecord(e0, ctx->stream) != hipSuccess) break;
-            launch();
-            launch();
-            if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
+            // the fastest of three launches, each timed on its own
+            float ms = 1e30f;
+            for (int rep = 0; rep < 3 && !failed; rep++) {
+                float tm = 0.0f;
+                if (hipEventRecord(e0, ctx->stream) != hipSuccess) failed = true;
+                if (!failed) launch();
+                if (!failed && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                                hipEventSynchronize(e1) != hipSuccess ||
+                                hipEventElapsedTime(&tm, e0, e1) != hipSuccess))
+                    failed = true;
+                if (!failed) ms = std::min(ms, tm);
+            }
+            if (failed) break;
             if (ms < 0.97f * best_ms) {
                 best_ms = ms;
                 best = c;

@@ -69,6 +69,7 @@ struct StageArgs {
     double a[6], aSum;
     double b5[6];  // stage 5's coefficients times dt (stage 4 forms its partial sum)
     int TI;       // output rows per strip
+    int w4;       // 1: the kernel form held to four waves per SIMD (k_stage_w4)
     int nstrip;   // strips in x
     int nbx;      // tiles along p
     double sinv;  // MP: 2^-k, the scale of the FP32 copies of f
@@ -412,6 +413,14 @@ template <int STEP, bool MP>
 __global__ __launch_bounds__(256) void k_stage(StageArgs A) {
     stage_tile<STEP, MP>(A, xcd_tile());
 }
+// The same stage compiled for four resident waves per SIMD (at most 128
+// VGPRs; some stages then keep a few loop-invariant values in scratch): more
+// waves to hide latency against a few more loads.  The strip-height timing
+// picks between the two forms per patch shape and stage; both give the same bits.
+template <int STEP, bool MP>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_stage_w4(StageArgs A) {
+    stage_tile<STEP, MP>(A, xcd_tile());
+}
 
 // Several patches (of independent runs) at the same RK stage in one launch:
 // workgroups [wg_start[k], wg_start[k+1]) of the XCD-ordered tile sequence
@@ -430,14 +439,18 @@ __global__ __launch_bounds__(256) void k_stage_batch(const StageArgs *items, con
 
 template <bool MP>
 void launch_stage_t(vx_ctx *ctx, const StageArgs &A, dim3 grid) {
+#define VX_STAGE(S)                                                        \
+    if (A.w4) k_stage_w4<S, MP><<<grid, 256, 0, ctx->stream>>>(A);         \
+    else k_stage<S, MP><<<grid, 256, 0, ctx->stream>>>(A);
     switch (A.step) {
-        case 0: k_stage<0, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 1: k_stage<1, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 2: k_stage<2, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 3: k_stage<3, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
-        case 4: k_stage<4, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
-        default: k_stage<5, MP><<<grid, 256, 0, ctx->stream>>>(A); break;
+        case 0: VX_STAGE(0) break;
+        case 1: VX_STAGE(1) break;
+        case 2: VX_STAGE(2) break;
+        case 3: VX_STAGE(3) break;
+        case 4: VX_STAGE(4) break;
+        default: VX_STAGE(5) break;
     }
+#undef VX_STAGE
 }
 
 void launch_stage(vx_ctx *ctx, StageArgs &A, bool mp) {
@@ -459,6 +472,10 @@ void launch_stage(vx_ctx *ctx, StageArgs &A, bool mp) {
 // is being recorded the choice falls back to the shortest strip that still
 // gives 2048 wavefronts.
 //
+// Each strip height is timed in both kernel forms (k_stage: three waves per
+// SIMD for the stages that need more than 128 VGPRs; k_stage_w4: four), the
+// four-wave form winning only when more than 3 % faster; a choice is kept as
+// rows + W4_FORM for the four-wave form.
 // The choices are kept per device for the whole process (every context on the
 // device shares them: a serving loop's runs of one shape time it once).
 struct TileKey {
@@ -467,6 +484,7 @@ struct TileKey {
         return std::tie(device, nx, np, mp, step) < std::tie(o.device, o.nx, o.np, o.mp, o.step);
     }
 };
+constexpr int W4_FORM = 4096;
 std::mutex g_tile_mu;
 std::map<TileKey, int> g_tiles;
 
@@ -514,13 +532,10 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
     const int nx = A0.p.n_x, np = A0.p.n_p;
     for (auto &c : ctx->ti_cache)
         if (c[0] == nx && c[1] == np && c[3] == (int)mp && c[4] == A0.step) return c[2];
-    if (const int known = tile_lookup(ctx->device, nx, np, (int)mp, A0.step)) {
-        ctx->ti_cache.push_back({nx, np, known, (int)mp, A0.step});
-        return known;
-    }
     static const int cand[] = {64, 32, 16, 8, 4, 2};
-    // VX_TILE_ROWS pins the choice (one value, or six comma-separated values,
-    // one per RK stage), e.g. to profile the configuration a timed run chose
+    // VX_TILE_ROWS pins the choice for this context (one value, or six
+    // comma-separated values, one per RK stage; rows + 4096 for the four-wave
+    // form), e.g. to profile the configuration a timed run chose
     if (const char *env = std::getenv("VX_TILE_ROWS")) {
         int v[6], n = 0;
         for (const char *q = env; *q && n < 6;) {
@@ -529,10 +544,14 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
             if (*q == ',') q++;
         }
         const int ti = n == 6 ? v[A0.step] : (n >= 1 ? v[0] : 0);
-        if (ti >= 1 && ti <= 1024) {
+        if ((ti >= 1 && ti <= 1024) || (ti > W4_FORM && ti <= W4_FORM + 1024)) {
             ctx->ti_cache.push_back({nx, np, ti, (int)mp, A0.step});  // this context only
             return ti;
         }
+    }
+    if (const int known = tile_lookup(ctx->device, nx, np, (int)mp, A0.step)) {
+        ctx->ti_cache.push_back({nx, np, known, (int)mp, A0.step});
+        return known;
     }
     int guess = 2;
     for (int ti : cand)
@@ -564,23 +583,37 @@ int tile_rows(vx_ctx *ctx, const StageArgs &A0, bool mp) {
     int best = guess;
     if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
         float best_ms = 1e30f;
+        bool failed = false;
         for (int ti : cands) {
             if (ti > 2 && ti >= 2 * nx) continue;  // a single strip either way
-            StageArgs A = A0;
-            A.TI = ti;
-            launch_stage(ctx, A, mp);  // warm
-            if (hipEventRecord(e0, ctx->stream) != hipSuccess) break;
-            launch_stage(ctx, A, mp);
-            launch_stage(ctx, A, mp);
-            if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) break;
-            float ms = 0.0f;
-            if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) break;
-            // candidates run from tall to short strips: a shorter strip must be
-            // more than 3 % faster to win (ties go to fewer warm-up rows)
-            if (ms < 0.97f * best_ms) {
-                best_ms = ms;
-                best = ti;
+            for (int w4 = 0; w4 < 2 && !failed; w4++) {
+                StageArgs A = A0;
+                A.TI = ti;
+                A.w4 = w4;
+                launch_stage(ctx, A, mp);  // warm
+                // the fastest of three launches, each timed on its own (the
+                // minimum is steadier than a sum against the device's noise)
+                float ms = 1e30f;
+                for (int rep = 0; rep < 3 && !failed; rep++) {
+                    float t = 0.0f;
+                    if (hipEventRecord(e0, ctx->stream) != hipSuccess) failed = true;
+                    if (!failed) launch_stage(ctx, A, mp);
+                    if (!failed && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
+                                    hipEventSynchronize(e1) != hipSuccess ||
+                                    hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+                        failed = true;
+                    if (!failed) ms = std::min(ms, t);
+                }
+                if (failed) break;
+                // candidates run from tall to short strips, the three-wave form
+                // first: a later one must be more than 3 % faster to win (ties
+                // go to fewer warm-up rows and the three-wave form)
+                if (ms < 0.97f * best_ms) {
+                    best_ms = ms;
+                    best = ti + (w4 ? W4_FORM : 0);
+                }
             }
+            if (failed) break;
         }
     }
     if (e0) (void)hipEventDestroy(e0);
@@ -635,7 +668,10 @@ static int stage_args(vx_ctx *ctx, const vx_patch *p, int step, double dt, const
         A.sinv = std::ldexp(1.0, -std::ilogb(f_scale));
         A.eps32 = (float)(1.0e-10 * A.sinv * A.sinv);
     }
-    A.TI = tile_rows(ctx, A, mp);
+    A.w4 = 0;
+    const int choice = tile_rows(ctx, A, mp);
+    A.TI = choice % W4_FORM;
+    A.w4 = choice / W4_FORM;
     A.nstrip = (p->n_x + A.TI - 1) / A.TI;
     return VX_OK;
 }
@@ -753,11 +789,19 @@ int vx_fct_stage_fused_batch(vx_ctx *ctx, const vx_stage_item *items, int n, int
     return rc ? rc : batch_launch(ctx, mixed, step, true);
 }
 
-int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed) {
+static int tile_choice(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed) {
     for (auto &c : ctx->ti_cache)
         if (c[0] == n_x && c[1] == n_p && c[3] == (mixed ? 1 : 0) && c[4] == step) return c[2];
-    if (const int known = tile_lookup(ctx->device, n_x, n_p, mixed ? 1 : 0, step)) return known;
-    return 0;
+    return tile_lookup(ctx->device, n_x, n_p, mixed ? 1 : 0, step);
+}
+
+int vx_stage_tile_rows(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed) {
+    return tile_choice(ctx, n_x, n_p, step, mixed) % W4_FORM;
+}
+
+int vx_stage_tile_waves(vx_ctx *ctx, int32_t n_x, int32_t n_p, int step, int mixed) {
+    const int c = tile_choice(ctx, n_x, n_p, step, mixed);
+    return c == 0 ? 0 : (c >= W4_FORM ? 4 : 3);
 }
 
 }  // extern "C"
