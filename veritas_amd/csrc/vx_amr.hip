@@ -54,6 +54,7 @@ struct ProgArgs {
     // flux programs
     int step, ns;
     double a[6], aSum;
+    double b5[6];  // stage 5's coefficients times dt (stage 4 writes its partial sum)
     const double *a2;
     int n_f;
 };
@@ -226,6 +227,13 @@ __device__ void flux_task(const ProgArgs &A, const double *E, const vx_cell_task
     for (int k = 1; k < A.ns; k++) s = s + A.a[k] * get(KH[k], kp, T.di, T.dj);
     double ls = A.aSum * get(KL, kp, T.di, T.dj);
     at(xdir ? K.FxDS : K.FpDS, kp, T.di, T.dj) = s - ls;
+    if (A.step == 4) {
+        // stage 5's partial sum at this face, which the stage-5 flux launch reads
+        // from the stage-5 planes (in its left-to-right order)
+        double q = A.b5[0] * get(KH[0], kp, T.di, T.dj);
+        for (int k = 1; k < 5; k++) q = q + A.b5[k] * get(KH[k], kp, T.di, T.dj);
+        at(KH[5], kp, T.di, T.dj) = q;
+    }
 }
 
 __device__ __forceinline__ void program_task(const ProgArgs &A, const double *E, const int t) {
@@ -667,6 +675,7 @@ int vx_program_run_fluxes(vx_ctx *ctx, const vx_patch_table *table, const vx_pro
     for (int k = 0; k < A.ns; k++) A.a[k] = vx::rk_a[step][k] * dt;
     A.aSum = A.a[0];
     for (int k = 1; k < A.ns; k++) A.aSum = A.aSum + A.a[k];
+    for (int k = 0; k < 6; k++) A.b5[k] = vx::rk_a[5][k] * dt;  // as stage 5 forms its a[k]
     A.a2 = fl->a2;
     A.n_f = fl->n_f;
     return launch_batches(ctx, prog, A, fl->E);

@@ -32,6 +32,7 @@ struct Sub0Coef {
     int ns;          // step + 1 terms in the RK combination
     double a[6];     // RK coefficients times dt (Rectangle.cpp:1396-1517)
     double aSum;     // a[0] + ... + a[ns-1], summed left to right
+    double b5[6];    // stage 5's coefficients times dt (stage 4 forms its partial sum)
     double sinv = 1.0;   // mixed precision: 2^-k, the scale of the FP32 copies of f
     float eps32 = 0.0f;  // mixed precision: the WENO epsilon (1e-10) in the scaled units
 };
@@ -61,6 +62,12 @@ __device__ __forceinline__ double patch_E(const vx_patch &p, const vx_fields &fl
 // program may have replaced them at interface faces).
 // MP: the WENO weights from FP32 copies of f scaled by A.sinv (vxd::weno_mp);
 // candidates, fluxes and the RK combination stay FP64.
+// Stage 5 needs the earlier stages' fluxes only through its combination: stage
+// 4 also writes stage 5's partial sum b_0 F_0 + ... + b_4 F_4 (left to right,
+// stage 5's order) into the stage-5 flux planes, which stage 5 reads at each
+// position it owns before writing its own fluxes there (positions it does not
+// own are only warm-up whose results are discarded); the coarse-fine flux
+// program writes the sum at the faces whose fluxes it replaces.
 constexpr int S0W = 58;  // positions per wavefront
 
 struct Sub0Tiles {
@@ -99,14 +106,17 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
     double G_m1 = gamma_(pj, as_m1, P.mc2inv);
     double G_0 = (as_0 == as_m1) ? G_m1 : gamma_(pj, as_0, P.mc2inv);
     double pfs_a = ld(r0), pfs_b = ld(r0 + 1);
-    constexpr int NH = STEP > 0 ? STEP : 1;
+    // stored planes read: those of the earlier stages, at stage 5 only its
+    // partial sum (in plane 5)
+    constexpr int NH = STEP == 5 ? 1 : (STEP > 0 ? STEP : 1);
     double hx[NH], hp[NH], lx = 0.0, lp = 0.0;
     auto load_h = [&](int pos) {  // stored planes at position (pos, j)
         const bool v = pos >= -2 && pos < nx + 2 && in_j;
 #pragma unroll
         for (int k = 0; k < NH; k++) {
-            hx[k] = (k < STEP && v) ? P.FxH[k][pidx(pitch, pos, j)] : 0.0;
-            hp[k] = (k < STEP && v) ? P.FpH[k][pidx(pitch, pos, j)] : 0.0;
+            const int plane = STEP == 5 ? 5 : k;
+            hx[k] = (k < STEP && v) ? P.FxH[plane][pidx(pitch, pos, j)] : 0.0;
+            hp[k] = (k < STEP && v) ? P.FpH[plane][pidx(pitch, pos, j)] : 0.0;
         }
         if (STEP > 0) {
             lx = v ? P.FxL0[pidx(pitch, pos, j)] : 0.0;
@@ -177,14 +187,30 @@ __device__ __forceinline__ void sub0s_tile(const vx_patch &P, const Sub0Coef &A,
                 __builtin_nontemporal_store(fph, &P.FpH[STEP][id]);
                 if (STEP == 0) P.FpL0[id] = fpl;
             }
+            if (STEP == 4) {  // stage 5's partial sum, in its order
+                double qx = A.b5[0] * chx[0], qp = A.b5[0] * chp[0];
+#pragma unroll
+                for (int k = 1; k < NH; k++) {
+                    qx = qx + A.b5[k] * chx[k];
+                    qp = qp + A.b5[k] * chp[k];
+                }
+                if (vFx) __builtin_nontemporal_store(qx + A.b5[4] * fxh, &P.FxH[5][id]);
+                if (vFp) __builtin_nontemporal_store(qp + A.b5[4] * fph, &P.FpH[5][id]);
+            }
             const double xLS = A.aSum * (STEP == 0 ? fxl : clx);
             const double pLS = A.aSum * (STEP == 0 ? fpl : clp);
-            double sx = A.a[0] * (STEP == 0 ? fxh : chx[0]);
-            double sp = A.a[0] * (STEP == 0 ? fph : chp[0]);
+            double sx, sp;
+            if (STEP == 5) {  // the partial sum stage 4 formed (+0 where not stored)
+                sx = chx[0] + A.a[5] * fxh;
+                sp = chp[0] + A.a[5] * fph;
+            } else {
+                sx = A.a[0] * (STEP == 0 ? fxh : chx[0]);
+                sp = A.a[0] * (STEP == 0 ? fph : chp[0]);
 #pragma unroll
-            for (int k = 1; k <= STEP; k++) {
-                sx = sx + A.a[k] * (k == STEP ? fxh : chx[k < NH ? k : 0]);
-                sp = sp + A.a[k] * (k == STEP ? fph : chp[k < NH ? k : 0]);
+                for (int k = 1; k <= STEP; k++) {
+                    sx = sx + A.a[k] * (k == STEP ? fxh : chx[k < NH ? k : 0]);
+                    sp = sp + A.a[k] * (k == STEP ? fph : chp[k < NH ? k : 0]);
+                }
             }
             P.FxDS[id] = sx - xLS;
             P.FpDS[id] = sp - pLS;
@@ -263,9 +289,10 @@ __device__ __forceinline__ void lo_predictor(const vx_patch &P, double aSum, int
 // [-1, n_x+1) x [-1, n_p+1), then C on the NS range [-2, n_x+2) x [-2, n_p+2):
 // 1 everywhere, the limiter formula on interior faces i in [1, n_x), j in
 // [0, n_p).  Register-streaming form: a wavefront owns 64 consecutive p
-// positions, lane l <-> j = jw - 2 + l, and produces lanes 2..49 (48
-// positions, so that each wavefront's row segments start on a 64-byte
-// boundary of the pitched row; p neighbours come from DPP lane shifts).  It marches along x over
+// positions, lane l <-> j = jw - 2 + l, and produces lanes 2..57 (56
+// positions, so that each wavefront's own row segments start on a 64-byte
+// boundary of the pitched row; p neighbours come from DPP lane shifts, and the
+// corrector's C at (i, j+1) of lane 57 from lane 58, whose R needs lane 59).  It marches along x over
 // a strip of C rows [i0, i1): at row i it forms R of row i from rows i-1..i+1
 // of f^n, f^td and the antidiffusive fluxes (held in registers, one new row
 // loaded per step), and C of row i from R of rows i and i-1.  R is stored
@@ -278,7 +305,7 @@ __device__ __forceinline__ void lo_predictor(const vx_patch &P, double aSum, int
 // and C are formed but not stored).  The values use C before the cross-patch
 // C synchronisation; cells next to a face that the synchronisation changes
 // are corrected again after it (VX_OP_CORRECT).
-constexpr int LIMW = 48;  // positions per limiter wavefront (of 62 it could form)
+constexpr int LIMW = 56;  // positions per limiter wavefront (7 x 64 bytes)
 
 struct LimTiles {
     int TI, nstrip, nbx;
@@ -688,6 +715,7 @@ int sub0_fluxes_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const
     A.fl = *fl;
     A.step = step;
     rk_coefficients(step, dt, A.a, A.ns, A.aSum);
+    for (int k = 0; k < 6; k++) A.b5[k] = vx::rk_a[5][k] * dt;  // as stage 5 forms its a[k]
     if (!t->sub0_tuned[mp] && !ctx->capturing) {
         rc = tune_policy(ctx, t, 0, mp, {-2048, 128, 64, 32, 16, 8}, [&] {
             if (mp) launch_sub0s_batch<true>(ctx, t, step, A);
@@ -714,6 +742,7 @@ int vx_fct_sub0_fluxes(vx_ctx *ctx, const vx_patch *p, int step, double dt, cons
     A.fl = *fl;
     A.step = step;
     rk_coefficients(step, dt, A.a, A.ns, A.aSum);
+    for (int k = 0; k < 6; k++) A.b5[k] = vx::rk_a[5][k] * dt;  // as stage 5 forms its a[k]
     const Sub0Tiles T = sub0_tiles(*p);
     dim3 grid((unsigned)(T.nbx * ((T.nstrip + 3) / 4)));
     {
