@@ -347,7 +347,9 @@ typedef struct vx_range_task {
 int vx_transfer_ranges(vx_ctx *ctx, const vx_patch_table *table, const vx_range_task *ranges, int32_t n,
                        int32_t r);
 /* Coarse-fine flux program of RK stage `step` (run after vx_fct_sub0_fluxes
- * of every patch and before vx_fct_sub0_predictor), finest coarse level first. */
+ * of every patch and before vx_fct_sub0_predictor), finest coarse level first.
+ * At step 4 it also writes stage 5's partial RK sum at the faces whose fluxes
+ * it replaces (see vx_fct_sub0_fluxes_table). */
 int vx_program_run_fluxes(vx_ctx *ctx, const vx_patch_table *table, const vx_program *prog,
                           int step, double dt, int r, const vx_fields *fl);
 
@@ -358,6 +360,11 @@ int vx_program_run_fluxes(vx_ctx *ctx, const vx_patch_table *table, const vx_pro
  * Each patch's results are bit-identical to its own launch: within a sub-step
  * the patches only write their own planes.  The table must hold the current
  * plane pointers (vx_patch_table_update). */
+/* The sub-step-0 flux launches (these and vx_fct_sub0_fluxes) at step 4 also
+ * write stage 5's partial RK sum (its coefficients times the same dt over the
+ * fluxes of steps 0-4, left to right) into the FxH[5]/FpH[5] planes, and at
+ * step 5 read it there in place of the five earlier flux planes: run steps 4
+ * and 5 of a time step with one dt. */
 int vx_fct_sub0_fluxes_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, const vx_fields *fl);
 int vx_fct_sub0_predictor_table(vx_ctx *ctx, vx_patch_table *t, int step, double dt, int32_t n_f);
 int vx_fct_sub1_table(vx_ctx *ctx, vx_patch_table *t, int32_t n_f);
