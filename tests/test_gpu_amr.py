@@ -156,16 +156,18 @@ def test_regrid_to_the_same_hierarchy_keeps_the_patches(amr_pair):
 
 
 def test_moment_forms_agree_on_every_level():
-    """The p-moments in both forms (the split form of large launches and the
-    producer/consumer pairs of small ones, pinned with VX_MOMENTS_FORM) on the
-    three-level case, whose patches have relativeToBottom 1, 2 and 4: the same
-    bits, and the oracle's."""
+    """The p-moments in every form (the split form of large launches, the
+    producer/consumer pairs of small ones and the wide form of relativeToBottom
+    1 patches, pinned with VX_MOMENTS_FORM) on the three-level case, whose
+    patches have relativeToBottom 1, 2 and 4, and whose finest columns have
+    the laser's A^2 (read from a logarithm table or formed per column): the
+    same bits, and the oracle's."""
     import os
     p = two_slab_laser()
     o = OracleRun(p)
     dt = o.calc_dt(0.4)
     runs = {}
-    for form in ("split", "pair"):
+    for form in ("split", "pair", "wide"):
         say(f"case: moments form {form}, 3 levels, 3 steps")
         os.environ["VX_MOMENTS_FORM"] = form
         try:
@@ -177,6 +179,7 @@ def test_moment_forms_agree_on_every_level():
     o.advance(dt, 3)
     compare_mesh(runs["split"], o, "split form")
     compare_mesh(runs["pair"], o, "pair form")
+    compare_mesh(runs["wide"], o, "wide form")
     compare_fields(runs["split"], o, "split form")
     for g in runs.values():
         g.close()

@@ -511,6 +511,171 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     }
 }
 
+// Wide form for relativeToBottom 1 (one chain per column and moment): a
+// workgroup owns WCOL consecutive x-columns.  Producer wavefront c stages the
+// summands of column c for one 64-cell p-chunk at a time in LDS (the same
+// operands as k_moments: the cell values of p-cells j-1, j, j+1 and the
+// logarithm terms of j, j+1, j+2 from the next lanes by DPP shifts), with the
+// loads of the next WDEPTH chunks in flight; the consumer wavefront adds the
+// previous chunk in p order, lane c the charge chain of column c and lane
+// 32 + c its current chain, all 2 WCOL chains in lockstep, so one wavefront's
+// dependent additions serve WCOL columns.  A workgroup barrier per chunk hands
+// the buffers over.  The same bits as k_moments<1, C>, for patches without
+// nested cells (the finest level: its cells are never covered by a finer one).
+constexpr int WCOL = 8;
+
+// Whole-wavefront rotation by one lane (DPP wave_rol:1): lane l receives the
+// value of lane (l + 1) mod 64.
+__device__ __forceinline__ double rol1(double v) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x134, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x134, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// Entries l + 1 and l + 2 (lane l's view) of the sequence cur[0..63],
+// nxt[0], nxt[1] (this chunk's values, then the next chunk's first two).
+__device__ __forceinline__ void next2_rot(int lane, double cur, double nxt, double &n1, double &n2) {
+    n1 = rol1(lane == 0 ? nxt : cur);
+    n2 = rol1(rol1(lane < 2 ? nxt : cur));
+}
+
+// Buffer resource over nbytes bytes from p (raw: no stride, no swizzle); a
+// load past nbytes returns 0.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t raw_rsrc(const void *p, int nbytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, nbytes, 0x00020000);
+}
+__device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
+template <int WDEPTH>
+__global__ __launch_bounds__(64 * (WCOL + 1)) void k_moments_wide(MomArgs A) {
+    const vx_patch &P = A.p;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = (int)(threadIdx.x & 63);
+    const int i0 = (int)blockIdx.x * WCOL;
+    // buffer, charge/current, column, p-cell of the chunk (rows of 65: the
+    // consumer's lanes read one row each, in different LDS banks)
+    __shared__ double T[2][2][WCOL][65];
+    // chunks k < nchunk hold p-cells; the loops run NK iterations, a multiple
+    // of WDEPTH, so that the producers' unrolled loop has no exit inside
+    // (chunks past nchunk consist of skipped cells: -0.0 summands, which
+    // change no sum)
+    const int nchunk = (P.n_p + 63) / 64;
+    const int NK = (nchunk + 1 + WDEPTH - 1) / WDEPTH * WDEPTH;
+    if (wave < WCOL) {
+        const int i = i0 + wave;
+        const bool col_ok = i < P.n_x;
+        const int ic = col_ok ? i : P.n_x - 1;
+        // this column's A^2 and a logarithm table built for it, if there is one
+        const double asq = P.q * P.q * cell_a2(A.fl, (long)(ic + P.x_pos));
+        const double *tab = nullptr;
+        for (int q = 1; q >= 0 && !tab; q--) {
+            const double *tq = A.tab + q * tab_stride(P.n_p);
+            if (tq[P.n_p + 2] != 0.0 && asq == tq[P.n_p + 3]) tab = tq;
+        }
+        // Chunk k (jb = 64 k) loads entry r = jb + l of two sequences: the
+        // column's f from p = -1 (f at p-cell jb - 1 + l) and the logarithm
+        // table (entry jb + l), with r clamped to n_p + 2 (f at p-cell n_p + 1,
+        // the table's last entry); those clamped entries serve skipped cells
+        // only.  The two values past the chunk's last lane (f at jb + 63,
+        // jb + 64; table entries jb + 64, jb + 65) are lanes 0 and 1 of chunk
+        // k + 1.  Lane offsets are min(l, n_p + 2 - jb'), jb' = min(jb, n_p + 2),
+        // and jb' is the scalar offset.
+        const __amdgpu_buffer_rsrc_t rf = raw_rsrc(P.f[1] + pidx(P.pitch, ic, -1), 8 * (P.n_p + 3));
+        const __amdgpu_buffer_rsrc_t rt = raw_rsrc(tab ? tab : P.f[1], 8 * (P.n_p + 3));
+        auto h = [&](int m) {
+            const double pm = momentum(P.pmin, P.dp, m, P.p_pos);
+            return gamma_(pm, asq, P.mc2inv) + A.c1 * pm;
+        };
+        // the column's own logarithm terms c2 log(h(m)/h(m-1)) at m = jb + l
+        auto logs = [&](int jb) { return A.c2 * vx_log(h(jb + lane) / h(jb + lane - 1)); };
+        // The loop is specialised on the table case, so that no branch joins
+        // the registers of the chunks in flight (a join makes the loads wait),
+        // and its ring of chunks in flight is unrolled: slot d holds chunk
+        // k0 + d in fixed registers and is reloaded with chunk k0 + d + WDEPTH
+        // once used (a rotating copy would wait for the loads in flight).
+        auto run = [&](auto tab_c) __attribute__((always_inline)) {
+            constexpr bool TAB = decltype(tab_c)::value;
+            double pv[WDEPTH], pt[WDEPTH];
+            auto load = [&](int k, int d) __attribute__((always_inline)) {
+                const int jl = min(64 * k, P.n_p + 2);
+                const int vo = 8 * min(lane, P.n_p + 2 - jl), so = 8 * jl;
+                pv[d] = buf_load(rf, vo, so);
+                if (TAB) pt[d] = buf_load(rt, vo, so);
+            };
+            // the slots are loaded in the order they are used (the scheduling
+            // barriers keep the compiler from reordering them), so that waiting
+            // for slot d leaves the later slots' loads in flight
+#pragma unroll
+            for (int d = 0; d < WDEPTH; d++) {
+                load(d, d);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+            double lc = TAB ? 0.0 : logs(0);  // no table: chunk k's logarithm terms
+            for (int k0 = 0; k0 < NK; k0 += WDEPTH) {
+#pragma unroll
+                for (int d = 0; d < WDEPTH; d++) {
+                    // chunk k (every summand skipped at k >= nchunk): no branch,
+                    // so the wait for each slot's loads counts only the slots
+                    // loaded after it
+                    const int k = k0 + d, dn = (d + 1) % WDEPTH;  // dn: chunk k + 1
+                    const int jb = 64 * k;
+                    double gm1, gnx;
+                    if (TAB) {
+                        gm1 = pt[d];
+                        gnx = pt[dn];
+                    } else {
+                        gm1 = lc;
+                        gnx = logs(jb + 64);
+                        lc = gnx;
+                    }
+                    double t0, tp, gm, gp1;
+                    next2_rot(lane, pv[d], pv[dn], t0, tp);
+                    next2_rot(lane, gm1, gnx, gm, gp1);
+                    const double tm = pv[d];
+                    const bool skip = (jb + lane >= P.n_p) | !col_ok;
+                    const double jv = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
+                    T[k & 1][0][wave][lane] = skip ? -0.0 : t0;
+                    T[k & 1][1][wave][lane] = skip ? -0.0 : jv;
+                    load(k + WDEPTH, d);
+                    __builtin_amdgcn_sched_barrier(0);
+                    __syncthreads();
+                }
+            }
+        };
+        if (tab) run(std::true_type{});
+        else run(std::false_type{});
+    } else {
+        // consumer: chain (lane & 31) of charge (lanes < 32) or current; lanes
+        // past WCOL repeat chain 0 and are not stored
+        // the dependent additions are the critical path of each chunk: the
+        // consumer issues ahead of the producers that share its SIMD
+        __builtin_amdgcn_s_setprio(3);
+        const int ch = (lane & 31) < WCOL ? (lane & 31) : 0;
+        const int kind = lane >> 5;
+        double acc = 0.0;
+        for (int k = 0; k < NK; k++) {
+            if (k > 0) {
+                const double *row = &T[(k - 1) & 1][kind][ch][0];
+                double a = acc;
+#pragma unroll
+                for (int e = 0; e < 64; e += 8) {
+                    const double v0 = row[e], v1 = row[e + 1], v2 = row[e + 2], v3 = row[e + 3];
+                    const double v4 = row[e + 4], v5 = row[e + 5], v6 = row[e + 6], v7 = row[e + 7];
+                    a += v0; a += v1; a += v2; a += v3; a += v4; a += v5; a += v6; a += v7;
+                }
+                acc = a;
+            }
+            __syncthreads();
+        }
+        const int c = lane & 31;
+        if (c < WCOL && i0 + c < P.n_x) {
+            if (lane < 32) P.chargeR[i0 + c] = acc * A.dpq;
+            else P.currentR[i0 + c] = acc * A.jscale;
+        }
+    }
+}
+
 // Rectangle::CalculateEnergy (Rectangle.cpp:284-305) without its data race:
 // chain (j, k) sums the p-sub-cell value k of every non-nested cell (i, j) in
 // increasing i, then scales by dx (the reference's OpenMP loop adds to the same
@@ -652,11 +817,22 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
     const unsigned g1 = (unsigned)((p->n_x + 15) / 16), g2 = (unsigned)((p->n_x + 7) / 8),
                    g4 = (unsigned)((p->n_x + 3) / 4);
     // wavefronts of the launch (4 sub-columns each) against the device's SIMDs;
-    // VX_MOMENTS_FORM=split or =pair pins the form (both give the same bits)
+    // VX_MOMENTS_FORM=split, =pair or =wide pins the form (all give the same
+    // bits); the wide form serves relativeToBottom 1 patches without nested
+    // cells (the finest level's) by default
     bool fuse = (size_t)(p->n_x * p->rtb + 3) / 4 <= (size_t)4 * ctx->n_cu;
+    bool wide = p->rtb == 1 && !comp && !p->nested;
     if (const char *form = std::getenv("VX_MOMENTS_FORM")) {
-        if (std::strcmp(form, "split") == 0) fuse = false;
-        if (std::strcmp(form, "pair") == 0) fuse = true;
+        if (std::strcmp(form, "split") == 0) fuse = wide = false;
+        if (std::strcmp(form, "pair") == 0) {
+            fuse = true;
+            wide = false;
+        }
+    }
+    if (wide) {
+        const unsigned gw = (unsigned)((p->n_x + WCOL - 1) / WCOL);
+        k_moments_wide<4><<<gw, 64 * (WCOL + 1), 0, ctx->stream>>>(A);
+        return check_launch(ctx, "k_moments_wide");
     }
     // the pair form: one workgroup of two wavefronts per group of 4 sub-columns
     const unsigned gp1 = (unsigned)((p->n_x + 3) / 4), gp2 = (unsigned)((p->n_x + 1) / 2),
