@@ -2,15 +2,16 @@
 //
 // Poisson (EMSolver.cpp:28-88,156-192).  The reference factors the dense N x N
 // matrix with LAPACK and solves it at every RK stage, O(N^2) per solve.  Here
-// the same matrix is solved to near correct rounding in O(N) (one 1024-thread
+// the same matrix is solved to near correct rounding in O(N) (one 512-thread
 // workgroup with the vector in registers up to N = 4096, a sequence of
 // multi-workgroup launches above):
 //   rows 1..N-1 of the matrix are the periodic 4th-order stencil
 //   (w2, w1, w3, w1, w2) with column 0 removed, i.e. the periodic operator with
 //   phi_0 pinned to 0; row 0 then gives phi_0.  The periodic operator factors
 //   exactly as L4 = L2 * M with L2 = [-1, 2, -1] and M = [-1, 14, -1] / 12, so
-//   an approximate solve is two prefix sums (L2^-1 on zero-mean data) and a
-//   33-tap convolution (M^-1, whose entries decay as (7 - 4 sqrt 3)^|k|).
+//   an approximate solve is two prefix sums (L2^-1 on zero-mean data) and M^-1,
+//   whose entries decay as (7 - 4 sqrt 3)^|k| (a 33-tap convolution, or two
+//   first-order recurrences).
 //   Three rounds of iterative refinement with a double-double residual of the
 //   reference's own coefficients take the result to the correctly rounded
 //   solution (the CPU oracle reaches the same point with a band LU).
@@ -30,17 +31,16 @@ using namespace vxd;
 
 namespace {
 
-constexpr int PT = 1024;  // threads of the Poisson workgroup
 constexpr int NG = 17;    // convolution half-width + 1 for M^-1
 
 struct PoissonArgs {
     vx_field_state fs;
     double w1, w2, w3;
     double g[NG];  // M^-1 coefficients, g[|k|]
+    double rho, C;  // M^-1 = C (1 - rho S^-1)^-1 (1 - rho S)^-1
     double invN;
 };
 
-constexpr int PW = PT / 64;  // wavefronts in the Poisson workgroup
 
 __device__ __forceinline__ double wave_incl_scan(double v) {
     const int lane = threadIdx.x & 63;
@@ -54,20 +54,6 @@ __device__ __forceinline__ double wave_incl_scan(double v) {
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
-}
-
-// EMFieldSolver::GetEfield (EMSolver.cpp:137-154)
-__device__ __forceinline__ double efield(const double *phi, int N, double fc, double ex0, int i) {
-    int ip1 = i + 1, im1 = i - 1, ip2 = i + 2, im2 = i - 2;
-    ip1 = (ip1 > -1) ? ip1 : (ip1 + N);
-    im1 = (im1 > -1) ? im1 : (im1 + N);
-    ip2 = (ip2 > -1) ? ip2 : (ip2 + N);
-    im2 = (im2 > -1) ? im2 : (im2 + N);
-    ip1 = (ip1 < N) ? ip1 : (ip1 - N);
-    im1 = (im1 < N) ? im1 : (im1 - N);
-    ip2 = (ip2 < N) ? ip2 : (ip2 - N);
-    im2 = (im2 < N) ? im2 : (im2 - N);
-    return -fc * (8 * (phi[ip1] - phi[im1]) - phi[ip2] + phi[im2]) + ex0;
 }
 
 // ------------------------------------------------ multi-workgroup solve, n_f > 4096
@@ -329,75 +315,151 @@ __global__ __launch_bounds__(MT) void k_pm_conv(PoissonMB P) {
     }
 }
 
-// Register/LDS variant for n_f <= 4 * PT: thread t owns the CH consecutive
-// entries [t*CH, t*CH + CH); neighbours (residual, convolution) come from LDS.
+// Single-workgroup variant for n_f <= SMALL_MAX: thread t owns the CH
+// consecutive entries [t*CH, t*CH + CH) in registers; neighbours come from an
+// LDS copy of the vector with a periodic halo of VH entries on either side, so
+// no stencil or recurrence needs index wrapping.
 //
-// Barriers: a workgroup sum or scan exchanges the 16 wavefront partials through
-// LDS once (one barrier); every thread then forms the same sequential sums of
-// the partials itself.  Consecutive exchanges alternate between two partial
+// M^-1 = C (1 - rho S^-1)^-1 (1 - rho S)^-1 with C = 12 rho / (1 - rho^2), so
+// (M^-1 y)_i = C (u_i + v_i - y_i) with the first-order recurrences
+// u_i = y_i + rho u_{i-1} and v_i = y_i + rho v_{i+1}.  Each thread runs them
+// over its entries after VH warm-up entries (rho^17 < 4e-20, the reach of the
+// convolution the multi-workgroup variant uses).  The approximate solve only
+// has to be accurate: the refinement rounds take x to the correctly rounded
+// solution whatever the rounding inside it.
+//
+// Barriers: a workgroup sum or scan exchanges the PW1 wavefront partials
+// through LDS once (one barrier); every thread then forms the same sums of the
+// partials itself.  Consecutive exchanges alternate between two partial
 // buffers, so a buffer is rewritten only after a later barrier has ordered
 // every read of it.
-constexpr int SMALL_MAX = 4 * PT;
+constexpr int PT1 = 512;             // threads of the single-workgroup solve
+constexpr int PW1 = PT1 / 64;        // its wavefronts
+constexpr int SMALL_MAX = 8 * PT1;   // largest n_f it serves (CH = 8)
+constexpr int VH = NG - 1;           // periodic halo of the LDS copy
+
+constexpr int VLEN = SMALL_MAX + 2 * VH + 8;  // positions of the LDS copy
+constexpr int VDUMMY = VLEN - VH;             // at(VDUMMY): a slot past them, written only
 
 struct SmallLds {
-    double v[SMALL_MAX];  // neighbour exchange: y for the convolution, x for the residual
-    double part[2][PW];   // wavefront partials, alternating
+    // at(j) = vec[j mod N] for j in [-VH, N + VH); the 8 positions after that
+    // are 0 (the last thread's backward recurrence may start there).  Position
+    // p = VH + j is stored at p + p / 8: a thread reads 8 consecutive entries
+    // per step of its loops, and the pad after every 8 puts the lanes of a
+    // wavefront (9 doubles apart) in different LDS banks.
+    double v[(VLEN + 1) + (VLEN + 1) / 8 + 1];
+    double part[2][PW1];  // wavefront partials, alternating
     double edge[5];       // c at entries 0, 1, 2, N-2, N-1 (row 0 of the matrix)
+    __device__ __forceinline__ double &at(int j) {
+        const int p = VH + j;
+        return v[p + (p >> 3)];
+    }
 };
 
-// Workgroup exclusive scan of the per-thread totals (thread order), and the
-// workgroup total; every thread gets the same total bits.
-__device__ __forceinline__ double block_scan(double tot, double (&part)[2][PW], int &buf, double *total) {
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const double inc = wave_incl_scan(tot);
-    if (lane == 63) part[buf][w] = inc;
-    __syncthreads();
-    double off = 0.0, all = 0.0;
-#pragma unroll
-    for (int k = 0; k < PW; k++) {
-        const double v = part[buf][k];
-        if (k == w) off = all;
-        all += v;
-    }
-    buf ^= 1;
-    if (total) *total = all;
-    return off + (inc - tot);
+// DPP move of a double; lanes outside row_mask or without a source lane get 0
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ double dpp_or_0(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, ROW_MASK, 0xf, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, ROW_MASK, 0xf, true);
+    return __hiloint2double(hi, lo);
 }
 
-__device__ __forceinline__ double block_sum(double v, double (&part)[2][PW], int &buf) {
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    v = wave_sum(v);
-    if (lane == 0) part[buf][w] = v;
+// Inclusive scan over the wavefront's lanes: within rows of 16 by row shifts
+// of 1, 2, 4, 8, then row 0's total into row 1 and row 2's into row 3
+// (row_bcast:15), then rows 0-1's total into rows 2 and 3 (row_bcast:31).
+__device__ __forceinline__ double wave_scan_dpp(double v) {
+    v += dpp_or_0<0x111>(v);
+    v += dpp_or_0<0x112>(v);
+    v += dpp_or_0<0x114>(v);
+    v += dpp_or_0<0x118>(v);
+    v += dpp_or_0<0x142, 0xa>(v);
+    v += dpp_or_0<0x143, 0xc>(v);
+    return v;
+}
+
+__device__ __forceinline__ double lane63(double v) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                            __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
+// Workgroup sum of v; every thread gets the same bits.
+__device__ __forceinline__ double block_sum1(double v, double (&part)[2][PW1], int &buf) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double tot = lane63(wave_scan_dpp(v));
+    if (lane == 0) part[buf][w] = tot;
     __syncthreads();
     double all = 0.0;
 #pragma unroll
-    for (int k = 0; k < PW; k++) all += part[buf][k];
+    for (int k = 0; k < PW1; k++) all += part[buf][k];
     buf ^= 1;
     return all;
 }
 
-// One approximate solve of A out = in (the steps of the multi-workgroup
-// variant above): S = exclusive prefix of (-sum_{k>=1} in_k, in_1, in_2, ...),
-// D = mean(S) - S (D_0 = 0), y = inclusive prefix of D, c = M^-1 y - (M^-1 y)_0,
-// and c_0 from row 0 of the matrix.
+// Workgroup exclusive scan of the per-thread totals in thread order.
+__device__ __forceinline__ double block_excl_scan1(double tot, double (&part)[2][PW1], int &buf) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const double inc = wave_scan_dpp(tot);
+    if (lane == 63) part[buf][w] = inc;
+    const double exc = dpp_or_0<0x138>(inc);  // wave_shr:1: lane l - 1's inclusive sum, 0 at lane 0
+    __syncthreads();
+    double off = 0.0;
+#pragma unroll
+    for (int k = 0; k < PW1; k++)
+        if (k < w) off += part[buf][k];
+    buf ^= 1;
+    return off + exc;
+}
+
+// The thread's entries into the LDS copy, with their periodic images in the
+// halo.  For N >= 2 VH an entry has at most one image (i + N for i < VH,
+// i - N for i >= N - VH), stored without a branch (an entry without an image
+// stores itself twice); below that (N >= 8) the images are looped over, a
+// branch the whole workgroup takes alike.
 template <int CH>
-__device__ void approx_small(const PoissonArgs &P, const double *in, double *out, SmallLds &L, int &buf) {
+__device__ __forceinline__ void put_periodic(SmallLds &L, const double *val, int N, int base) {
+    if (N >= 2 * VH) {
+#pragma unroll
+        for (int k = 0; k < CH; k++) {
+            const int i = base + k;
+            if (i < N) {
+                const int q = i < VH ? i + N : (i >= N - VH ? i - N : i);
+                L.at(i) = val[k];
+                L.at(q) = val[k];
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int k = 0; k < CH; k++) {
+        const int i = base + k;
+        if (i >= N) continue;
+        L.at(i) = val[k];
+        for (int q = i - N; q >= -VH; q -= N) L.at(q) = val[k];
+        for (int q = i + N; q < N + VH; q += N) L.at(q) = val[k];
+    }
+}
+
+// One approximate solve of A out = in: S = exclusive prefix of
+// (-sum_{k>=1} in_k, in_1, in_2, ...), D = mean(S) - S (D_0 = 0), y = inclusive
+// prefix of D, c = M^-1 y - (M^-1 y)_0, and c_0 from row 0 of the matrix.
+template <int CH>
+__device__ void approx_one(const PoissonArgs &P, const double *in, double *out, SmallLds &L, int &buf) {
     const int N = P.fs.n_f;
     const int base = threadIdx.x * CH;
     double loc = 0.0;
 #pragma unroll
     for (int k = 0; k < CH; k++)
         if (base + k >= 1 && base + k < N) loc += in[k];
-    const double tail = block_sum(loc, L.part, buf);
+    const double tail = block_sum1(loc, L.part, buf);
     double rp[CH], S[CH];
     double tot = 0.0;
 #pragma unroll
     for (int k = 0; k < CH; k++) {
-        int i = base + k;
+        const int i = base + k;
         rp[k] = (i == 0) ? -tail : (i < N ? in[k] : 0.0);
         tot += rp[k];
     }
-    double run = block_scan(tot, L.part, buf, nullptr);
+    double run = block_excl_scan1(tot, L.part, buf);
     double ssum = 0.0;
 #pragma unroll
     for (int k = 0; k < CH; k++) {
@@ -405,65 +467,58 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
         run += rp[k];
         if (base + k < N) ssum += S[k];
     }
-    const double d0 = block_sum(ssum, L.part, buf) * P.invN;
+    const double d0 = block_sum1(ssum, L.part, buf) * P.invN;
     double D[CH];
     tot = 0.0;
 #pragma unroll
     for (int k = 0; k < CH; k++) {
-        int i = base + k;
+        const int i = base + k;
         D[k] = (i == 0 || i >= N) ? 0.0 : d0 - S[k];
         tot += D[k];
     }
-    run = block_scan(tot, L.part, buf, nullptr);
+    run = block_excl_scan1(tot, L.part, buf);
+    double y[CH];
 #pragma unroll
     for (int k = 0; k < CH; k++) {
         run += D[k];
-        if (base + k < N) L.v[base + k] = run;
+        y[k] = run;
     }
+    // (every thread's reads of L.v before this solve came before its first
+    // barrier above)
+    put_periodic<CH>(L, y, N, base);
     __syncthreads();
-    // the thread's CH entries share one window of CH + 32 taps:
-    // w[q] = y[base - 16 + q], periodic (below 34 entries the taps can wrap
-    // more than once)
-    constexpr int WN = CH + 2 * (NG - 1);
     double cp[CH];
     if (base < N) {
-        double w[WN];
-        if (N >= 2 * NG) {
+        const double rho = P.rho;
+        double fw[CH], bw[CH];
+        double u = L.at(base - VH);
 #pragma unroll
-            for (int q = 0; q < WN; q++) {
-                int u = base - (NG - 1) + q;
-                u = u < 0 ? u + N : (u >= N ? u - N : u);
-                w[q] = L.v[u];
-            }
-        } else {
-#pragma unroll
-            for (int q = 0; q < WN; q++) {
-                int u = base - (NG - 1) + q;
-                while (u >= N) u -= N;
-                while (u < 0) u += N;
-                w[q] = L.v[u];
-            }
-        }
+        for (int m = 1; m < VH; m++) u = fma(rho, u, L.at(base - VH + m));
 #pragma unroll
         for (int k = 0; k < CH; k++) {
-            double v = P.g[0] * w[NG - 1 + k];
-#pragma unroll
-            for (int m = 1; m < NG; m++) v += P.g[m] * (w[NG - 1 + k + m] + w[NG - 1 + k - m]);
-            cp[k] = base + k < N ? v : 0.0;
+            u = fma(rho, u, L.at(base + k));
+            fw[k] = u;
         }
+        double b = L.at(base + CH - 1 + VH);
+#pragma unroll
+        for (int m = VH - 1; m >= 1; m--) b = fma(rho, b, L.at(base + CH - 1 + m));
+#pragma unroll
+        for (int k = CH - 1; k >= 0; k--) {
+            b = fma(rho, b, L.at(base + k));
+            bw[k] = b;
+        }
+#pragma unroll
+        for (int k = 0; k < CH; k++) cp[k] = base + k < N ? P.C * ((fw[k] + bw[k]) - y[k]) : 0.0;
     } else {
 #pragma unroll
         for (int k = 0; k < CH; k++) cp[k] = 0.0;
     }
-    // row 0 needs c at entries 0, 1, 2, N-2 and N-1
+    // row 0 needs c at entries 0, 1, 2, N-2 and N-1 (N >= 8: distinct)
 #pragma unroll
     for (int k = 0; k < CH; k++) {
         const int i = base + k;
-        if (i == 0) L.edge[0] = cp[k];
-        if (i == 1) L.edge[1] = cp[k];
-        if (i == 2) L.edge[2] = cp[k];
-        if (i == N - 2) L.edge[3] = cp[k];
-        if (i == N - 1) L.edge[4] = cp[k];
+        const int e = i < 3 ? i : (i == N - 2 ? 3 : (i == N - 1 ? 4 : -1));
+        if (e >= 0) L.edge[e] = cp[k];
     }
     __syncthreads();
     const double c0 = L.edge[0];
@@ -479,94 +534,132 @@ __device__ void approx_small(const PoissonArgs &P, const double *in, double *out
 // coefficients (rows 1..N-1: the periodic stencil without column 0; row 0:
 // x_0 + w1 x_1 + w2 x_2 + w2 x_{N-2} + w1 x_{N-1})
 template <int CH>
-__device__ void residual_small(const PoissonArgs &P, const double *b, const double *x, double *r, SmallLds &L) {
+__device__ void residual_one(const PoissonArgs &P, const double *b, const double *x, double *r, SmallLds &L) {
     const int N = P.fs.n_f;
     const int base = threadIdx.x * CH;
-#pragma unroll
-    for (int k = 0; k < CH; k++)
-        if (base + k < N) L.v[base + k] = x[k];
+    put_periodic<CH>(L, x, N, base);
     __syncthreads();
-    const double *xb = L.v;
+    // x at entries base - 2 .. base + CH + 1, read at once (inside the LDS
+    // array for every thread; entries >= N are not used)
+    double w[CH + 4];
+#pragma unroll
+    for (int m = 0; m < CH + 4; m++) w[m] = L.at(base - 2 + m);
+    auto sub = [](double &sh, double &sl, double a, double xq) {
+        const double p = a * xq;
+        const double e = fma(a, xq, -p);
+        const double tt = sh - p;
+        const double bv = tt - sh;
+        const double err = (sh - (tt - bv)) + (-p - bv);
+        sl = sl + (err - e);
+        sh = tt;
+    };
+    const double cf[5] = {P.w2, P.w1, P.w3, P.w1, P.w2};
 #pragma unroll
     for (int k = 0; k < CH; k++) {
-        int i = base + k;
-        if (i >= N) {
-            r[k] = 0.0;
-            continue;
-        }
+        const int i = base + k;
+        // rows i >= 1: neighbours j = i + d in [-1, N + 1]; column 0 (j = 0 or
+        // j = N) gets a zero coefficient, which changes neither sum
         double sh = b[k], sl = 0.0;
-        auto sub = [&](double a, double xv) {
-            double p = a * xv;
-            double e = fma(a, xv, -p);
-            double tt = sh - p;
-            double bv = tt - sh;
-            double err = (sh - (tt - bv)) + (-p - bv);
-            sl = sl + (err - e);
-            sh = tt;
-        };
-        if (i == 0) {
-            sub(1.0, xb[0]); sub(P.w1, xb[1]); sub(P.w2, xb[2]); sub(P.w2, xb[N - 2]); sub(P.w1, xb[N - 1]);
-        } else {
-            const double cf[5] = {P.w2, P.w1, P.w3, P.w1, P.w2};
-            for (int d = -2; d <= 2; d++) {
-                int jj = i + d;
-                jj = jj < 0 ? jj + N : (jj >= N ? jj - N : jj);
-                if (jj == 0) continue;
-                sub(cf[d + 2], xb[jj]);
-            }
+#pragma unroll
+        for (int d = -2; d <= 2; d++) {
+            const int j = i + d;
+            sub(sh, sl, (j == 0 || j == N) ? 0.0 : cf[d + 2], w[k + d + 2]);
         }
-        r[k] = sh + sl;
+        r[k] = i < N ? sh + sl : 0.0;
+    }
+    if (base == 0) {  // row 0
+        double sh = b[0], sl = 0.0;
+        sub(sh, sl, 1.0, L.at(0));
+        sub(sh, sl, P.w1, L.at(1));
+        sub(sh, sl, P.w2, L.at(2));
+        sub(sh, sl, P.w2, L.at(N - 2));
+        sub(sh, sl, P.w1, L.at(N - 1));
+        r[0] = sh + sl;
     }
 }
 
 template <int CH>
-__global__ __launch_bounds__(PT) void k_poisson_small(PoissonArgs P) {
+__global__ __launch_bounds__(PT1) void k_poisson_one(PoissonArgs P) {
     const vx_field_state &F = P.fs;
     const int N = F.n_f;
     const int t = threadIdx.x, base = t * CH;
     __shared__ SmallLds L;
     int buf = 0;
+    // the entries past the last image, where the last thread's backward
+    // recurrence may start, hold 0 (never written again)
+    if (t < 8) L.at(N + VH + t) = 0.0;
     double b[CH], x[CH], r[CH], c[CH];
     const double te = vx_eps0_inv;
+    // the right-hand side formed at entries t + PT1 m (coalesced loads, at
+    // clamped indices, all in flight at once) and handed to the owners of
+    // consecutive entries through the LDS copy
+    double chv[CH], nev[CH];
+#pragma unroll
+    for (int m = 0; m < CH; m++) {
+        const int i = min(t + PT1 * m, N - 1);
+        chv[m] = F.charge[i];
+        nev[m] = F.neutral[i];
+    }
+    // the entry Ex0 (written back at the end) as a vector load behind them,
+    // so that its latency overlaps theirs (a scalar load would be waited for
+    // before them)
+    const double ex0 = __builtin_bit_cast(
+        double, __builtin_amdgcn_raw_buffer_load_b64(
+                    __builtin_amdgcn_make_buffer_rsrc(F.Ex0, 0, 8, 0x00020000), 0, 0, 0));
+#pragma unroll
+    for (int m = 0; m < CH; m++) {
+        const int i = t + PT1 * m;
+        double v = te * (chv[m] + nev[m]);
+        v *= F.dx2;
+        L.at(i < N ? i : VDUMMY) = v;  // no branch around the loads' use
+    }
+    __syncthreads();
+    // (these reads come before the first barrier of the first solve, which
+    // writes the LDS copy only after it)
 #pragma unroll
     for (int k = 0; k < CH; k++) {
-        int i = base + k;
-        double v = 0.0;
-        if (i < N) {
-            v = te * (F.charge[i] + F.neutral[i]);
-            v *= F.dx2;
-        }
-        b[k] = v;
+        b[k] = base + k < N ? L.at(base + k) : 0.0;
+        x[k] = 0.0;
     }
-    approx_small<CH>(P, b, x, L, buf);
-    for (int it = 0; it < 3; it++) {
-        // the previous solve's last barrier ordered its reads of L.v
-        residual_small<CH>(P, b, x, r, L);
+    // the approximate solve of b, then three rounds of refinement (residual r
+    // of x, x += the approximate solve of r), as one loop: the kernel runs
+    // once per field solve and its code is fetched cold, so one copy of each
+    // step is fetched once and then reused
+#pragma nounroll
+    for (int it = 0; it < 4; it++) {
+        if (it > 0) {
+            // the previous solve's last barrier ordered its reads of L.v
+            residual_one<CH>(P, b, x, r, L);
+        } else {
+#pragma unroll
+            for (int k = 0; k < CH; k++) r[k] = b[k];
+        }
         // the first barrier of the solve below orders the residual's reads of
         // L.v before the solve writes it
-        approx_small<CH>(P, r, c, L, buf);
+        approx_one<CH>(P, r, c, L, buf);
 #pragma unroll
-        for (int k = 0; k < CH; k++) x[k] = x[k] + c[k];
+        for (int k = 0; k < CH; k++) x[k] = it == 0 ? c[k] : x[k] + c[k];
     }
-    // E from the final potential: every thread's reads of L.v in the last
-    // solve came before its final barrier
-#pragma unroll
-    for (int k = 0; k < CH; k++)
-        if (base + k < N) {
-            L.v[base + k] = x[k];
-            F.phi[base + k] = x[k];
-        }
+    // phi and E from the final potential (EMSolver.cpp:137-154, 191), stored
+    // at entries t + PT1 m (coalesced) from the LDS copy: every thread's reads
+    // of it in the last solve came before its final barrier
+    put_periodic<CH>(L, x, N, base);
     __syncthreads();
-    const double *xb = L.v;
-    double ex0 = *F.Ex0;
-    double e_m1 = efield(xb, N, F.field_coef, ex0, -1);
-    double e_0 = efield(xb, N, F.field_coef, ex0, 0);
-    double ex0n = ex0 + -(e_m1 + e_0) * 0.5;
-    __syncthreads();
+    const double fc = F.field_coef;
+    auto ef = [&](double e0, double pm2, double pm1, double pp1, double pp2) {
+        return -fc * (8 * (pp1 - pm1) - pp2 + pm2) + e0;
+    };
+    const double e_m1 = ef(ex0, L.at(-3), L.at(-2), L.at(0), L.at(1)), e_0 = ef(ex0, L.at(-2), L.at(-1), L.at(1), L.at(2));
+    const double ex0n = ex0 + -(e_m1 + e_0) * 0.5;
     if (t == 0) *F.Ex0 = ex0n;
 #pragma unroll
-    for (int k = 0; k < CH; k++)
-        if (base + k < N) F.E[base + k] = efield(xb, N, F.field_coef, ex0n, base + k);
+    for (int m = 0; m < CH; m++) {
+        const int i = t + PT1 * m;
+        if (i < N) {
+            F.phi[i] = L.at(i);
+            F.E[i] = ef(ex0n, L.at(i - 2), L.at(i - 1), L.at(i + 1), L.at(i + 2));
+        }
+    }
 }
 
 // ---------------------------------------------------------------- Maxwell
@@ -733,13 +826,16 @@ static int poisson(vx_ctx *ctx, const vx_field_state *fs, double *work) {
         P.g[k] = C * pw;
         pw *= rho;
     }
+    P.rho = rho;
+    P.C = C;
     P.invN = 1.0 / fs->n_f;
     LaunchTimer lt(ctx, "k_poisson");
-    int ch = (fs->n_f + PT - 1) / PT;
+    const int ch = (fs->n_f + PT1 - 1) / PT1;
     if (fs->n_f <= SMALL_MAX) {
-        if (ch <= 1) k_poisson_small<1><<<1, PT, 0, ctx->stream>>>(P);
-        else if (ch <= 2) k_poisson_small<2><<<1, PT, 0, ctx->stream>>>(P);
-        else k_poisson_small<4><<<1, PT, 0, ctx->stream>>>(P);
+        if (ch <= 1) k_poisson_one<1><<<1, PT1, 0, ctx->stream>>>(P);
+        else if (ch <= 2) k_poisson_one<2><<<1, PT1, 0, ctx->stream>>>(P);
+        else if (ch <= 4) k_poisson_one<4><<<1, PT1, 0, ctx->stream>>>(P);
+        else k_poisson_one<8><<<1, PT1, 0, ctx->stream>>>(P);
         return check_launch(ctx, "k_poisson");
     }
     PoissonMB M;
