@@ -155,20 +155,23 @@ def test_regrid_to_the_same_hierarchy_keeps_the_patches(amr_pair):
     compare_fields(g, o, "after an unchanged regrid + 3 steps")
 
 
-def test_moment_forms_agree_on_every_level():
+@pytest.mark.parametrize("np_", [64, 256])
+def test_moment_forms_agree_on_every_level(np_):
     """The p-moments in every form (the split form of large launches, the
     producer/consumer pairs of small ones and the wide form of relativeToBottom
     1 patches, pinned with VX_MOMENTS_FORM) on the three-level case, whose
-    patches have relativeToBottom 1, 2 and 4, and whose finest columns have
-    the laser's A^2 (read from a logarithm table or formed per column): the
-    same bits, and the oracle's."""
+    patches have relativeToBottom 1, 2 and 4, and whose columns have the
+    laser's A^2 (read from a logarithm table or formed per column): the same
+    bits, and the oracle's.  With 256 base p-cells every level's columns span
+    several 64-cell chunks, whose logarithm terms carry the last two of the
+    chunk before."""
     import os
-    p = two_slab_laser()
+    p = two_slab_laser(np=np_)
     o = OracleRun(p)
     dt = o.calc_dt(0.4)
     runs = {}
     for form in ("split", "pair", "wide"):
-        say(f"case: moments form {form}, 3 levels, 3 steps")
+        say(f"case: moments form {form}, 3 levels, np {np_}, 3 steps")
         os.environ["VX_MOMENTS_FORM"] = form
         try:
             g = va.Run(p, fused=False)

@@ -357,6 +357,42 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
         // the ordered sums of the (absent) chunk before the first add -0.0
         for (int e = lane; e < 2 * NCH * 65; e += 64) (&T[FUSE ? 1 : 0][wave][0][0][0])[e] = -0.0;
     }
+    // Logarithm terms of chain ch for chunk jb: Lb[wave][ch][e] = L(jb + e),
+    // e in [0, 66), with L(m) = c2 log(h(m) / h(m-1)) and h(m) = Gamma(p_m) + c1 p_m
+    // for the chain's A^2.  The first chunk forms h at all 67 p-cells it needs;
+    // each later one takes L(jb), L(jb + 1) from the previous chunk's entries 64
+    // and 65 and h(jb + 1) from hcar, so that its 64 lanes form one h and one
+    // logarithm each (h(m - 1) from the next lower lane): the same operands, so
+    // the same bits, for half the arithmetic.
+    double hcar[NCH];  // h(jb + 65) of the last chunk, per chain (uniform)
+    auto log_row = [&](int jb, int ch) {
+        double *Lrow = &Lb[wave][ch][0];
+        if (jb == 0) {
+            for (int e = lane; e < 67; e += 64) {
+                double pm = momentum(P.pmin, P.dp, jb - 1 + e, P.p_pos);
+                H[wave][e] = gamma_(pm, asq[ch], P.mc2inv) + A.c1 * pm;
+            }
+            wave_sync();
+            for (int e = lane; e < 66; e += 64) Lrow[e] = A.c2 * vx_log(H[wave][e + 1] / H[wave][e]);
+            hcar[ch] = H[wave][66];
+            wave_sync();
+            return;
+        }
+        const double pm = momentum(P.pmin, P.dp, jb + 2 + lane, P.p_pos);
+        const double hm = gamma_(pm, asq[ch], P.mc2inv) + A.c1 * pm;
+        // the lane shift runs with every lane active (a DPP move from a lane
+        // outside the mask reads 0), then lane 0 takes the carried h
+        const double hs = shup1(hm);
+        const double hp = lane == 0 ? hcar[ch] : hs;
+        const double lv = A.c2 * vx_log(hm / hp);
+        const double keep = Lrow[64 + min(lane, 1)];  // L(jb), L(jb + 1) for lanes 0, 1
+        wave_sync();
+        if (lane < 2) Lrow[lane] = keep;
+        Lrow[2 + lane] = lv;
+        hcar[ch] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(hm), 63),
+                                    __builtin_amdgcn_readlane(__double2loint(hm), 63));
+        wave_sync();
+    };
     // chunk jb from pr (summand buffer bb), whose registers then take chunk jb + 128
     auto chunk = [&](int jb, int bb, Pre &pr) {
         for (int ch = 0; ch < NCH; ch++) {
@@ -376,13 +412,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
                 for (int e = lane; e < 66; e += 64) Lb[wave][ch][e] = Lb[wave][ch - 1][e];
                 continue;
             }
-            for (int e = lane; e < 67; e += 64) {
-                double pm = momentum(P.pmin, P.dp, jb - 1 + e, P.p_pos);
-                H[wave][e] = gamma_(pm, asq[ch], P.mc2inv) + A.c1 * pm;
-            }
-            wave_sync();
-            for (int e = lane; e < 66; e += 64) Lb[wave][ch][e] = A.c2 * vx_log(H[wave][e + 1] / H[wave][e]);
-            wave_sync();
+            log_row(jb, ch);
         }
         wave_sync();
         if (COMP) {
@@ -437,13 +467,7 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
                     for (int e = lane; e < 66; e += 64) Lb[0][ch][e] = Lb[0][ch - 1][e];
                     continue;
                 }
-                for (int e = lane; e < 67; e += 64) {
-                    double pm = momentum(P.pmin, P.dp, jb - 1 + e, P.p_pos);
-                    H[0][e] = gamma_(pm, asq[ch], P.mc2inv) + A.c1 * pm;
-                }
-                wave_sync();
-                for (int e = lane; e < 66; e += 64) Lb[0][ch][e] = A.c2 * vx_log(H[0][e + 1] / H[0][e]);
-                wave_sync();
+                log_row(jb, ch);  // wave is 0 in the pair form
             }
             wave_sync();
             summands(jb, bb, nflag);
@@ -548,7 +572,10 @@ __device__ __forceinline__ double buf_load(__amdgpu_buffer_rsrc_t r, int voff, i
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
-template <int WDEPTH>
+// COMP (mixed-precision form): no consumer; each producer keeps its lanes'
+// compensated (TwoSum) sums over the chunks and ends with one compensated
+// reduction across its lanes, as the compensated k_moments form does.
+template <int WDEPTH, bool COMP>
 __global__ __launch_bounds__(64 * (WCOL + 1)) void k_moments_wide(MomArgs A) {
     const vx_patch &P = A.p;
     const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)), lane = (int)(threadIdx.x & 63);
@@ -594,6 +621,7 @@ __global__ __launch_bounds__(64 * (WCOL + 1)) void k_moments_wide(MomArgs A) {
         // and its ring of chunks in flight is unrolled: slot d holds chunk
         // k0 + d in fixed registers and is reloaded with chunk k0 + d + WDEPTH
         // once used (a rotating copy would wait for the loads in flight).
+        double aR = 0.0, cR = 0.0, aJ = 0.0, cJ = 0.0;  // COMP: the lanes' sums
         auto run = [&](auto tab_c) __attribute__((always_inline)) {
             constexpr bool TAB = decltype(tab_c)::value;
             double pv[WDEPTH], pt[WDEPTH];
@@ -635,17 +663,31 @@ __global__ __launch_bounds__(64 * (WCOL + 1)) void k_moments_wide(MomArgs A) {
                     const double tm = pv[d];
                     const bool skip = (jb + lane >= P.n_p) | !col_ok;
                     const double jv = t0 * gm + A.c3 * (gp1 - gm1) * (tp - tm);
-                    T[k & 1][0][wave][lane] = skip ? -0.0 : t0;
-                    T[k & 1][1][wave][lane] = skip ? -0.0 : jv;
+                    if (COMP) {
+                        add_two(aR, cR, skip ? -0.0 : t0);
+                        add_two(aJ, cJ, skip ? -0.0 : jv);
+                    } else {
+                        T[k & 1][0][wave][lane] = skip ? -0.0 : t0;
+                        T[k & 1][1][wave][lane] = skip ? -0.0 : jv;
+                    }
                     load(k + WDEPTH, d);
                     __builtin_amdgcn_sched_barrier(0);
-                    __syncthreads();
+                    if (!COMP) __syncthreads();
                 }
             }
         };
         if (tab) run(std::true_type{});
         else run(std::false_type{});
-    } else {
+        if (COMP) {
+            double sr, er, sj, ej;
+            wave_two_sum(aR, cR, sr, er);
+            wave_two_sum(aJ, cJ, sj, ej);
+            if (lane == 0 && col_ok) {
+                P.chargeR[i] = (sr + er) * A.dpq;
+                P.currentR[i] = (sj + ej) * A.jscale;
+            }
+        }
+    } else if (!COMP) {
         // consumer: chain (lane & 31) of charge (lanes < 32) or current; lanes
         // past WCOL repeat chain 0 and are not stored
         // the dependent additions are the critical path of each chunk: the
@@ -817,21 +859,26 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
     const unsigned g1 = (unsigned)((p->n_x + 15) / 16), g2 = (unsigned)((p->n_x + 7) / 8),
                    g4 = (unsigned)((p->n_x + 3) / 4);
     // wavefronts of the launch (4 sub-columns each) against the device's SIMDs;
-    // VX_MOMENTS_FORM=split, =pair or =wide pins the form (all give the same
-    // bits); the wide form serves relativeToBottom 1 patches without nested
-    // cells (the finest level's) by default
+    // VX_MOMENTS_FORM=split, =pair or =wide pins the form (the ordered forms
+    // all give the same bits); the wide form serves relativeToBottom 1 patches
+    // without nested cells (the finest level's) and with at least 512 p-cells
+    // by default, ordered or compensated
     bool fuse = (size_t)(p->n_x * p->rtb + 3) / 4 <= (size_t)4 * ctx->n_cu;
-    bool wide = p->rtb == 1 && !comp && !p->nested;
+    // (the wide form runs a column's chunks in lockstep with its workgroup's
+    // barriers: for short columns the split and pair forms are faster)
+    bool wide = p->rtb == 1 && !p->nested && p->n_p >= 512;
     if (const char *form = std::getenv("VX_MOMENTS_FORM")) {
         if (std::strcmp(form, "split") == 0) fuse = wide = false;
         if (std::strcmp(form, "pair") == 0) {
             fuse = true;
             wide = false;
         }
+        if (std::strcmp(form, "wide") == 0) wide = p->rtb == 1 && !p->nested;
     }
     if (wide) {
         const unsigned gw = (unsigned)((p->n_x + WCOL - 1) / WCOL);
-        k_moments_wide<4><<<gw, 64 * (WCOL + 1), 0, ctx->stream>>>(A);
+        if (comp) k_moments_wide<4, true><<<gw, 64 * WCOL, 0, ctx->stream>>>(A);
+        else k_moments_wide<4, false><<<gw, 64 * (WCOL + 1), 0, ctx->stream>>>(A);
         return check_launch(ctx, "k_moments_wide");
     }
     // the pair form: one workgroup of two wavefronts per group of 4 sub-columns
