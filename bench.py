@@ -32,9 +32,11 @@ def kernel_bytes_per_cell(kernel, step):
     through its interface, each array touched once."""
     s = step
     if kernel == "k_sub0":
-        # read f_s, write FxH_s/FpH_s and FxDS/FpDS, read FxH/FpH of stages < s,
-        # FxL0/FpL0 written at stage 0 and read afterwards
-        return 8 + 16 + 16 + 16 * s + 16
+        # read f_s, write FxH_s/FpH_s and FxDS/FpDS, read FxH/FpH of stages < s
+        # (at the last stage the partial-sum plane per direction that stage 4
+        # writes beside its own fluxes), FxL0/FpL0 written at stage 0 and read
+        # afterwards
+        return 8 + 16 + 16 + (16 * s if s < 5 else 16) + (16 if s == 4 else 0) + 16
     if kernel == "k_stage":
         # store-flux fused stage: read f_s and f^n, read FxH/FpH of stages < s
         # (at the last stage one partial-sum plane per direction, which stage 4

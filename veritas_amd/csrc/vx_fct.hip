@@ -642,29 +642,42 @@ int tune_policy(vx_ctx *ctx, vx_patch_table *t, int kind, bool mp, const std::ve
     }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int best = pol, rc = VX_OK;
-    float best_ms = 1e30f;
     if (hipEventCreate(&e0) == hipSuccess && hipEventCreate(&e1) == hipSuccess) {
         bool failed = false;
-        for (int c : cands) {
+        // one launch of policy c, timed on its own
+        auto timed = [&](int c) {
+            float tm = 1e30f;
             pol = c;
-            if ((rc = table_rebuild(ctx, t))) break;
-            launch();  // warm
-            // the fastest of three launches, each timed on its own
-            float ms = 1e30f;
-            for (int rep = 0; rep < 3 && !failed; rep++) {
-                float tm = 0.0f;
-                if (hipEventRecord(e0, ctx->stream) != hipSuccess) failed = true;
-                if (!failed) launch();
-                if (!failed && (hipEventRecord(e1, ctx->stream) != hipSuccess ||
-                                hipEventSynchronize(e1) != hipSuccess ||
-                                hipEventElapsedTime(&tm, e0, e1) != hipSuccess))
-                    failed = true;
-                if (!failed) ms = std::min(ms, tm);
-            }
-            if (failed) break;
-            if (ms < 0.97f * best_ms) {
-                best_ms = ms;
-                best = c;
+            if ((rc = table_rebuild(ctx, t))) return tm;
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) failed = true;
+            if (!failed) launch();
+            if (!failed && (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&tm, e0, e1) != hipSuccess))
+                failed = true;
+            return tm;
+        };
+        // one warm-up launch, one timed launch of each candidate, then the two
+        // fastest timed twice more, keeping their fastest launch; among those,
+        // in candidate order, a later one must be more than 3 % faster to win
+        pol = cands.empty() ? pol : cands[0];
+        if (!(rc = table_rebuild(ctx, t))) launch();  // warm
+        std::vector<float> ms(cands.size(), 1e30f);
+        for (size_t k = 0; k < cands.size() && !failed && !rc; k++) ms[k] = timed(cands[k]);
+        if (!failed && !rc && !cands.empty()) {
+            std::vector<int> order(cands.size());
+            for (size_t k = 0; k < cands.size(); k++) order[k] = (int)k;
+            std::sort(order.begin(), order.end(), [&](int a, int b) { return ms[a] < ms[b]; });
+            std::vector<int> fin(order.begin(), order.begin() + std::min<size_t>(2, order.size()));
+            for (int k : fin)
+                for (int rep = 0; rep < 2 && !failed && !rc; rep++) ms[k] = std::min(ms[k], timed(cands[k]));
+            if (!failed && !rc) {
+                std::sort(fin.begin(), fin.end());  // candidate order
+                float best_ms = 1e30f;
+                for (int k : fin)
+                    if (ms[k] < 0.97f * best_ms) {
+                        best_ms = ms[k];
+                        best = cands[k];
+                    }
             }
         }
     }
