@@ -720,28 +720,24 @@ __global__ void k_maxwell_rhs(MaxArgs A) {
     By[s] = rBy; Bz[s] = rBz; Ey[s] = rEy; Ez[s] = rEz; Ay[s] = rAy; Az[s] = rAz;
 }
 
-// EMSolver.cpp:204-338
+// EMSolver.cpp:204-338; one thread per element and field (blockIdx.y): the
+// six fields' updates are independent
 __global__ void k_maxwell_update(MaxArgs A) {
     const vx_field_state &F = A.fs;
     const int ne = F.n_e;
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= ne) return;
-    double *Fs[6] = {F.By, F.Bz, F.Ey, F.Ez, F.Ay, F.Az};
+    const int v = blockIdx.y;
+    double *y = v == 0 ? F.By : v == 1 ? F.Bz : v == 2 ? F.Ey : v == 3 ? F.Ez : v == 4 ? F.Ay : F.Az;
     if (A.step <= 4) {
-        for (int v = 0; v < 6; v++) {
-            double *y = Fs[v];
-            double t = y[fidx(ne, i, 0)] + A.a[0] * y[fidx(ne, i, 2)];
-            for (int k = 1; k <= A.step; k++) t = t + A.a[k] * y[fidx(ne, i, k + 2)];
-            y[fidx(ne, i, 1)] = t;
-        }
+        double t = y[fidx(ne, i, 0)] + A.a[0] * y[fidx(ne, i, 2)];
+        for (int k = 1; k <= A.step; k++) t = t + A.a[k] * y[fidx(ne, i, k + 2)];
+        y[fidx(ne, i, 1)] = t;
     } else {
-        for (int v = 0; v < 6; v++) {
-            double *y = Fs[v];
-            double t = y[fidx(ne, i, 1)] + A.b[0] * y[fidx(ne, i, 2)];
-            for (int k = 1; k < 6; k++) t = t + A.b[k] * y[fidx(ne, i, k + 2)];
-            y[fidx(ne, i, 0)] = t;
-            y[fidx(ne, i, 1)] = t;
-        }
+        double t = y[fidx(ne, i, 1)] + A.b[0] * y[fidx(ne, i, 2)];
+        for (int k = 1; k < 6; k++) t = t + A.b[k] * y[fidx(ne, i, k + 2)];
+        y[fidx(ne, i, 0)] = t;
+        y[fidx(ne, i, 1)] = t;
     }
 }
 
@@ -909,7 +905,7 @@ static int maxwell_stage(vx_ctx *ctx, const vx_field_state *fs, int step, double
     unsigned g = (unsigned)((fs->n_e + 255) / 256);
     k_maxwell_rhs<<<g, 256, 0, ctx->stream>>>(A);
     if ((rc = check_launch(ctx, "k_maxwell_rhs"))) return rc;
-    k_maxwell_update<<<g, 256, 0, ctx->stream>>>(A);
+    k_maxwell_update<<<dim3(g, 6), 256, 0, ctx->stream>>>(A);
     if ((rc = check_launch(ctx, "k_maxwell_update"))) return rc;
     k_faces<<<(unsigned)((fs->n_f + 255) / 256), 256, 0, ctx->stream>>>(*fs);
     return check_launch(ctx, "k_faces");
