@@ -409,8 +409,11 @@ __device__ __forceinline__ void stage_tile(const StageArgs &A, const int tile) {
     else stage_strip<STEP, MP, false>(A, jw, i0);
 }
 
+// Compiled for at least three resident waves per SIMD (at most 168 VGPRs):
+// the FP64 stages fit as they are; the mixed-precision stage 4 (173 otherwise)
+// keeps a few values in scratch instead of running two waves per SIMD.
 template <int STEP, bool MP>
-__global__ __launch_bounds__(256) void k_stage(StageArgs A) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_stage(StageArgs A) {
     stage_tile<STEP, MP>(A, xcd_tile());
 }
 // The same stage compiled for four resident waves per SIMD (at most 128
@@ -424,9 +427,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
 
 // Several patches (of independent runs) at the same RK stage in one launch:
 // workgroups [wg_start[k], wg_start[k+1]) of the XCD-ordered tile sequence
-// belong to item k.
+// belong to item k.  Held to three waves per SIMD like k_stage (stages 3 and
+// 4 would take 185-206 VGPRs, two waves).
 template <int STEP, bool MP>
-__global__ __launch_bounds__(256) void k_stage_batch(const StageArgs *items, const int *wg_start, int n) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_stage_batch(const StageArgs *items, const int *wg_start, int n) {
     const int tile = xcd_tile();
     int lo = 0, hi = n - 1;  // the last k with wg_start[k] <= tile (uniform)
     while (lo < hi) {
