@@ -145,7 +145,7 @@ __device__ __forceinline__ void add_two(double &acc, double &cmp, double v) {
 // instruction streams, which hides the latency of each (for launches of at
 // most one group per SIMD).
 template <int RTB, int C, bool COMP, bool FUSE, bool PAIR = false>
-__global__ __launch_bounds__(256) void k_moments(MomArgs A) {
+__device__ __forceinline__ void moments_body(MomArgs A) {
     constexpr int NCH = C * RTB;
     static_assert(NCH <= 32, "chains must fit in half a wavefront");
     const vx_patch &P = A.p;
@@ -535,6 +535,18 @@ __global__ __launch_bounds__(256) void k_moments(MomArgs A) {
     }
 }
 
+template <int RTB, int C, bool COMP, bool FUSE, bool PAIR = false>
+__global__ __launch_bounds__(256) void k_moments(MomArgs A) {
+    moments_body<RTB, C, COMP, FUSE, PAIR>(A);
+}
+// The same kernel compiled for four resident waves per SIMD (at most 128
+// VGPRs): the split form of relativeToBottom-4 patches fits without spilling
+// (137 VGPRs, three waves, otherwise); the other forms would spill.
+template <int RTB, int C, bool COMP, bool FUSE, bool PAIR = false>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_moments_w4(MomArgs A) {
+    moments_body<RTB, C, COMP, FUSE, PAIR>(A);
+}
+
 // Wide form for relativeToBottom 1 (one chain per column and moment): a
 // workgroup owns WCOL consecutive x-columns.  Producer wavefront c stages the
 // summands of column c for one 64-cell p-chunk at a time in LDS (the same
@@ -891,7 +903,7 @@ static int moments_patch(vx_ctx *ctx, const vx_patch *p, const vx_fields *fl, bo
         case 8: k_moments<2, 2, false, false><<<g2, 256, 0, ctx->stream>>>(A); break;
         case 9: k_moments<2, 2, false, false, true><<<gp2, 128, 0, ctx->stream>>>(A); break;
         case 10: case 11: k_moments<2, 2, true, false><<<g2, 256, 0, ctx->stream>>>(A); break;
-        case 16: k_moments<4, 1, false, false><<<g4, 256, 0, ctx->stream>>>(A); break;
+        case 16: k_moments_w4<4, 1, false, false><<<g4, 256, 0, ctx->stream>>>(A); break;
         case 17: k_moments<4, 1, false, false, true><<<gp4, 128, 0, ctx->stream>>>(A); break;
         case 18: case 19: k_moments<4, 1, true, false><<<g4, 256, 0, ctx->stream>>>(A); break;
         default: return fail(ctx, VX_ENOTSUP, "moments: relativeToBottom %d not in {1,2,4}", p->rtb);
