@@ -97,10 +97,13 @@ def test_two_stream_full_size_against_fp64():
     r.close()
 
 
-def test_compensated_moments_agree_with_ordered_sums():
+@pytest.mark.parametrize("nx,np_", [(256, 1024), (130, 600)])
+def test_compensated_moments_agree_with_ordered_sums(nx, np_):
     """Same state, both reductions: charge and current agree to a few ulps of
-    the largest entry (they differ only in summation order and rounding)."""
-    p = two_stream(nx=256, np=1024, length=4 * math.pi, amplitude=0.05)
+    the largest entry (they differ only in summation order and rounding).
+    130 x 600 leaves a partial group of columns and a partial last chunk of
+    p-cells in the wide form of both reductions."""
+    p = two_stream(nx=nx, np=np_, length=4 * math.pi, amplitude=0.05)
     a = va.Run(p)
     a.stage_moments()
     q0, j0 = a.get("charge"), a.get("J")

@@ -180,12 +180,14 @@ def test_large_finest_grid_poisson_path(nx):
 
 
 @pytest.mark.parametrize("nx,np_,fused", [(8, 38, True), (8, 38, False), (130, 60, True), (130, 60, False),
-                                          (34, 300, True), (8, 8, True), (8, 8, False), (510, 62, True)])
+                                          (34, 300, True), (8, 8, True), (8, 8, False), (510, 62, True),
+                                          (130, 600, True)])
 def test_ragged_and_tiny_shapes(nx, np_, fused):
     """Shapes that leave partial strips, partial 58-cell lanes, single strips
     and tiny planes (edge cases of the tile mapping; the reference requires
     sizes divisible by the refinement ratio, Settings.cpp:35-50): 6 steps
-    bitwise."""
+    bitwise.  130 x 600 takes the wide moments form (600 p-cells) with a
+    partial group of columns and a partial last chunk."""
     p = two_stream(nx=nx, np=np_, length=4 * math.pi, amplitude=0.05)
     g, o = va.Run(p, fused=fused), OracleRun(p)
     dt = o.calc_dt()
