@@ -9,6 +9,7 @@
 // task, one launch runs one batch.  These are O(patch perimeter) operations;
 // the per-cell arithmetic keeps the reference's expressions so that the
 // results match the CPU oracle bit for bit.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -20,11 +21,25 @@ using namespace vxi;
 using namespace vxd;
 
 
+// The RESTRICT tasks of a large batch that tile a rectangle of coarse cells
+// (dst cells [i0, i0 + ni) x [j0, j0 + nj), coarse coordinates (di + oi,
+// dj + oj)) run as that rectangle (k_restrict_rects): one thread per cell, as
+// the tasks would, without reading a task per cell.
+struct vx_restrict_rect {
+    int32_t dst, src, i0, j0, ni, nj, oi, oj;
+    int32_t blk0;  // first workgroup of the rectangle in its batch's launch
+    int32_t pad;
+};
+
 struct vx_program {
     vx_ctx *ctx = nullptr;
-    vx_cell_task *d = nullptr;
-    int32_t *d_batch = nullptr;   // the batch starts on the device (k_program_seq)
-    std::vector<int32_t> batch;   // host copy of the batch starts
+    vx_cell_task *d = nullptr;     // the tasks that run as cell tasks
+    int32_t *d_batch = nullptr;    // their batch starts on the device (k_program_seq)
+    std::vector<int32_t> batch;    // host copy of the batch starts of all tasks
+    std::vector<int32_t> cbatch;   // batch starts in d
+    std::vector<int32_t> rbatch;   // first rectangle of each batch (n_batches + 1)
+    std::vector<int32_t> rblocks;  // workgroups of each batch's rectangle launch
+    vx_restrict_rect *d_rect = nullptr;
     std::vector<vx_cell_task> host;
     uint64_t checked_serial = 0;  // table this program was bounds-checked against
     int checked_r = 0;
@@ -279,6 +294,35 @@ __global__ __launch_bounds__(256) void k_program(ProgArgs A, const double *E) {
     program_task(A, E, t);
 }
 
+// One batch with rectangles [r0, r1): workgroup b < nrblk covers 256 cells of
+// the rectangle whose workgroups it falls in (each cell is the RESTRICT task of
+// program_task); the workgroups after those run the batch's cell tasks
+// [A.begin, A.end), in the same launch.
+__global__ __launch_bounds__(256) void k_restrict_rects(ProgArgs A, const double *E, const vx_restrict_rect *rects,
+                                                        int r0, int r1, int nrblk) {
+    const int b = (int)blockIdx.x;
+    if (b >= nrblk) {
+        const int t = A.begin + (b - nrblk) * 256 + (int)threadIdx.x;
+        if (t < A.end) program_task(A, E, t);
+        return;
+    }
+    int k = r0;
+    while (k + 1 < r1 && rects[k + 1].blk0 <= b) k++;  // uniform: a few rectangles per batch
+    const vx_restrict_rect R = rects[k];
+    const int c = (b - R.blk0) * 256 + (int)threadIdx.x;
+    if (c >= R.ni * R.nj) return;
+    const int di = R.i0 + c / R.nj, dj = R.j0 + c % R.nj;
+    const vx_patch &D = A.pt[R.dst], &S = A.pt[R.src];
+    const int rs = A.mode == VX_MODE_TRANSFER ? 1 : A.slot;
+    const double v = finer_mean(A, S, S.f[rs], di + R.oi, dj + R.oj);
+    if (A.mode == VX_MODE_TRANSFER) {
+        at(D.f[0], D.pitch, di, dj) = v;
+        at(D.f[1], D.pitch, di, dj) = v;
+    } else {
+        at(D.f[A.slot], D.pitch, di, dj) = v;
+    }
+}
+
 // A run of consecutive small batches of a program in one workgroup: batch b
 // (tasks [starts[b], starts[b + 1])) is done by the whole workgroup before
 // batch b + 1 starts, the order the batches' separate launches give.  The
@@ -462,10 +506,11 @@ int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r
 constexpr int SEQ_MAX = 4 * SEQ_THREADS;
 
 int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const double *E) {
-    const int nb = prog->batch.empty() ? 0 : (int)prog->batch.size() - 1;
+    const int nb = prog->cbatch.empty() ? 0 : (int)prog->cbatch.size() - 1;
+    auto has_rects = [&](int b) { return prog->rbatch[b + 1] > prog->rbatch[b]; };
     for (int b = 0; b < nb;) {
         int e = b;
-        while (e < nb && prog->batch[e + 1] - prog->batch[e] <= SEQ_MAX) e++;
+        while (e < nb && !has_rects(e) && prog->cbatch[e + 1] - prog->cbatch[e] <= SEQ_MAX) e++;
         if (e - b >= 2) {  // a run of small batches
             k_program_seq<<<1, SEQ_THREADS, 0, ctx->stream>>>(A, E, prog->d_batch, b, e);
             int rc = check_launch(ctx, "k_program_seq");
@@ -473,9 +518,18 @@ int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const doubl
             b = e;
             continue;
         }
-        A.begin = prog->batch[b];
-        A.end = prog->batch[b + 1];
+        A.begin = prog->cbatch[b];
+        A.end = prog->cbatch[b + 1];
         const int n = A.end - A.begin;
+        if (has_rects(b)) {  // the rectangles and the batch's cell tasks in one launch
+            const int nr = prog->rblocks[b];
+            k_restrict_rects<<<(unsigned)(nr + (n + 255) / 256), 256, 0, ctx->stream>>>(
+                A, E, prog->d_rect, prog->rbatch[b], prog->rbatch[b + 1], nr);
+            int rc = check_launch(ctx, "k_restrict_rects");
+            if (rc) return rc;
+            b++;
+            continue;
+        }
         b++;
         if (n <= 0) continue;
         k_program<<<(n + 255) / 256, 256, 0, ctx->stream>>>(A, E);
@@ -483,6 +537,94 @@ int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const doubl
         if (rc) return rc;
     }
     return VX_OK;
+}
+
+// Split the RESTRICT tasks of each batch of more than SEQ_MAX tasks into
+// rectangles (row runs of consecutive j merged over consecutive i, with the
+// same coarse-coordinate offsets) of at least 256 cells; every other task stays
+// a cell task.  The tasks of a batch touch disjoint data, so their order within
+// the batch is free.
+void compact_program(const std::vector<vx_cell_task> &host, const std::vector<int32_t> &batch,
+                     std::vector<vx_cell_task> &cells, std::vector<int32_t> &cbatch,
+                     std::vector<vx_restrict_rect> &rects, std::vector<int32_t> &rbatch,
+                     std::vector<int32_t> &rblocks) {
+    const int nb = (int)batch.size() - 1;
+    cells.clear();
+    rects.clear();
+    cbatch.assign(1, 0);
+    rbatch.assign(1, 0);
+    rblocks.clear();
+    for (int b = 0; b < nb; b++) {
+        const int lo = batch[b], hi = batch[b + 1];
+        std::vector<int> rt;  // RESTRICT tasks of a large batch
+        if (hi - lo > SEQ_MAX)
+            for (int t = lo; t < hi; t++)
+                if (host[t].op == VX_OP_RESTRICT) rt.push_back(t);
+        std::sort(rt.begin(), rt.end(), [&](int x, int y) {
+            const vx_cell_task &X = host[x], &Y = host[y];
+            if (X.dst != Y.dst) return X.dst < Y.dst;
+            if (X.src != Y.src) return X.src < Y.src;
+            if (X.di != Y.di) return X.di < Y.di;
+            return X.dj < Y.dj;
+        });
+        // row runs, then rectangles of runs with the same columns on consecutive rows
+        struct Run {
+            int dst, src, i, j0, nj, oi, oj, first;  // first: index into rt
+        };
+        std::vector<Run> runs;
+        for (size_t k = 0; k < rt.size(); k++) {
+            const vx_cell_task &T = host[rt[k]];
+            const int oi = T.si - T.di, oj = T.sj - T.dj;
+            if (!runs.empty()) {
+                Run &R = runs.back();
+                if (R.dst == T.dst && R.src == T.src && R.i == T.di && R.j0 + R.nj == T.dj && R.oi == oi &&
+                    R.oj == oj) {
+                    R.nj++;
+                    continue;
+                }
+            }
+            runs.push_back({T.dst, T.src, T.di, T.dj, 1, oi, oj, (int)k});
+        }
+        std::vector<char> in_rect(rt.size(), 0);
+        std::vector<vx_restrict_rect> br;
+        for (size_t a = 0; a < runs.size();) {
+            size_t e = a + 1;
+            while (e < runs.size() && runs[e].dst == runs[a].dst && runs[e].src == runs[a].src &&
+                   runs[e].i == runs[e - 1].i + 1 && runs[e].j0 == runs[a].j0 && runs[e].nj == runs[a].nj &&
+                   runs[e].oi == runs[a].oi && runs[e].oj == runs[a].oj)
+                e++;
+            const long ncell = (long)(e - a) * runs[a].nj;
+            if (ncell >= 256) {
+                vx_restrict_rect R{};
+                R.dst = runs[a].dst;
+                R.src = runs[a].src;
+                R.i0 = runs[a].i;
+                R.j0 = runs[a].j0;
+                R.ni = (int)(e - a);
+                R.nj = runs[a].nj;
+                R.oi = runs[a].oi;
+                R.oj = runs[a].oj;
+                br.push_back(R);
+                for (size_t q = a; q < e; q++)
+                    for (int m = 0; m < runs[q].nj; m++) in_rect[runs[q].first + m] = 1;
+            }
+            a = e;
+        }
+        std::vector<char> take(hi - lo, 1);
+        for (size_t k = 0; k < rt.size(); k++)
+            if (in_rect[k]) take[rt[k] - lo] = 0;
+        for (int t = lo; t < hi; t++)
+            if (take[t - lo]) cells.push_back(host[t]);
+        cbatch.push_back((int32_t)cells.size());
+        int blk = 0;
+        for (vx_restrict_rect &R : br) {
+            R.blk0 = blk;
+            blk += (R.ni * R.nj + 255) / 256;
+            rects.push_back(R);
+        }
+        rbatch.push_back((int32_t)rects.size());
+        rblocks.push_back(blk);
+    }
 }
 
 }  // namespace
@@ -561,15 +703,27 @@ int vx_program_create(vx_ctx *ctx, const vx_cell_task *tasks, int32_t n_tasks, c
     if (n_tasks > 0) {
         p->batch.assign(batch_start, batch_start + n_batches + 1);
         p->host.assign(tasks, tasks + n_tasks);
-        hipError_t e = hipMalloc(&p->d, sizeof(vx_cell_task) * n_tasks);
-        if (e == hipSuccess)
-            e = hipMemcpy(p->d, tasks, sizeof(vx_cell_task) * n_tasks, hipMemcpyHostToDevice);
+        std::vector<vx_cell_task> cells;
+        std::vector<vx_restrict_rect> rects;
+        compact_program(p->host, p->batch, cells, p->cbatch, rects, p->rbatch, p->rblocks);
+        hipError_t e = hipSuccess;
+        if (!cells.empty()) {
+            e = hipMalloc(&p->d, sizeof(vx_cell_task) * cells.size());
+            if (e == hipSuccess)
+                e = hipMemcpy(p->d, cells.data(), sizeof(vx_cell_task) * cells.size(), hipMemcpyHostToDevice);
+        }
         if (e == hipSuccess) e = hipMalloc(&p->d_batch, sizeof(int32_t) * (n_batches + 1));
         if (e == hipSuccess)
-            e = hipMemcpy(p->d_batch, batch_start, sizeof(int32_t) * (n_batches + 1), hipMemcpyHostToDevice);
+            e = hipMemcpy(p->d_batch, p->cbatch.data(), sizeof(int32_t) * (n_batches + 1), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !rects.empty()) {
+            e = hipMalloc(&p->d_rect, sizeof(vx_restrict_rect) * rects.size());
+            if (e == hipSuccess)
+                e = hipMemcpy(p->d_rect, rects.data(), sizeof(vx_restrict_rect) * rects.size(), hipMemcpyHostToDevice);
+        }
         if (e != hipSuccess) {
             if (p->d) (void)hipFree(p->d);
             if (p->d_batch) (void)hipFree(p->d_batch);
+            if (p->d_rect) (void)hipFree(p->d_rect);
             delete p;
             return fail(ctx, VX_ENOMEM, "program upload: %s", hipGetErrorString(e));
         }
@@ -582,6 +736,7 @@ void vx_program_destroy(vx_program *prog) {
     if (!prog) return;
     if (prog->d) (void)hipFree(prog->d);
     if (prog->d_batch) (void)hipFree(prog->d_batch);
+    if (prog->d_rect) (void)hipFree(prog->d_rect);
     delete prog;
 }
 
