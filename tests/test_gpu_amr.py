@@ -237,3 +237,27 @@ def test_per_patch_launches_equal_the_batched_launches():
     compare_fields(a, b, "per-patch vs batched")
     a.close()
     b.close()
+
+
+def test_refinement_ratio_four():
+    """Refinement ratio 4 (the task programs are compiled per ratio): the
+    initial two-level hierarchy of 7 finer patches (ghost fill from the
+    coarser level and restriction at r = 4) bitwise against the oracle.  A step
+    is refused with an error: the reference keeps 2 ghost cells per patch
+    (Rectangle.hpp:96-97), and for r = 4 its coarse-fine fluxes
+    (CalculateFluxToCoarseX/P, Rectangle.cpp:1313-1394) read finer fluxes up to
+    4 cells beyond the finer patch, and its coarse-to-fine transfer
+    (Rectangle.cpp:892-918) writes that far, outside the patch's data; the
+    device checks every task against the patch bounds first."""
+    p = two_slab_laser(r=4, levels=2)
+    say("case: two-slab laser, 2 levels, ratio 4")
+    g, o = va.Run(p, fused=False), OracleRun(p)
+    compare_mesh(g, o, "r=4 initial")
+    assert int(g.layout()[:, 0].max()) == 1 and len(g.layout()) > 2
+    dt = o.calc_dt(0.4)
+    assert g.calc_dt(0.4) == dt
+    say("step: advance 1 (refused)")
+    with pytest.raises(va.VeritasError, match="out of range"):
+        g.advance(dt, 1)
+    g.close()
+    o.close()

@@ -77,30 +77,34 @@ struct ProgArgs {
 __device__ __forceinline__ double &at(double *pl, int pitch, int i, int j) { return pl[pidx(pitch, i, j)]; }
 __device__ __forceinline__ double get(const double *pl, int pitch, int i, int j) { return pl[pidx(pitch, i, j)]; }
 
-// GetInterpolantsREF (Rectangle.cpp:121-137), element k of r
+// GetInterpolantsREF (Rectangle.cpp:121-137), element k of r.  The device
+// code of the task programs is compiled per refinement ratio R (2, 3 or 4), so
+// the per-element arrays stay in registers.
+template <int R>
 __device__ __forceinline__ void interp_ref(const ProgArgs &A, double f1, double f2, double f3, double f4,
                                            double f5, double *out) {
     f5 -= f3; f4 -= f3; f2 -= f3; f1 -= f3;
     double a1 = kInterpA[0] * f1 + kInterpA[1] * f2 + kInterpA[2] * f4 + kInterpA[3] * f5;
     double a2 = kInterpA[4] * f1 + kInterpA[5] * f2 + kInterpA[6] * f4 + kInterpA[7] * f5;
     double a3 = kInterpA[8] * f1 + kInterpA[9] * f2 + kInterpA[10] * f4 + kInterpA[11] * f5;
-    for (int i = 0; i < A.r; i++)
+    for (int i = 0; i < R; i++)
         out[i] = A.ref[3 * i] * a1 + A.ref[3 * i + 1] * a2 + A.ref[3 * i + 2] * a3 + f3;
 }
 
 // GetWenoValueFromCoarseLevel (Rectangle.cpp:343-415): element e = k * r + l
 // of the corrected r x r interpolants of the coarse cell holding (i, j).
+template <int R>
 __device__ double coarse_element(const ProgArgs &A, const vx_patch &C, const double *g, int i, int j, int e) {
-    const int r = A.r, pitch = C.pitch;
+    const int r = R, pitch = C.pitch;
     const int ic = i / r - C.x_pos, jc = j / r - C.p_pos;
-    double temps[5][MAXR];
+    double temps[5][R];
     for (int k = -2; k < 3; k++)
-        interp_ref(A, get(g, pitch, ic - 2, jc + k), get(g, pitch, ic - 1, jc + k), get(g, pitch, ic, jc + k),
+        interp_ref<R>(A, get(g, pitch, ic - 2, jc + k), get(g, pitch, ic - 1, jc + k), get(g, pitch, ic, jc + k),
                    get(g, pitch, ic + 1, jc + k), get(g, pitch, ic + 2, jc + k), temps[k + 2]);
     double sum(0.0), want = 0.0;
     for (int k = 0; k < r; k++) {
-        double part[MAXR];
-        interp_ref(A, temps[0][k], temps[1][k], temps[2][k], temps[3][k], temps[4][k], part);
+        double part[R];
+        interp_ref<R>(A, temps[0][k], temps[1][k], temps[2][k], temps[3][k], temps[4][k], part);
         for (int l = 0; l < r; l++) {
             if (k * r + l == e) want = part[l];
             sum += part[l];
@@ -111,8 +115,9 @@ __device__ double coarse_element(const ProgArgs &A, const vx_patch &C, const dou
 }
 
 // GetValueFromFinerLevel (Rectangle.cpp:314-327)
+template <int R>
 __device__ double finer_mean(const ProgArgs &A, const vx_patch &F, const double *g, int i, int j) {
-    const int r = A.r;
+    const int r = R;
     const int i0 = i * r - F.x_pos, j0 = j * r - F.p_pos;
     double temp(0.0);
     for (int k = 0; k < r; k++)
@@ -125,6 +130,7 @@ __device__ double finer_mean(const ProgArgs &A, const vx_patch &F, const double 
 __device__ __forceinline__ double mn(double a, double b) { return stdmin(a, b); }
 
 // one limiter-C task (Rectangle.cpp:1625-1706, 1794-1830, 1879-1915)
+template <int R>
 __device__ void c_task(const ProgArgs &A, const vx_cell_task &T) {
     const vx_patch &N = A.pt[T.dst];  // neighbour (same level) or coarse patch
     const vx_patch &Q = A.pt[T.src];  // caller
@@ -153,7 +159,7 @@ __device__ void c_task(const ProgArgs &A, const vx_cell_task &T) {
         at(Nc, np_, di, dj) = c;
         at(Qc, qp, si, sj) = c;
     } else {  // VX_OP_C_DIFF: N is the coarse patch, Q the finer caller
-        const int r = A.r;
+        const int r = R;
         if (t < 2) {
             double c = get(N.Cx, np_, di, dj);
             for (int k = 0; k < r; k++) {
@@ -213,10 +219,11 @@ __device__ double lo_flux_p(const ProgArgs &A, const vx_patch &F, const double *
 
 // coarse-fine flux task (Rectangle.cpp:1313-1394 else-branches, then the RK
 // combination of Rectangle.cpp:1396-1517 at that face)
+template <int R>
 __device__ void flux_task(const ProgArgs &A, const double *E, const vx_cell_task &T) {
     const vx_patch &K = A.pt[T.dst];
     const vx_patch &F = A.pt[T.src];
-    const int r = A.r, kp = K.pitch, fp = F.pitch;
+    const int r = R, kp = K.pitch, fp = F.pitch;
     const bool xdir = T.op == VX_OP_FLUX_X;
     double *const *KH = xdir ? K.FxH : K.FpH;
     double *const *FH = xdir ? F.FxH : F.FpH;
@@ -251,14 +258,15 @@ __device__ void flux_task(const ProgArgs &A, const double *E, const vx_cell_task
     }
 }
 
+template <int R>
 __device__ __forceinline__ void program_task(const ProgArgs &A, const double *E, const int t) {
     const vx_cell_task T = A.task[t];
     if (T.op >= VX_OP_C_SAME && T.op <= VX_OP_C_UPDATE) {
-        c_task(A, T);
+        c_task<R>(A, T);
         return;
     }
     if (T.op == VX_OP_FLUX_X || T.op == VX_OP_FLUX_P) {
-        flux_task(A, E, T);
+        flux_task<R>(A, E, T);
         return;
     }
     if (T.op == VX_OP_CORRECT) {
@@ -274,10 +282,10 @@ __device__ __forceinline__ void program_task(const ProgArgs &A, const double *E,
         v = get(S.f[rs], S.pitch, T.si, T.sj);
     } else if (T.op == VX_OP_COARSE) {
         const vx_patch &S = A.pt[T.src];
-        v = coarse_element(A, S, S.f[rs], T.si, T.sj, T.aux);
+        v = coarse_element<R>(A, S, S.f[rs], T.si, T.sj, T.aux);
     } else if (T.op == VX_OP_RESTRICT) {
         const vx_patch &S = A.pt[T.src];
-        v = finer_mean(A, S, S.f[rs], T.si, T.sj);
+        v = finer_mean<R>(A, S, S.f[rs], T.si, T.sj);
     }
     if (A.mode == VX_MODE_TRANSFER) {
         at(D.f[0], D.pitch, T.di, T.dj) = v;
@@ -288,33 +296,35 @@ __device__ __forceinline__ void program_task(const ProgArgs &A, const double *E,
 }
 
 // One batch of a program: one thread per task.
+template <int R>
 __global__ __launch_bounds__(256) void k_program(ProgArgs A, const double *E) {
     const int t = A.begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= A.end) return;
-    program_task(A, E, t);
+    program_task<R>(A, E, t);
 }
 
 // One batch with rectangles [r0, r1): workgroup b < nrblk covers 256 cells of
 // the rectangle whose workgroups it falls in (each cell is the RESTRICT task of
 // program_task); the workgroups after those run the batch's cell tasks
 // [A.begin, A.end), in the same launch.
+template <int R>
 __global__ __launch_bounds__(256) void k_restrict_rects(ProgArgs A, const double *E, const vx_restrict_rect *rects,
                                                         int r0, int r1, int nrblk) {
     const int b = (int)blockIdx.x;
     if (b >= nrblk) {
         const int t = A.begin + (b - nrblk) * 256 + (int)threadIdx.x;
-        if (t < A.end) program_task(A, E, t);
+        if (t < A.end) program_task<R>(A, E, t);
         return;
     }
     int k = r0;
     while (k + 1 < r1 && rects[k + 1].blk0 <= b) k++;  // uniform: a few rectangles per batch
-    const vx_restrict_rect R = rects[k];
-    const int c = (b - R.blk0) * 256 + (int)threadIdx.x;
-    if (c >= R.ni * R.nj) return;
-    const int di = R.i0 + c / R.nj, dj = R.j0 + c % R.nj;
-    const vx_patch &D = A.pt[R.dst], &S = A.pt[R.src];
+    const vx_restrict_rect Q = rects[k];
+    const int c = (b - Q.blk0) * 256 + (int)threadIdx.x;
+    if (c >= Q.ni * Q.nj) return;
+    const int di = Q.i0 + c / Q.nj, dj = Q.j0 + c % Q.nj;
+    const vx_patch &D = A.pt[Q.dst], &S = A.pt[Q.src];
     const int rs = A.mode == VX_MODE_TRANSFER ? 1 : A.slot;
-    const double v = finer_mean(A, S, S.f[rs], di + R.oi, dj + R.oj);
+    const double v = finer_mean<R>(A, S, S.f[rs], di + Q.oi, dj + Q.oj);
     if (A.mode == VX_MODE_TRANSFER) {
         at(D.f[0], D.pitch, di, dj) = v;
         at(D.f[1], D.pitch, di, dj) = v;
@@ -329,11 +339,12 @@ __global__ __launch_bounds__(256) void k_restrict_rects(ProgArgs A, const double
 // workgroup fence of the barrier makes each batch's writes visible to the
 // next batch's reads in this workgroup (one compute unit).
 constexpr int SEQ_THREADS = 1024;
+template <int R>
 __global__ __launch_bounds__(SEQ_THREADS) void k_program_seq(ProgArgs A, const double *E, const int *starts,
                                                              int b0, int b1) {
     for (int b = b0; b < b1; b++) {
         const int lo = starts[b], hi = starts[b + 1];
-        for (int t = lo + (int)threadIdx.x; t < hi; t += SEQ_THREADS) program_task(A, E, t);
+        for (int t = lo + (int)threadIdx.x; t < hi; t += SEQ_THREADS) program_task<R>(A, E, t);
         __syncthreads();
     }
 }
@@ -392,6 +403,7 @@ int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r
 
 // One thread per cell of a phase of regrid-transfer rectangles: range k covers
 // cells [start[k], start[k+1]) of the launch (binary search on start).
+template <int R>
 __global__ __launch_bounds__(256) void k_ranges(ProgArgs A, const vx_range_task *rg, const long *start, int n) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= start[n]) return;
@@ -401,24 +413,24 @@ __global__ __launch_bounds__(256) void k_ranges(ProgArgs A, const vx_range_task 
         if (start[mid] <= t) lo = mid;
         else hi = mid - 1;
     }
-    const vx_range_task R = rg[lo];
+    const vx_range_task G = rg[lo];
     const long local = t - start[lo];
-    const vx_patch &D = A.pt[R.dst], &S = A.pt[R.src];
+    const vx_patch &D = A.pt[G.dst], &S = A.pt[G.src];
     double v;
     int di, dj;
-    if (R.op == VX_OP_COPY) {
-        const int a = (int)(local / R.nj), b = (int)(local % R.nj);
-        di = R.di0 + a;
-        dj = R.dj0 + b;
-        v = get(S.f[1], S.pitch, R.si0 + a, R.sj0 + b);
+    if (G.op == VX_OP_COPY) {
+        const int a = (int)(local / G.nj), b = (int)(local % G.nj);
+        di = G.di0 + a;
+        dj = G.dj0 + b;
+        v = get(S.f[1], S.pitch, G.si0 + a, G.sj0 + b);
     } else {  // VX_OP_COARSE
-        const int rr = A.r * A.r;
+        const int rr = R * R;
         const int cell = (int)(local / rr), e = (int)(local % rr);
-        const int k = e / A.r, l = e % A.r;
-        const int ci = R.si0 + cell / R.nj, cj = R.sj0 + cell % R.nj;
-        di = (ci - R.di0) * A.r + k;
-        dj = (cj - R.dj0) * A.r + l;
-        v = coarse_element(A, S, S.f[1], A.r * ci + 1, A.r * cj + 1, l + A.r * k);
+        const int k = e / R, l = e % R;
+        const int ci = G.si0 + cell / G.nj, cj = G.sj0 + cell % G.nj;
+        di = (ci - G.di0) * R + k;
+        dj = (cj - G.dj0) * R + l;
+        v = coarse_element<R>(A, S, S.f[1], R * ci + 1, R * cj + 1, l + R * k);
     }
     at(D.f[0], D.pitch, di, dj) = v;
     at(D.f[1], D.pitch, di, dj) = v;
@@ -505,14 +517,15 @@ int check_program(vx_ctx *ctx, const vx_patch_table *tb, vx_program *prog, int r
 // a launch each.  Larger batches keep a launch of their own over the device.
 constexpr int SEQ_MAX = 4 * SEQ_THREADS;
 
-int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const double *E) {
+template <int R>
+int launch_batches_r(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const double *E) {
     const int nb = prog->cbatch.empty() ? 0 : (int)prog->cbatch.size() - 1;
     auto has_rects = [&](int b) { return prog->rbatch[b + 1] > prog->rbatch[b]; };
     for (int b = 0; b < nb;) {
         int e = b;
         while (e < nb && !has_rects(e) && prog->cbatch[e + 1] - prog->cbatch[e] <= SEQ_MAX) e++;
         if (e - b >= 2) {  // a run of small batches
-            k_program_seq<<<1, SEQ_THREADS, 0, ctx->stream>>>(A, E, prog->d_batch, b, e);
+            k_program_seq<R><<<1, SEQ_THREADS, 0, ctx->stream>>>(A, E, prog->d_batch, b, e);
             int rc = check_launch(ctx, "k_program_seq");
             if (rc) return rc;
             b = e;
@@ -523,7 +536,7 @@ int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const doubl
         const int n = A.end - A.begin;
         if (has_rects(b)) {  // the rectangles and the batch's cell tasks in one launch
             const int nr = prog->rblocks[b];
-            k_restrict_rects<<<(unsigned)(nr + (n + 255) / 256), 256, 0, ctx->stream>>>(
+            k_restrict_rects<R><<<(unsigned)(nr + (n + 255) / 256), 256, 0, ctx->stream>>>(
                 A, E, prog->d_rect, prog->rbatch[b], prog->rbatch[b + 1], nr);
             int rc = check_launch(ctx, "k_restrict_rects");
             if (rc) return rc;
@@ -532,11 +545,18 @@ int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const doubl
         }
         b++;
         if (n <= 0) continue;
-        k_program<<<(n + 255) / 256, 256, 0, ctx->stream>>>(A, E);
+        k_program<R><<<(n + 255) / 256, 256, 0, ctx->stream>>>(A, E);
         int rc = check_launch(ctx, "k_program");
         if (rc) return rc;
     }
     return VX_OK;
+}
+
+int launch_batches(vx_ctx *ctx, const vx_program *prog, ProgArgs &A, const double *E) {
+    if (A.r == 2) return launch_batches_r<2>(ctx, prog, A, E);
+    if (A.r == 3) return launch_batches_r<3>(ctx, prog, A, E);
+    static_assert(MAXR == 4, "one launch_batches_r per admitted ratio");
+    return launch_batches_r<4>(ctx, prog, A, E);  // fill_common admits 2..MAXR
 }
 
 // Split the RESTRICT tasks of each batch of more than SEQ_MAX tasks into
@@ -796,8 +816,12 @@ int vx_transfer_ranges(vx_ctx *ctx, const vx_patch_table *table, const vx_range_
     std::memcpy(h.data() + rb, start.data(), sb);
     rc = check_hip(ctx, hipMemcpyAsync(d, h.data(), rb + sb, hipMemcpyHostToDevice, ctx->stream), "transfer ranges");
     if (!rc && start[n] > 0) {
-        k_ranges<<<(unsigned)((start[n] + 255) / 256), 256, 0, ctx->stream>>>(A, (const vx_range_task *)d,
-                                                                               (const long *)(d + rb), n);
+        const unsigned g = (unsigned)((start[n] + 255) / 256);
+        const vx_range_task *rg = (const vx_range_task *)d;
+        const long *st = (const long *)(d + rb);
+        if (r == 2) k_ranges<2><<<g, 256, 0, ctx->stream>>>(A, rg, st, n);
+        else if (r == 3) k_ranges<3><<<g, 256, 0, ctx->stream>>>(A, rg, st, n);
+        else k_ranges<4><<<g, 256, 0, ctx->stream>>>(A, rg, st, n);
         rc = check_launch(ctx, "k_ranges");
     }
     const int rs = check_hip(ctx, hipStreamSynchronize(ctx->stream), "transfer ranges");
