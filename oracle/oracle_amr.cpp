@@ -6,6 +6,15 @@
 // (Rectangle.cpp, Rectangle.hpp, Level.cpp).  Loop ranges, operation order and
 // association follow the reference literally; the domain-edge sentinel
 // (BoundaryCondition) is a null neighbour whose same-level value is 0.
+//
+// Pinned by tests/: restriction = mean of the finer cells, coarse faces on a
+// finer edge carry the mean of the finer fluxes, conservation across levels
+// (test_oracle_amr.py); clustering by hand-derived boxes (test_clustering.py);
+// the 2-D coarse-to-fine ghost fill, corner blocks, same-level copies and the
+// regrid transfer by exact fine-cell averages of bilinear data and by
+// per-coarse-cell conservation (test_oracle_amr_fill.py).  Parity unpinned:
+// the neighbour tables of arrangements those cases do not reach, and the
+// limiter-C synchronisation beyond its one-C-per-face invariant.
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>

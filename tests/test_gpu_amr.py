@@ -261,3 +261,39 @@ def test_refinement_ratio_four():
         g.advance(dt, 1)
     g.close()
     o.close()
+
+
+@pytest.mark.parametrize("data", ["bilinear", "gaussian"])
+def test_fills_of_two_touching_patches(data):
+    """The case of the oracle's fill pins (test_oracle_amr_fill.py): two
+    touching finer patches placed by RefinementOverride, whose ghost cells come
+    from the coarser level (edges and corner blocks) and from each other, then
+    a regrid to a shifted region (the new patches filled from the old ones and
+    from the coarser level).  On the device bitwise against the oracle; with
+    bilinear data every finer cell and ghost equals the exact fine-cell
+    average, with Gaussian data every interpolated block averages to its coarse
+    cell."""
+    import test_oracle_amr_fill as F
+    f = getattr(F, data)
+    say(f"case: two touching finer patches, {data} data")
+    g, gflag = F.make_run(f, F.L_SHAPE, factory=lambda d, h: va.Run.from_desc(d, h, fused=False))
+    o, oflag = F.make_run(f, F.L_SHAPE)
+    compare_mesh(g, o, f"{data} initial")
+    assert len(F.fine_patches(g)) == 2
+    old = F.boxes_of(g)
+    if data == "bilinear":
+        F.check_bilinear(g, "device, initial")
+    else:
+        F.check_block_means(g, "device, initial")
+    say("step: regrid to the shifted region")
+    gflag["boxes"] = oflag["boxes"] = F.SHIFTED
+    g.regrid()
+    o.regrid()
+    assert F.boxes_of(g) != old
+    compare_mesh(g, o, f"{data} after the regrid")
+    if data == "bilinear":
+        F.check_bilinear(g, "device, after the regrid")
+    else:
+        F.check_block_means(g, "device, after the regrid", interior_from=old)
+    g.close()
+    o.close()

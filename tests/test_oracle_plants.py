@@ -2,7 +2,8 @@
 
 Each case copies the oracle's sources, plants one plausible slip in the copy
 (a wrong stage index in the Runge-Kutta flux combination, a flipped 1/48 term,
-a wrong speed or force factor, a wrong tableau weight or stage time), builds
+a wrong speed or force factor, a wrong tableau weight or stage time, a
+transposed or mis-layered coarse-to-fine fill), builds
 it, and runs the pins named for it against the planted build: at least one of
 them must fail.  The oracle in oracle/ is not touched."""
 import os
@@ -14,6 +15,7 @@ import pytest
 
 import _oracle
 import test_clustering as K
+import test_oracle_amr_fill as F
 import test_oracle_vlasov as V
 
 ORACLE = os.path.join(_oracle.ROOT, "oracle")
@@ -63,6 +65,37 @@ PLANTS = {
                      [("cutPosition = std::floor((lengthx / 2) + rectangle.first.first);",
                        "cutPosition = std::ceil((lengthx / 2) + rectangle.first.first);")],
                      [K.check_oracle_cases]),
+    # Rectangle.cpp:343-415: the r x r interpolants stored transposed
+    "fill_transpose": ("oracle_amr.cpp",
+                       [("interpolants[k * r + l] = part[l];", "interpolants[l * r + k] = part[l];")],
+                       [F.test_ghost_fill_is_exact_for_bilinear_data]),
+    # Rectangle.cpp:389-395: the ghost layer next to a lower p edge given the
+    # sub-cell row of the layer behind it
+    "fill_layer": ("oracle_amr.cpp",
+                   [("red[2 * k] = interpolants[r * k + r - 1];", "red[2 * k] = interpolants[r * k + r - 2];")],
+                   [F.test_ghost_fill_is_exact_for_bilinear_data]),
+    # Rectangle.cpp:385-387: the mean correction taken from the next coarse cell
+    "fill_correction_cell": ("oracle_amr.cpp",
+                             [("double correction = g[NS(i_coarse, j_coarse)] - 1.0",
+                               "double correction = g[NS(i_coarse, j_coarse + 1)] - 1.0")],
+                             [F.test_interpolated_blocks_average_to_their_coarse_cell]),
+    # Rectangle.cpp:76-89: one constant of the interpolation table's linear row
+    # 1 % off (its quadratic row has no effect at r = 2: the sub-cell weights
+    # of that term are zero on both halves)
+    "fill_table": ("oracle.hpp",
+                   [("0.104166666666667, -0.708333333333334,", "0.105208333333334, -0.708333333333334,")],
+                   [F.test_ghost_fill_is_exact_for_bilinear_data]),
+    # Rectangle.cpp:904-912: the transfer's sub-cells transposed
+    "transfer_transpose": ("oracle_amr.cpp",
+                           [("rect->f[0][rect->NS(fi, fj)] = in[l + r * k];",
+                             "rect->f[0][rect->NS(fi, fj)] = in[k + r * l];")],
+                           [F.test_regrid_transfer_is_exact_for_bilinear_data]),
+    # Rectangle.cpp:476-517: a same-level corner block's outer ghost column
+    # copied from the inner one
+    "corner_same_level": ("oracle_amr.cpp",
+                          [("g[P.NS(-2, i * r + j)] = same(s, P.x_pos - 2, P.p_pos + i * r + j);",
+                            "g[P.NS(-2, i * r + j)] = same(s, P.x_pos - 1, P.p_pos + i * r + j);")],
+                          [F.test_ghost_fill_is_exact_for_bilinear_data]),
     # Settings.cpp:166-179: the third stage time 0.333 instead of 0.332
     "stage_clock": ("oracle.cpp",
                     [("time += (0.332 - 0.5) * dt;", "time += (0.333 - 0.5) * dt;"),
